@@ -1,0 +1,353 @@
+"""ctypes binding of libcpm_mi355x.so (include/cpm_mi355x.h) and the GPU integrator.
+
+The integrator mirrors CorePotts' ScientificPottsIntegrator surface
+(lib/CorePotts/src/algorithms/sequential.jl:216-234, 598-632): ``step`` enqueues MCS
+without synchronizing, ``current_mcs_report`` costs one launch and one synchronization,
+``logical_state`` downloads the state. There is no CPU fallback: a missing or unloadable
+library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import NamedTuple, Optional
+
+import numpy as np
+
+from .model import CompiledScientificState, StaticCartesianRelation
+from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ModelSpec, build_spec, state_arrays
+
+MAX_DIRS = 128
+MAX_TYPES = 16
+MAX_MEDIA = 16
+ABI_VERSION = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcpm_mi355x.so")
+
+
+class CpmRelation(C.Structure):
+    _fields_ = [("count", C.c_int32), ("offsets", (C.c_int32 * 3) * MAX_DIRS),
+                ("weights", C.c_double * MAX_DIRS)]
+
+
+class CpmModelDesc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("algorithm", C.c_int32), ("ndim", C.c_int32),
+        ("dims", C.c_int32 * 3), ("face_kind", (C.c_int32 * 2) * 3),
+        ("face_medium", (C.c_uint32 * 2) * 3), ("mutable_mask", C.c_void_p),
+        ("immutable_tags", C.c_void_p), ("immutable_ids", C.c_void_p),
+        ("proposal", CpmRelation), ("contact", CpmRelation), ("surface", CpmRelation),
+        ("n_energies", C.c_int32), ("energy_order", C.c_int32 * 3),
+        ("volume_f64", C.c_int32), ("contact_f64", C.c_int32), ("boundary_f64", C.c_int32),
+        ("n_types", C.c_int32), ("J", C.c_double * (MAX_TYPES * MAX_TYPES)),
+        ("n_table", C.c_int32), ("table_owner", C.c_uint32 * MAX_MEDIA),
+        ("table_type", C.c_uint32 * MAX_MEDIA), ("n_media", C.c_int32),
+        ("medium_ids", C.c_uint32 * MAX_MEDIA), ("temperature", C.c_float),
+        ("capacity", C.c_uint32), ("seed", C.c_uint64),
+        ("retire_target_volume", C.c_double), ("retire_volume_strength", C.c_double),
+        ("retire_target_boundary", C.c_int64), ("retire_boundary_strength", C.c_double),
+    ]
+
+
+EXPORTS = (
+    "cpm_last_error", "cpm_abi_version", "cpm_create", "cpm_destroy", "cpm_upload_state",
+    "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
+    "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_set_graph_chunk",
+    "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
+    "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
+    "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_step", "cpm_batch_progress",
+    "cpm_batch_capacity", "cpm_batch_report", "cpm_batch_sync", "cpm_batch_set_graph_chunk",
+    "cpm_batch_bytes",
+)
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the engine; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libcpm_mi355x.so is not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    lib.cpm_last_error.restype = C.c_char_p
+    lib.cpm_last_error.argtypes = [P]
+    lib.cpm_batch_last_error.restype = C.c_char_p
+    lib.cpm_batch_last_error.argtypes = [P]
+    lib.cpm_abi_version.restype = C.c_int32
+    lib.cpm_create.argtypes = [C.POINTER(CpmModelDesc), C.POINTER(P)]
+    lib.cpm_destroy.argtypes = [P]
+    state_args = [P] * 11
+    lib.cpm_upload_state.argtypes = [P] + state_args
+    lib.cpm_download_state.argtypes = [P] + state_args
+    lib.cpm_set_seed.argtypes = [P, C.c_uint64]
+    lib.cpm_set_mcs.argtypes = [P, C.c_uint64]
+    lib.cpm_get_mcs.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.cpm_step.argtypes = [P, C.c_uint64]
+    lib.cpm_report.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.cpm_sync.argtypes = [P]
+    lib.cpm_schedule.argtypes = [P, C.POINTER(C.c_int32), P]
+    lib.cpm_site_colors.argtypes = [P, P]
+    lib.cpm_metrics.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.cpm_set_graph_chunk.argtypes = [P, C.c_uint32]
+    lib.cpm_ensemble_seed.restype = C.c_uint64
+    lib.cpm_ensemble_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+    lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
+    lib.cpm_batch_destroy.argtypes = [P]
+    lib.cpm_replica_bytes.argtypes = [C.POINTER(CpmModelDesc), C.POINTER(C.c_uint64)]
+    lib.cpm_batch_add.argtypes = [P, C.POINTER(CpmModelDesc), C.POINTER(C.c_int32)]
+    lib.cpm_batch_remove.argtypes = [P, C.c_int32]
+    lib.cpm_batch_upload_state.argtypes = [P, C.c_int32] + state_args
+    lib.cpm_batch_download_state.argtypes = [P, C.c_int32] + state_args
+    lib.cpm_batch_set_target.argtypes = [P, C.c_int32, C.c_uint64]
+    lib.cpm_batch_step.argtypes = [P, C.c_uint64]
+    lib.cpm_batch_progress.argtypes = [P, P, C.c_int32]
+    lib.cpm_batch_capacity.argtypes = [P]
+    lib.cpm_batch_report.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64)]
+    lib.cpm_batch_sync.argtypes = [P]
+    lib.cpm_batch_set_graph_chunk.argtypes = [P, C.c_uint32]
+    lib.cpm_batch_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    for name in EXPORTS:
+        getattr(lib, name)
+    if lib.cpm_abi_version() != ABI_VERSION:
+        raise RuntimeError("libcpm_mi355x.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"cpm error {code}: {message}")
+        self.code = code
+
+
+def _fill_relation(dst: CpmRelation, rel: Optional[StaticCartesianRelation], ndim: int):
+    if rel is None:
+        dst.count = 0
+        return
+    dst.count = len(rel.offsets)
+    for k, (o, w) in enumerate(zip(rel.offsets, rel.weights)):
+        for a in range(3):
+            dst.offsets[k][a] = o[a] if a < ndim else 0
+        dst.weights[k] = w
+
+
+def make_desc(spec: ModelSpec, keep: list) -> CpmModelDesc:
+    """Fill cpm_model_desc; arrays referenced by pointer are appended to `keep`."""
+    d = CpmModelDesc()
+    d.abi_version = ABI_VERSION
+    d.algorithm = spec.algorithm
+    d.ndim = spec.ndim
+    for a in range(3):
+        d.dims[a] = spec.dims[a]
+        for f in range(2):
+            d.face_kind[a][f] = spec.face_kind[a][f]
+            d.face_medium[a][f] = spec.face_medium[a][f]
+    if spec.mutable_mask is not None:
+        arrs = [np.ascontiguousarray(spec.mutable_mask, dtype=np.uint8),
+                np.ascontiguousarray(spec.immutable_tags, dtype=np.uint8),
+                np.ascontiguousarray(spec.immutable_ids, dtype=np.uint32)]
+        keep.extend(arrs)
+        d.mutable_mask = arrs[0].ctypes.data
+        d.immutable_tags = arrs[1].ctypes.data
+        d.immutable_ids = arrs[2].ctypes.data
+    _fill_relation(d.proposal, spec.proposal, spec.ndim)
+    _fill_relation(d.contact, spec.contact, spec.ndim)
+    _fill_relation(d.surface, spec.surface, spec.ndim)
+    d.n_energies = len(spec.energy_order)
+    for t, term in enumerate(spec.energy_order):
+        d.energy_order[t] = term
+    d.volume_f64 = int(spec.volume_f64)
+    d.contact_f64 = int(spec.contact_f64)
+    d.boundary_f64 = int(spec.boundary_f64)
+    nt = spec.n_types
+    d.n_types = nt
+    for i in range(nt):
+        for j in range(nt):
+            d.J[i * nt + j] = float(spec.J[i, j])
+    d.n_table = len(spec.table_owner)
+    for i, (o, t) in enumerate(zip(spec.table_owner, spec.table_type)):
+        d.table_owner[i] = o
+        d.table_type[i] = t
+    d.n_media = len(spec.medium_ids)
+    for i, m in enumerate(spec.medium_ids):
+        d.medium_ids[i] = m
+    d.temperature = float(spec.temperature)
+    d.capacity = spec.capacity
+    d.seed = spec.seed
+    d.retire_target_volume, d.retire_volume_strength = spec.retire[0], spec.retire[1]
+    d.retire_target_boundary, d.retire_boundary_strength = spec.retire[2], spec.retire[3]
+    return d
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class ScientificMCSReport(NamedTuple):
+    """Accounting of one completed MCS (sequential.jl:236-251)."""
+    mcs: int
+    internal_rounds: int
+    scheduler_candidates: int
+    activated_attempts: int
+    realized_proposals: int
+    same_owner_no_ops: int
+    boundary_no_ops: int
+    immutable_recipient_no_ops: int
+    dynamic_conflicts: int
+    constraint_rejections: int
+    acceptance_rejections: int
+    accepted_copies: int
+
+
+class ScientificPottsIntegrator:
+    """Device-resident integrator over one lattice, backed by a cpm_ctx."""
+
+    def __init__(self, compiled: CompiledScientificState, spec: ModelSpec):
+        if spec.algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY):
+            raise NotImplementedError("the MI355X engine runs CheckerboardSweepCPM and LotteryCPM")
+        self.lib = load_library()
+        self.spec = spec
+        self.compiled = compiled
+        keep: list = []
+        desc = make_desc(spec, keep)
+        handle = C.c_void_p()
+        rc = self.lib.cpm_create(C.byref(desc), C.byref(handle))
+        if rc != 0:
+            raise EngineError(rc, self.lib.cpm_last_error(None).decode())
+        self._ctx = handle
+        self.upload(compiled)
+        self.seed = spec.seed
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self.lib.cpm_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise EngineError(rc, self.lib.cpm_last_error(self._ctx).decode())
+
+    # -- state -------------------------------------------------------------
+    def upload(self, compiled: CompiledScientificState):
+        a = state_arrays(compiled, self.spec)
+        self._check(self.lib.cpm_upload_state(
+            self._ctx, _ptr(a["tags"]), _ptr(a["ids"]), _ptr(a["finite_volumes"]),
+            _ptr(a["medium_volumes"]), _ptr(a["boundary_measures"]), _ptr(a["active"]),
+            _ptr(a["cell_types"]), _ptr(a["target_volume"]), _ptr(a["volume_strength"]),
+            _ptr(a["target_boundary"]), _ptr(a["boundary_strength"])))
+
+    def download(self) -> dict:
+        n = int(np.prod(self.spec.dims))
+        cap = self.spec.capacity
+        out = dict(
+            tags=np.zeros(n, np.uint8), ids=np.zeros(n, np.uint32),
+            finite_volumes=np.zeros(cap, np.int32),
+            medium_volumes=np.zeros(max(len(self.spec.medium_ids), 1), np.int32),
+            boundary_measures=np.zeros(cap, np.int64), active=np.zeros(cap, np.uint8),
+            cell_types=np.zeros(cap, np.uint32), target_volume=np.zeros(cap, np.float64),
+            volume_strength=np.zeros(cap, np.float64), target_boundary=np.zeros(cap, np.int64),
+            boundary_strength=np.zeros(cap, np.float64))
+        self._check(self.lib.cpm_download_state(
+            self._ctx, _ptr(out["tags"]), _ptr(out["ids"]), _ptr(out["finite_volumes"]),
+            _ptr(out["medium_volumes"]), _ptr(out["boundary_measures"]), _ptr(out["active"]),
+            _ptr(out["cell_types"]), _ptr(out["target_volume"]), _ptr(out["volume_strength"]),
+            _ptr(out["target_boundary"]), _ptr(out["boundary_strength"])))
+        out["medium_volumes"] = out["medium_volumes"][:len(self.spec.medium_ids)]
+        return out
+
+    def logical_state(self) -> CompiledScientificState:
+        """Snapshot of the device state as a compiled host state (logical_state)."""
+        s = self.download()
+        snap = self.compiled.copy()
+        snap.tags, snap.ids = s["tags"], s["ids"]
+        snap.finite_volumes = s["finite_volumes"]
+        snap.medium_volumes = s["medium_volumes"]
+        snap.boundary_measures = s["boundary_measures"]
+        snap.active = s["active"]
+        snap.cell_types = s["cell_types"]
+        for name in ("target_volume", "volume_strength", "target_boundary", "boundary_strength"):
+            if name in snap.properties:
+                snap.properties[name] = s[name].astype(snap.properties[name].dtype)
+        return snap
+
+    # -- stepping ----------------------------------------------------------
+    @property
+    def mcs(self) -> int:
+        v = C.c_uint64()
+        self._check(self.lib.cpm_get_mcs(self._ctx, C.byref(v)))
+        return int(v.value)
+
+    @mcs.setter
+    def mcs(self, value: int):
+        self._check(self.lib.cpm_set_mcs(self._ctx, int(value)))
+
+    def set_seed(self, seed: int):
+        self._check(self.lib.cpm_set_seed(self._ctx, int(seed)))
+        self.seed = int(seed)
+
+    def step(self, n: int = 1):
+        if n < 0:
+            raise ValueError("scientific step count must be non-negative")
+        self._check(self.lib.cpm_step(self._ctx, int(n)))
+        return self
+
+    def synchronize(self):
+        self._check(self.lib.cpm_sync(self._ctx))
+
+    def current_mcs_report(self) -> Optional[ScientificMCSReport]:
+        if self.mcs == 0:
+            return None
+        out = (C.c_uint64 * 12)()
+        self._check(self.lib.cpm_report(self._ctx, out))
+        return ScientificMCSReport(*[int(v) for v in out])
+
+    def schedule(self):
+        count = C.c_int32()
+        self._check(self.lib.cpm_schedule(self._ctx, C.byref(count), None))
+        order = np.zeros(max(count.value, 1), dtype=np.uint32)
+        self._check(self.lib.cpm_schedule(self._ctx, C.byref(count), order.ctypes.data))
+        return count.value, order[:count.value]
+
+    def site_colors(self) -> np.ndarray:
+        out = np.zeros(int(np.prod(self.spec.dims)), dtype=np.uint8)
+        self._check(self.lib.cpm_site_colors(self._ctx, out.ctypes.data))
+        return out
+
+    def metrics(self) -> dict:
+        out = (C.c_uint64 * 6)()
+        self._check(self.lib.cpm_metrics(self._ctx, out))
+        keys = ("launches", "host_synchronizations", "host_to_device_transfers",
+                "device_to_host_transfers", "device_bytes", "graph_launches")
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def set_graph_chunk(self, chunk: int):
+        self._check(self.lib.cpm_set_graph_chunk(self._ctx, int(chunk)))
+
+
+def init_scientific(state: CompiledScientificState, proposal_relation, components, algorithm,
+                    seed: int = 0) -> ScientificPottsIntegrator:
+    """init_scientific(state, proposal_relation, components, algorithm; seed)
+    (checkerboard.jl:197-219, lottery.jl:94-115) on the MI355X engine."""
+    spec = build_spec(state, proposal_relation, components, algorithm, seed)
+    return ScientificPottsIntegrator(state, spec)
+
+
+def ensemble_seed(master_seed: int, trajectory: int, rerun: int = 1) -> int:
+    """ensemble_seed(EnsembleSeedDerivationV1(), master, trajectory, rerun)
+    (sciml/interface.jl:1009-1026)."""
+    if not 1 <= trajectory <= 0xFFFFFFFF:
+        raise ValueError("ensemble trajectory identity must lie in 1:typemax(UInt32)")
+    if not 1 <= rerun <= 256:
+        raise ValueError("ensemble rerun identity exceeds the v1 address domain")
+    return int(load_library().cpm_ensemble_seed(int(master_seed), int(trajectory), int(rerun)))

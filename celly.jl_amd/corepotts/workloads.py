@@ -1,0 +1,140 @@
+"""Deterministic workload builders: square/cube cell layouts on periodic lattices.
+
+Each builder returns (compiled_state, proposal_relation, components, algorithm, seed) ready
+for ``init_scientific``. Layout rules:
+
+* plain_2d: L x L periodic lattice, medium id 1 (type 1); cells are side x side squares whose
+  lower-left corners sit on a regular grid; cell id k (1-based) takes type 2 + (k-1) mod ntypes.
+* The contact matrix over types (1 = medium, 2, 3) is J11=0, J12=J13=16, J22=2, J33=14, J23=11.
+* Volume constraint lambda_V, target V*; optional boundary constraint lambda_S, target S* on
+  the von Neumann edge count; Moore contact relation; von Neumann proposal relation.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .model import (BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
+                    CheckerboardSweepCPM, LogicalPottsState, LotteryCPM, MediumTypeTable,
+                    ProposalRole, QuadraticBoundaryHamiltonian, QuadraticVolumeHamiltonian,
+                    ScientificComponentSet, SurfaceRole, ContactRole, UnorderedContactHamiltonian,
+                    compile_scientific_state, first_shell_relation, moore_relation,
+                    required_properties)
+
+J_TWO_TYPES = np.array([[0, 16, 16], [16, 2, 11], [16, 11, 14]], dtype=np.float32)
+
+
+def _schema(*components):
+    schema = {}
+    for c in components:
+        schema.update(required_properties(c))
+    return schema
+
+
+def square_cells_2d(L: int, side: int, corners, ntypes: int = 2):
+    """Owner words (column-major) and cell types for squares at the given corners."""
+    owners = np.full((L, L), -1, dtype=np.int64)  # indexed [x, y]
+    types = {}
+    for k, (x0, y0) in enumerate(corners, start=1):
+        owners[x0:x0 + side, y0:y0 + side] = k
+        types[k] = 2 + (k - 1) % ntypes
+    return owners.reshape(-1, order="F"), types
+
+
+def plain_2d(L: int, corners, side: int = 10, *, temperature=np.float32(10.0), seed: int = 1234,
+             lambda_v=np.float32(1.0), target_v=np.float32(100.0), lambda_s=None, target_s=40,
+             algorithm="checkerboard", J=J_TWO_TYPES, ntypes: int = 2, contact="moore"):
+    owners, types = square_cells_2d(L, side, corners, ntypes)
+    ncells = len(corners)
+    domain = CartesianDomain((L, L))
+    proposal = first_shell_relation(ProposalRole, 2)
+    surface = first_shell_relation(SurfaceRole, 2)
+    contact_rel = moore_relation(ContactRole, 2) if contact == "moore" else first_shell_relation(ContactRole, 2)
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    contact_h = UnorderedContactHamiltonian(np.asarray(J, dtype=np.float32),
+                                            MediumTypeTable({1: 1}), contact_rel)
+    energies = [volume, contact_h]
+    boundary = None
+    if lambda_s is not None:
+        boundary = QuadraticBoundaryHamiltonian(BoundaryEdgeCount(), surface, number_type=np.float32)
+        energies.append(boundary)
+    state = LogicalPottsState(owners, max(ncells, 1), cell_types=types, medium_domains=(1,),
+                              property_schema=_schema(*energies), dims=(L, L))
+    state.property_values("target_volume")[:ncells] = target_v
+    state.property_values("volume_strength")[:ncells] = lambda_v
+    if boundary is not None:
+        state.property_values("target_boundary")[:ncells] = target_s
+        state.property_values("boundary_strength")[:ncells] = lambda_s
+    tracker = BoundaryMeasureTracker(BoundaryEdgeCount(), surface)
+    compiled = compile_scientific_state(state, domain, tracker)
+    alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
+    return compiled, proposal, ScientificComponentSet(energies=tuple(energies)), alg, seed
+
+
+def config1_plain(algorithm="checkerboard", seed: int = 1234):
+    """200x200, 100 cells of 10x10 at (5+20i, 5+20j), volume + Moore contact, T=10."""
+    corners = [(5 + 20 * i, 5 + 20 * j) for j in range(10) for i in range(10)]
+    return plain_2d(200, corners, algorithm=algorithm, seed=seed)
+
+
+def grid_corners(L: int, side: int, count: int, pitch: int = 10):
+    per_row = L // pitch
+    out = []
+    for k in range(count):
+        row, col = divmod(k, per_row)
+        out.append((pitch * col, pitch * row))
+    return out
+
+
+def config2_large(algorithm="checkerboard", L: int = 4096, cells: int = 163840, seed: int = 1234):
+    """4096x4096, 163,840 cells of 10x10 on a pitch-10 slot grid filled row-major;
+    volume + Moore contact + boundary (lambda_S = 1, S* = 40), T = 10."""
+    corners = grid_corners(L, 10, cells)
+    return plain_2d(L, corners, algorithm=algorithm, seed=seed, lambda_s=np.float32(1.0))
+
+
+def scaled_2d(L: int, algorithm="checkerboard", seed: int = 1234, boundary: bool = True):
+    """The config-2 layout rule on an L x L lattice (cells fill the slot grid)."""
+    per_row = L // 10
+    cells = per_row * per_row
+    corners = grid_corners(L, 10, cells)
+    return plain_2d(L, corners, algorithm=algorithm, seed=seed,
+                    lambda_s=np.float32(1.0) if boundary else None)
+
+
+def contact_matrix_3d(seed: int = 20261203) -> np.ndarray:
+    """4x4 symmetric Float32 J with entries U[2, 20] from a fixed PCG64 stream."""
+    rng = np.random.default_rng(seed)
+    upper = rng.uniform(2.0, 20.0, size=(4, 4)).astype(np.float32)
+    J = np.triu(upper) + np.triu(upper, 1).T
+    return J.astype(np.float32)
+
+
+def cubes_3d(L: int = 256, side: int = 8, *, temperature=np.float32(20.0), seed: int = 1234,
+             algorithm="checkerboard", J=None):
+    """L^3 periodic lattice tiled exactly by side^3 cubes (no medium sites), 3 types."""
+    n_per = L // side
+    idx = np.arange(L)
+    cx, cy, cz = np.meshgrid(idx // side, idx // side, idx // side, indexing="ij")
+    cell = 1 + cx + n_per * (cy + n_per * cz)
+    owners = cell.reshape(-1, order="F").astype(np.int64)
+    ncells = n_per ** 3
+    types = {k: 2 + (k - 1) % 3 for k in range(1, ncells + 1)}
+    domain = CartesianDomain((L, L, L))
+    proposal = first_shell_relation(ProposalRole, 3)
+    surface = first_shell_relation(SurfaceRole, 3)
+    contact_rel = moore_relation(ContactRole, 3)
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    J = contact_matrix_3d() if J is None else J
+    contact_h = UnorderedContactHamiltonian(np.asarray(J, np.float32), MediumTypeTable({1: 1}),
+                                            contact_rel)
+    boundary = QuadraticBoundaryHamiltonian(BoundaryEdgeCount(), surface, number_type=np.float32)
+    energies = (volume, contact_h, boundary)
+    state = LogicalPottsState(owners, ncells, cell_types=types, medium_domains=(1,),
+                              property_schema=_schema(*energies), dims=(L, L, L))
+    state.property_values("target_volume")[:] = np.float32(side ** 3)
+    state.property_values("volume_strength")[:] = np.float32(1.5)
+    state.property_values("target_boundary")[:] = 6 * side * side
+    state.property_values("boundary_strength")[:] = np.float32(0.5)
+    compiled = compile_scientific_state(state, domain, BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
+    alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
+    return compiled, proposal, ScientificComponentSet(energies=energies), alg, seed

@@ -1,0 +1,208 @@
+// Device-side data structures and primitives of the MI355X CPM engine.
+//
+// Owner encoding on the device: one int32 word per site, +id for a cell, -id for a
+// medium; 0 never names an owner and marks an absent neighbour inside LDS tiles.
+// OwnerRef equality (lib/CorePotts/src/state/logical.jl:9-21) is word equality.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cpm_mi355x.h"
+
+namespace cpm {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kMaxRounds = 256;    // colour / round order capacity per replica
+constexpr int kSlotFields = 8;     // per-tile accounting fields
+
+// Semantic RNG stream ids (lib/CorePotts/src/rng/semantic.jl:37-58).
+enum : uint32_t {
+    kStreamRecipient = 3,
+    kStreamDirection = 4,
+    kStreamAcceptance = 5,
+    kStreamLotteryActivation = 6,
+    kStreamLotteryPriority = 7,
+    kStreamCheckerboardOrder = 8,
+    kStreamCheckerboardPriority = 17,
+};
+enum : uint32_t { kGlobalEntity = 0, kSiteEntity = 1 };
+
+// Relation offsets compiled for the device (canonical direction order kept).
+struct RelDev {
+    int32_t n;
+    int8_t off[CPM_MAX_DIRS][3];
+    float wf[CPM_MAX_DIRS];
+    double wd[CPM_MAX_DIRS];
+};
+
+// Static description of one replica (lattice, model, schedule and buffers).
+struct RepDev {
+    int32_t ndim;
+    int32_t dims[3];
+    uint32_t nsites;
+    int32_t face_kind[3][2];
+    int32_t face_word[3][2];     // owner word of fixed faces
+    int32_t simple;              // every face periodic, no obstacles, dims >= halo
+    int32_t ntiles[3];
+    uint32_t tile_base;          // first tile of this replica inside its group
+    uint32_t tile_count;
+    int32_t algorithm;
+    int32_t rounds;              // colours (checkerboard) or lottery rounds
+    int32_t uniform_degree;      // lottery: every site has rounds-1 conflict neighbours
+    float temperature;
+    uint32_t key_lo, key_hi;     // Philox key = seed ^ mix64(generation domain)
+    int32_t pat[3];              // periodic colour pattern periods (0: per-site array)
+    uint8_t pattern[64];
+    int32_t n_energies;
+    int32_t energy_order[3];
+    int32_t volume_f64, contact_f64, boundary_f64;
+    int32_t has_contact;
+    int32_t n_types;
+    float Jf[CPM_MAX_TYPES * CPM_MAX_TYPES];
+    double Jd[CPM_MAX_TYPES * CPM_MAX_TYPES];
+    int32_t n_table;
+    int32_t table_word[CPM_MAX_MEDIA];
+    uint32_t table_type[CPM_MAX_MEDIA];
+    int32_t n_media;
+    int32_t medium_word[CPM_MAX_MEDIA];
+    double retire_tv, retire_vs, retire_bs;
+    int64_t retire_tb;
+    uint32_t claim_base;         // first claim slot of this replica (cell id 0)
+    RelDev prop, cont, surf, conf;
+    // device buffers
+    int32_t *lattice;
+    const uint8_t *mask;         // nullptr: all mutable
+    const uint8_t *site_color;   // nullptr: pattern colouring
+    const uint8_t *site_degree;  // lottery, nullptr: uniform degree
+    int32_t *vol;                // cap + 1, index = cell id
+    int64_t *bnd;
+    uint32_t *ctype;
+    uint8_t *active;
+    double *tv, *vs, *bs;
+    int64_t *tb;
+    int32_t *medium_vol;         // n_media
+};
+
+// Per-replica dynamic state, updated on the device by the begin-MCS kernel.
+struct RepDyn {
+    uint64_t mcs;      // MCS being performed (after begin) / last completed
+    uint64_t target;   // replica advances while mcs < target
+    uint32_t active;   // performing an MCS in the current step
+    uint32_t in_use;
+    uint32_t order[kMaxRounds];
+};
+
+// One actionable copy attempt staged for the claim-resolution / commit kernel.
+struct Entry {
+    uint32_t site;      // recipient (1-based linear site id)
+    uint32_t priority;  // claim priority (Philox word 1)
+    int32_t losing;     // owner word of the recipient
+    int32_t gaining;    // owner word of the donor
+};
+
+struct PassArgs {
+    const RepDev *reps;
+    RepDyn *dyn;
+    const uint32_t *tile_rep;   // group tile -> replica slot
+    unsigned long long *claims; // two buffers of total_claims each
+    Entry *seg;                 // per-tile segments of seg_cap entries
+    uint32_t *seg_count;
+    uint32_t *slots;            // [pass][tile][kSlotFields]
+    uint32_t ntiles;
+    uint32_t seg_cap;
+    uint32_t pass;              // ordinal inside the MCS (0-based)
+    uint32_t parity;            // claim buffer used by this pass
+    uint32_t total_claims;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (rng/semantic.jl:136-187) and the v1 address packing (:249-261)
+// ---------------------------------------------------------------------------
+struct Words {
+    uint32_t w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ Words philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                          uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9e3779b9u;
+            k1 += 0xbb67ae85u;
+        }
+        const uint64_t p0 = (uint64_t)0xd2511f53u * c0;
+        const uint64_t p1 = (uint64_t)0xcd9e8d57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    return Words{c0, c1, c2, c3};
+}
+
+// counter0 = mcs | subround << 48 | invocation << 56
+// counter1 = entity | operation << 32 | stream << 44 | kind << 52 | draw << 54
+__device__ __forceinline__ Words rng_words(uint32_t key_lo, uint32_t key_hi, uint32_t stream,
+                                           uint64_t mcs, uint32_t subround, uint32_t operation,
+                                           uint32_t kind, uint32_t entity, uint32_t invocation,
+                                           uint32_t draw) {
+    const uint32_t c0 = (uint32_t)mcs;
+    const uint32_t c1 = ((uint32_t)(mcs >> 32) & 0xffffu) | ((subround & 0xffu) << 16) |
+                        ((invocation & 0xffu) << 24);
+    const uint32_t c2 = entity;
+    const uint32_t c3 = operation | (stream << 12) | (kind << 20) | (draw << 22);
+    return philox10(c0, c1, c2, c3, key_lo, key_hi);
+}
+
+// bounded_uint (rng/semantic.jl:298-310): reject below (2^32 - bound) mod bound,
+// retrying with invocation + 1.
+__device__ __forceinline__ uint32_t bounded_uint(uint32_t key_lo, uint32_t key_hi,
+                                                 uint32_t stream, uint64_t mcs,
+                                                 uint32_t subround, uint32_t operation,
+                                                 uint32_t kind, uint32_t entity, uint32_t draw,
+                                                 uint32_t bound) {
+    const uint32_t threshold = (0u - bound) % bound;
+    uint32_t invocation = 0;
+    while (true) {
+        const Words w = rng_words(key_lo, key_hi, stream, mcs, subround, operation, kind,
+                                  entity, invocation, draw);
+        if (w.w0 >= threshold || invocation == 255u) return w.w0 % bound;
+        ++invocation;
+    }
+}
+
+// uniform_open01(Float32) (rng/semantic.jl:276-280), all Float32 operations.
+__device__ __forceinline__ float uniform_open01(uint32_t w0) {
+    const float bits = (float)(w0 >> 8);
+    return __fmul_rn(__fadd_rn(bits, 0.5f), 0x1.0p-24f);
+}
+
+// Fisher-Yates schedule permutation (checkerboard.jl:226-242, lottery.jl:124-140).
+__device__ inline void schedule_order(uint32_t key_lo, uint32_t key_hi, uint32_t stream,
+                                      uint32_t operation, uint64_t mcs, int count,
+                                      uint32_t *order) {
+    for (int i = 0; i < count; ++i) order[i] = (uint32_t)(i + 1);
+    for (int c = count; c >= 2; --c) {
+        const uint32_t draw = (uint32_t)(count - c);
+        const uint32_t sel = bounded_uint(key_lo, key_hi, stream, mcs, 0, operation,
+                                          kGlobalEntity, 0, draw, (uint32_t)c) + 1;
+        const uint32_t t = order[c - 1];
+        order[c - 1] = order[sel - 1];
+        order[sel - 1] = t;
+    }
+}
+
+// XCD-aware workgroup -> tile map. Workgroups are dispatched round-robin over the
+// eight XCDs; this gives each XCD one contiguous band of tiles so that halo rows
+// shared by neighbouring tiles stay in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t wg, uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t q = n / kXcd, rem = n % kXcd;
+    const uint32_t xcd = wg % kXcd, k = wg / kXcd;
+    return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+}
+
+}  // namespace cpm

@@ -1,0 +1,1312 @@
+// Host side of the MI355X CPM engine: schedule compilation (realized greedy colouring,
+// conflict degrees), replica groups and their device arenas, stepping with optional
+// HIP-graph chunks, state transfer and the C-ABI declared in include/cpm_mi355x.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cpm_mi355x.h"
+#include "cpm_device.h"
+
+extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz);
+extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const cpm::PassArgs *a,
+                                               hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
+                                                int nslots, hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
+                                                  uint64_t target, uint32_t in_use, uint32_t mask,
+                                                  hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_report(const cpm::RepDev *reps, const uint32_t *slots,
+                                                 uint32_t ntiles, int slot,
+                                                 unsigned long long *out, hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Failure {
+    int32_t code;
+    std::string message;
+};
+
+[[noreturn]] void fail(int32_t code, const std::string &msg) { throw Failure{code, msg}; }
+
+void hip_check(hipError_t err, const char *what) {
+    if (err != hipSuccess)
+        fail(CPM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+// rng/semantic.jl:191-195
+uint64_t mix64(uint64_t v) {
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+    return v ^ (v >> 31);
+}
+
+// Host Philox for cpm_ensemble_seed (rng/semantic.jl:144-187).
+void philox_host(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9e3779b9u;
+            k1 += 0xbb67ae85u;
+        }
+        const uint64_t p0 = (uint64_t)0xd2511f53u * c[0];
+        const uint64_t p1 = (uint64_t)0xcd9e8d57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+    }
+}
+
+constexpr uint64_t kGenerationDomain = 0xd2b74407b1ce6e93ull;
+
+template <typename T>
+T *dev_alloc(size_t count, uint64_t &bytes) {
+    if (count == 0) count = 1;
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    bytes += count * sizeof(T);
+    return static_cast<T *>(p);
+}
+
+void dev_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// Replica: one lattice with its compiled schedule and device state
+// ---------------------------------------------------------------------------
+struct Replica {
+    cpm_model_desc desc{};
+    std::vector<uint8_t> mask, imm_tags;
+    std::vector<uint32_t> imm_ids;
+    bool has_mask = false;
+    int64_t n = 0, nmut = 0;
+    int halo = 1;
+    int config = 0;
+    int rounds = 0;
+    std::vector<int32_t> conf_off;  // flattened union offsets (x,y,z)
+    std::vector<uint8_t> colors;    // checkerboard realized colouring
+    std::vector<uint8_t> degrees;   // lottery conflict degrees
+    bool uniform_degree = true;
+    bool per_site_color = false;
+    cpm::RepDev dev{};
+    // device buffers
+    int32_t *d_lattice = nullptr;
+    uint8_t *d_mask = nullptr, *d_color = nullptr, *d_degree = nullptr, *d_active = nullptr;
+    int32_t *d_vol = nullptr, *d_mvol = nullptr;
+    int64_t *d_bnd = nullptr, *d_tb = nullptr;
+    uint32_t *d_ctype = nullptr;
+    double *d_tv = nullptr, *d_vs = nullptr, *d_bs = nullptr;
+    uint64_t bytes = 0;
+    uint64_t base_mcs = 0;      // MCS counter when the state was last set
+    bool report_cached = false;  // cached_report valid while mcs == cached_mcs
+    uint64_t cached_mcs = 0;
+    uint64_t cached_report[12] = {};
+    int group = -1;
+
+    ~Replica() {
+        dev_free(d_lattice); dev_free(d_mask); dev_free(d_color); dev_free(d_degree);
+        dev_free(d_active); dev_free(d_vol); dev_free(d_mvol); dev_free(d_bnd); dev_free(d_tb);
+        dev_free(d_ctype); dev_free(d_tv); dev_free(d_vs); dev_free(d_bs);
+    }
+};
+
+int max_abs_offset(const cpm_relation_desc &r, int ndim) {
+    int h = 0;
+    for (int k = 0; k < r.count; ++k)
+        for (int a = 0; a < ndim; ++a) h = std::max(h, std::abs(r.offsets[k][a]));
+    return h;
+}
+
+bool has_term(const cpm_model_desc &d, int term) {
+    for (int t = 0; t < d.n_energies; ++t)
+        if (d.energy_order[t] == term) return true;
+    return false;
+}
+
+void validate_relation(const cpm_relation_desc &r, int ndim, const char *name) {
+    if (r.count < 1 || r.count > CPM_MAX_DIRS)
+        fail(CPM_E_INVALID, std::string(name) + " relation needs 1..128 directions");
+    for (int k = 0; k < r.count; ++k) {
+        bool nonzero = false;
+        for (int a = 0; a < 3; ++a) {
+            if (a >= ndim && r.offsets[k][a] != 0)
+                fail(CPM_E_INVALID, std::string(name) + " relation offset has a component beyond ndim");
+            nonzero |= r.offsets[k][a] != 0;
+        }
+        if (!nonzero) fail(CPM_E_INVALID, std::string(name) + " relation offsets must be nonzero");
+        if (!(r.weights[k] >= 0.0) || !std::isfinite(r.weights[k]))
+            fail(CPM_E_INVALID, std::string(name) + " relation weights must be finite and non-negative");
+    }
+}
+
+// Host realization of a neighbour position (cartesian.jl:445-525).
+// Returns 0-based linear index of the realized site, or -1 if absent/invalid/exterior.
+struct HostGeom {
+    int ndim;
+    int dims[3];
+    int face_kind[3][2];
+    uint32_t face_medium[3][2];
+    const uint8_t *mask;
+    bool simple;
+};
+
+int64_t realize_site(const HostGeom &g, const int c0[3], const int32_t *off) {
+    int c[3] = {c0[0] + off[0], c0[1] + off[1], c0[2] + off[2]};
+    bool closed = false, fixed = false, invalid = false;
+    uint32_t fixed_medium = 0;
+    for (int a = 0; a < g.ndim; ++a) {
+        const int n = g.dims[a];
+        int v = c[a];
+        int face;
+        if (v < 0) face = 0;
+        else if (v >= n) face = 1;
+        else continue;
+        const int kind = g.face_kind[a][face];
+        if (kind == CPM_FACE_PERIODIC) {
+            v %= n;
+            if (v < 0) v += n;
+            c[a] = v;
+        } else if (kind == CPM_FACE_CLOSED) {
+            invalid |= fixed;
+            closed = true;
+        } else {
+            const uint32_t m = g.face_medium[a][face];
+            const bool incompatible = fixed && fixed_medium != m;
+            invalid |= closed | incompatible;
+            fixed = true;
+            fixed_medium = m;
+        }
+    }
+    if (invalid || closed || fixed) return -1;
+    return (int64_t)c[0] + (int64_t)g.dims[0] * ((int64_t)c[1] + (int64_t)g.dims[1] * c[2]);
+}
+
+// Realized mutable conflict neighbours (distinct, excluding the site itself).
+int conflict_neighbours(const HostGeom &g, const std::vector<int32_t> &offs, int64_t i,
+                        int64_t *out) {
+    int c[3];
+    c[0] = (int)(i % g.dims[0]);
+    c[1] = (int)((i / g.dims[0]) % g.dims[1]);
+    c[2] = (int)(i / ((int64_t)g.dims[0] * g.dims[1]));
+    const int k = (int)offs.size() / 3;
+    int count = 0;
+    for (int d = 0; d < k; ++d) {
+        const int64_t j = realize_site(g, c, &offs[3 * d]);
+        if (j < 0 || j == i) continue;
+        if (g.mask && !g.mask[j]) continue;
+        bool dup = false;
+        for (int q = 0; q < count; ++q) dup |= out[q] == j;
+        if (!dup) out[count++] = j;
+    }
+    return count;
+}
+
+void compile_replica(Replica &r) {
+    const cpm_model_desc &d = r.desc;
+    if (d.abi_version != CPM_ABI_VERSION) fail(CPM_E_INVALID, "model descriptor ABI version mismatch");
+    if (d.algorithm != CPM_ALG_CHECKERBOARD && d.algorithm != CPM_ALG_LOTTERY)
+        fail(CPM_E_UNSUPPORTED, "only CheckerboardSweepCPM and LotteryCPM run on this engine");
+    if (d.ndim != 2 && d.ndim != 3) fail(CPM_E_INVALID, "Cartesian domains are two- or three-dimensional");
+    r.desc.dims[2] = d.ndim == 2 ? 1 : d.dims[2];
+    uint64_t n = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (r.desc.dims[a] <= 0) fail(CPM_E_INVALID, "Cartesian dimensions must be positive");
+        n *= (uint64_t)r.desc.dims[a];
+    }
+    if (n > 0xffffffffull - 1) fail(CPM_E_INVALID, "site count must fit the UInt32 site representation");
+    r.n = (int64_t)n;
+    for (int a = 0; a < d.ndim; ++a) {
+        const int k0 = d.face_kind[a][0], k1 = d.face_kind[a][1];
+        if (k0 < 0 || k0 > 2 || k1 < 0 || k1 > 2) fail(CPM_E_INVALID, "unknown boundary kind");
+        if ((k0 == CPM_FACE_PERIODIC) != (k1 == CPM_FACE_PERIODIC))
+            fail(CPM_E_INVALID, "periodic boundary faces must occur as a paired axis");
+    }
+    validate_relation(d.proposal, d.ndim, "proposal");
+    validate_relation(d.surface, d.ndim, "surface");
+    const bool contact = has_term(d, CPM_TERM_CONTACT);
+    if (contact) validate_relation(d.contact, d.ndim, "contact");
+    if (d.n_energies < 0 || d.n_energies > 3) fail(CPM_E_INVALID, "at most three energy terms");
+    for (int t = 0; t < d.n_energies; ++t)
+        if (d.energy_order[t] < 0 || d.energy_order[t] > 2) fail(CPM_E_INVALID, "unknown energy term");
+    if (contact && (d.n_types < 1 || d.n_types > CPM_MAX_TYPES))
+        fail(CPM_E_INVALID, "contact matrix size must be 1..16");
+    if (d.n_table < 0 || d.n_table > CPM_MAX_MEDIA || d.n_media < 0 || d.n_media > CPM_MAX_MEDIA)
+        fail(CPM_E_INVALID, "at most 16 media");
+    if (!(d.temperature >= 0.0f) || !std::isfinite(d.temperature))
+        fail(CPM_E_INVALID, "temperature must be finite and non-negative");
+    if (d.capacity < 1 || d.capacity > 0x7ffffffeu) fail(CPM_E_INVALID, "capacity must be 1..2^31-2");
+
+    r.halo = std::max(max_abs_offset(d.proposal, d.ndim), max_abs_offset(d.surface, d.ndim));
+    if (contact) r.halo = std::max(r.halo, max_abs_offset(d.contact, d.ndim));
+    if (r.halo > 2) fail(CPM_E_UNSUPPORTED, "relations reaching beyond distance 2 are not supported");
+    r.halo = std::max(r.halo, 1);
+
+    r.has_mask = false;
+    if (d.mutable_mask) {
+        r.mask.assign(d.mutable_mask, d.mutable_mask + r.n);
+        r.imm_tags.assign(d.immutable_tags, d.immutable_tags + r.n);
+        r.imm_ids.assign(d.immutable_ids, d.immutable_ids + r.n);
+        for (int64_t i = 0; i < r.n; ++i)
+            if (!r.mask[i]) r.has_mask = true;
+        if (!r.has_mask) {
+            r.mask.clear();
+            r.imm_tags.clear();
+            r.imm_ids.clear();
+        }
+    }
+    r.desc.mutable_mask = nullptr;
+    r.desc.immutable_tags = nullptr;
+    r.desc.immutable_ids = nullptr;
+    r.nmut = 0;
+    for (int64_t i = 0; i < r.n; ++i) r.nmut += r.has_mask ? (r.mask[i] != 0) : 1;
+    if (r.nmut < 1) fail(CPM_E_INVALID, "a Cartesian ownership domain requires a mutable site");
+
+    bool all_periodic = true;
+    for (int a = 0; a < d.ndim; ++a)
+        all_periodic = all_periodic && d.face_kind[a][0] == CPM_FACE_PERIODIC;
+    bool big = true;
+    for (int a = 0; a < d.ndim; ++a) big = big && r.desc.dims[a] >= r.halo;
+    const bool simple = all_periodic && !r.has_mask && big;
+
+    // conflict relations: proposal, boundary tracker, contact (checkerboard.jl:150-157)
+    r.conf_off.clear();
+    auto add_rel = [&](const cpm_relation_desc &rel) {
+        for (int k = 0; k < rel.count; ++k) {
+            bool dup = false;
+            for (size_t q = 0; q < r.conf_off.size(); q += 3)
+                dup |= r.conf_off[q] == rel.offsets[k][0] && r.conf_off[q + 1] == rel.offsets[k][1] &&
+                       r.conf_off[q + 2] == rel.offsets[k][2];
+            if (!dup)
+                for (int a = 0; a < 3; ++a) r.conf_off.push_back(rel.offsets[k][a]);
+        }
+    };
+    add_rel(d.proposal);
+    add_rel(d.surface);
+    if (contact) add_rel(d.contact);
+    if (r.conf_off.size() / 3 > CPM_MAX_DIRS) fail(CPM_E_UNSUPPORTED, "conflict relation too large");
+
+    HostGeom g{};
+    g.ndim = d.ndim;
+    for (int a = 0; a < 3; ++a) {
+        g.dims[a] = r.desc.dims[a];
+        g.face_kind[a][0] = d.face_kind[a][0];
+        g.face_kind[a][1] = d.face_kind[a][1];
+        g.face_medium[a][0] = d.face_medium[a][0];
+        g.face_medium[a][1] = d.face_medium[a][1];
+    }
+    g.mask = r.has_mask ? r.mask.data() : nullptr;
+    g.simple = simple;
+
+    std::vector<int64_t> nb(r.conf_off.size() / 3 + 1);
+    int pattern[3] = {0, 0, 0};
+    std::vector<uint8_t> pattern_table;
+    if (d.algorithm == CPM_ALG_CHECKERBOARD) {
+        // checkerboard.jl:57-90: greedy smallest free colour, ascending site order
+        r.colors.assign((size_t)r.n, 0);
+        int maxcolor = 0;
+        std::vector<char> forbidden;
+        for (int64_t i = 0; i < r.n; ++i) {
+            if (g.mask && !g.mask[i]) continue;
+            const int c = conflict_neighbours(g, r.conf_off, i, nb.data());
+            forbidden.assign((size_t)c + 2, 0);
+            for (int q = 0; q < c; ++q) {
+                const int col = r.colors[nb[q]];
+                if (col > 0 && col <= c + 1) forbidden[col] = 1;
+            }
+            int col = 1;
+            while (forbidden[col]) ++col;
+            if (col > 255) fail(CPM_E_UNSUPPORTED, "realized colouring exceeds 255 colours");
+            r.colors[i] = (uint8_t)col;
+            maxcolor = std::max(maxcolor, col);
+        }
+        r.rounds = maxcolor;
+        // periodic pattern detection: colour == f(x mod px, y mod py, z mod pz)
+        bool found = false;
+        for (int prod = 1; prod <= 64 && !found; ++prod) {
+            for (int px = 1; px <= 8 && !found; ++px)
+                for (int py = 1; py <= 8 && !found; ++py)
+                    for (int pz = 1; pz <= (d.ndim == 3 ? 8 : 1) && !found; ++pz) {
+                        if (px * py * pz != prod) continue;
+                        std::vector<uint8_t> table((size_t)prod, 0);
+                        bool ok = true;
+                        for (int64_t i = 0; i < r.n && ok; ++i) {
+                            if (g.mask && !g.mask[i]) continue;
+                            const int x = (int)(i % g.dims[0]);
+                            const int y = (int)((i / g.dims[0]) % g.dims[1]);
+                            const int z = (int)(i / ((int64_t)g.dims[0] * g.dims[1]));
+                            const int key = (x % px) + px * ((y % py) + py * (z % pz));
+                            if (table[key] == 0) table[key] = r.colors[i];
+                            else ok = table[key] == r.colors[i];
+                        }
+                        if (ok) {
+                            found = true;
+                            pattern[0] = px;
+                            pattern[1] = py;
+                            pattern[2] = pz;
+                            pattern_table = table;
+                        }
+                    }
+        }
+        r.per_site_color = !found;
+    } else {
+        // lottery.jl:46-92: rounds = maximum realized conflict degree + 1
+        r.degrees.assign((size_t)r.n, 0);
+        int maxdeg = 0;
+        for (int64_t i = 0; i < r.n; ++i) {
+            if (g.mask && !g.mask[i]) continue;
+            const int c = conflict_neighbours(g, r.conf_off, i, nb.data());
+            r.degrees[i] = (uint8_t)std::min(c, 255);
+            maxdeg = std::max(maxdeg, c);
+        }
+        if (maxdeg + 1 > 255) fail(CPM_E_UNSUPPORTED, "lottery conflict degree exceeds the subround domain");
+        r.rounds = maxdeg + 1;
+        r.uniform_degree = true;
+        for (int64_t i = 0; i < r.n; ++i) {
+            if (g.mask && !g.mask[i]) continue;
+            if (r.degrees[i] != maxdeg) {
+                r.uniform_degree = false;
+                break;
+            }
+        }
+    }
+
+    // tile configuration by dimension and halo
+    r.config = d.ndim == 2 ? (r.halo == 1 ? 0 : 1) : (r.halo == 1 ? 2 : 3);
+    int cdim, chalo, tx, ty, tz;
+    cpm_internal_tile_config(r.config, &cdim, &chalo, &tx, &ty, &tz);
+
+    cpm::RepDev &v = r.dev;
+    std::memset(&v, 0, sizeof(v));
+    v.ndim = d.ndim;
+    for (int a = 0; a < 3; ++a) v.dims[a] = r.desc.dims[a];
+    v.nsites = (uint32_t)r.n;
+    for (int a = 0; a < 3; ++a)
+        for (int f = 0; f < 2; ++f) {
+            v.face_kind[a][f] = a < d.ndim ? d.face_kind[a][f] : CPM_FACE_PERIODIC;
+            v.face_word[a][f] = -(int32_t)d.face_medium[a][f];
+        }
+    v.simple = simple ? 1 : 0;
+    v.ntiles[0] = (v.dims[0] + tx - 1) / tx;
+    v.ntiles[1] = (v.dims[1] + ty - 1) / ty;
+    v.ntiles[2] = (v.dims[2] + tz - 1) / tz;
+    v.algorithm = d.algorithm;
+    v.rounds = r.rounds;
+    v.uniform_degree = r.uniform_degree ? 1 : 0;
+    v.temperature = d.temperature;
+    const uint64_t key = d.seed ^ mix64(0ull ^ kGenerationDomain);
+    v.key_lo = (uint32_t)key;
+    v.key_hi = (uint32_t)(key >> 32);
+    if (d.algorithm == CPM_ALG_CHECKERBOARD && !r.per_site_color) {
+        for (int a = 0; a < 3; ++a) v.pat[a] = pattern[a];
+        std::memcpy(v.pattern, pattern_table.data(), pattern_table.size());
+    } else {
+        v.pat[0] = v.pat[1] = v.pat[2] = 1;
+    }
+    v.n_energies = d.n_energies;
+    for (int t = 0; t < 3; ++t) v.energy_order[t] = d.energy_order[t];
+    v.volume_f64 = d.volume_f64 ? 1 : 0;
+    v.contact_f64 = d.contact_f64 ? 1 : 0;
+    v.boundary_f64 = d.boundary_f64 ? 1 : 0;
+    v.has_contact = contact ? 1 : 0;
+    v.n_types = d.n_types;
+    for (int i = 0; i < CPM_MAX_TYPES * CPM_MAX_TYPES; ++i) {
+        v.Jd[i] = d.J[i];
+        v.Jf[i] = (float)d.J[i];
+    }
+    v.n_table = d.n_table;
+    for (int i = 0; i < d.n_table; ++i) {
+        v.table_word[i] = -(int32_t)d.table_owner[i];
+        v.table_type[i] = d.table_type[i];
+    }
+    v.n_media = d.n_media;
+    for (int i = 0; i < d.n_media; ++i) v.medium_word[i] = -(int32_t)d.medium_ids[i];
+    v.retire_tv = d.retire_target_volume;
+    v.retire_vs = d.retire_volume_strength;
+    v.retire_bs = d.retire_boundary_strength;
+    v.retire_tb = d.retire_target_boundary;
+    auto fill_rel = [](cpm::RelDev &o, const cpm_relation_desc &rel) {
+        o.n = rel.count;
+        for (int k = 0; k < rel.count; ++k) {
+            for (int a = 0; a < 3; ++a) o.off[k][a] = (int8_t)rel.offsets[k][a];
+            o.wd[k] = rel.weights[k];
+            o.wf[k] = (float)rel.weights[k];
+        }
+    };
+    fill_rel(v.prop, d.proposal);
+    fill_rel(v.surf, d.surface);
+    if (contact) fill_rel(v.cont, d.contact);
+    v.conf.n = (int32_t)(r.conf_off.size() / 3);
+    for (int k = 0; k < v.conf.n; ++k)
+        for (int a = 0; a < 3; ++a) v.conf.off[k][a] = (int8_t)r.conf_off[3 * k + a];
+}
+
+void allocate_replica(Replica &r) {
+    const size_t cap1 = (size_t)r.desc.capacity + 1;
+    r.d_lattice = dev_alloc<int32_t>((size_t)r.n, r.bytes);
+    if (r.has_mask) {
+        r.d_mask = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
+        hip_check(hipMemcpy(r.d_mask, r.mask.data(), (size_t)r.n, hipMemcpyHostToDevice), "copy mask");
+    }
+    if (r.desc.algorithm == CPM_ALG_CHECKERBOARD && r.per_site_color) {
+        r.d_color = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
+        hip_check(hipMemcpy(r.d_color, r.colors.data(), (size_t)r.n, hipMemcpyHostToDevice), "copy colours");
+    }
+    if (r.desc.algorithm == CPM_ALG_LOTTERY && !r.uniform_degree) {
+        r.d_degree = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
+        hip_check(hipMemcpy(r.d_degree, r.degrees.data(), (size_t)r.n, hipMemcpyHostToDevice), "copy degrees");
+    }
+    r.d_vol = dev_alloc<int32_t>(cap1, r.bytes);
+    r.d_bnd = dev_alloc<int64_t>(cap1, r.bytes);
+    r.d_ctype = dev_alloc<uint32_t>(cap1, r.bytes);
+    r.d_active = dev_alloc<uint8_t>(cap1, r.bytes);
+    r.d_tv = dev_alloc<double>(cap1, r.bytes);
+    r.d_vs = dev_alloc<double>(cap1, r.bytes);
+    r.d_tb = dev_alloc<int64_t>(cap1, r.bytes);
+    r.d_bs = dev_alloc<double>(cap1, r.bytes);
+    r.d_mvol = dev_alloc<int32_t>(CPM_MAX_MEDIA, r.bytes);
+    hip_check(hipMemset(r.d_lattice, 0, (size_t)r.n * 4), "memset");
+    hip_check(hipMemset(r.d_vol, 0, cap1 * 4), "memset");
+    hip_check(hipMemset(r.d_bnd, 0, cap1 * 8), "memset");
+    hip_check(hipMemset(r.d_ctype, 0, cap1 * 4), "memset");
+    hip_check(hipMemset(r.d_active, 0, cap1), "memset");
+    hip_check(hipMemset(r.d_tv, 0, cap1 * 8), "memset");
+    hip_check(hipMemset(r.d_vs, 0, cap1 * 8), "memset");
+    hip_check(hipMemset(r.d_tb, 0, cap1 * 8), "memset");
+    hip_check(hipMemset(r.d_bs, 0, cap1 * 8), "memset");
+    hip_check(hipMemset(r.d_mvol, 0, CPM_MAX_MEDIA * 4), "memset");
+    cpm::RepDev &v = r.dev;
+    v.lattice = r.d_lattice;
+    v.mask = r.d_mask;
+    v.site_color = r.d_color;
+    v.site_degree = r.d_degree;
+    v.vol = r.d_vol;
+    v.bnd = r.d_bnd;
+    v.ctype = r.d_ctype;
+    v.active = r.d_active;
+    v.tv = r.d_tv;
+    v.vs = r.d_vs;
+    v.tb = r.d_tb;
+    v.bs = r.d_bs;
+    v.medium_vol = r.d_mvol;
+}
+
+uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
+    uint64_t n = 1;
+    for (int a = 0; a < d.ndim; ++a) n *= (uint64_t)std::max(d.dims[a], 1);
+    const uint64_t cap1 = (uint64_t)d.capacity + 1;
+    // lattice + optional mask/colour/degree + segment share + per-cell state + claims
+    return n * 4 + n * 2 + n * 16 + cap1 * (4 + 8 + 4 + 1 + 8 * 4) + cap1 * 16 + (1u << 16);
+}
+
+// ---------------------------------------------------------------------------
+// Groups: replicas sharing one kernel instantiation and one set of arenas
+// ---------------------------------------------------------------------------
+struct Group {
+    int config = 0;
+    int tx = 0, ty = 0, tz = 0;
+    std::vector<int> slots;
+    uint32_t ntiles = 0, total_claims = 0, seg_cap = 0;
+    int max_rounds = 0;
+    uint32_t *d_tile_rep = nullptr;
+    unsigned long long *d_claims = nullptr;
+    cpm::Entry *d_seg = nullptr;
+    uint32_t *d_seg_count = nullptr;
+    uint32_t *d_slots = nullptr;
+    size_t cap_tiles = 0, cap_claims = 0, cap_slots = 0, cap_seg = 0;
+    uint64_t bytes = 0;
+    uint64_t pass_counter = 0;
+    bool dirty = true;
+    bool dirty_arena_fresh = true;  // no accounting written since the last rebuild
+
+    void release() {
+        dev_free(d_tile_rep); dev_free(d_claims); dev_free(d_seg); dev_free(d_seg_count);
+        dev_free(d_slots);
+        d_tile_rep = nullptr; d_claims = nullptr; d_seg = nullptr; d_seg_count = nullptr;
+        d_slots = nullptr;
+        cap_tiles = cap_claims = cap_slots = cap_seg = 0;
+        bytes = 0;
+    }
+};
+
+struct Engine {
+    int algorithm = CPM_ALG_CHECKERBOARD;
+    hipStream_t stream = nullptr;
+    std::vector<Replica *> reps;
+    cpm::RepDev *d_reps = nullptr;
+    cpm::RepDyn *d_dyn = nullptr;
+    int slot_cap = 0;
+    std::vector<Group> groups;
+    unsigned long long *d_report = nullptr;
+    unsigned long long *h_report = nullptr;
+    uint64_t metrics[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t budget = 0;
+    uint32_t graph_chunk = 0;
+    uint32_t graph_mcs = 0;  // MCS per captured graph
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool graph_valid = false;
+    std::string err;
+
+    ~Engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
+        for (Replica *r : reps) delete r;
+        for (Group &g : groups) g.release();
+        dev_free(d_reps);
+        dev_free(d_dyn);
+        dev_free(d_report);
+        if (h_report) (void)hipHostFree(h_report);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_valid = false;
+    }
+
+    void init(int alg) {
+        algorithm = alg;
+        hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+        hip_check(hipMalloc(&d_report, 16 * sizeof(unsigned long long)), "hipMalloc");
+        hip_check(hipHostMalloc((void **)&h_report, 16 * sizeof(unsigned long long), hipHostMallocDefault),
+                  "hipHostMalloc");
+    }
+
+    uint64_t bytes() const {
+        uint64_t b = (uint64_t)slot_cap * (sizeof(cpm::RepDev) + sizeof(cpm::RepDyn));
+        for (const Replica *r : reps)
+            if (r) b += r->bytes;
+        for (const Group &g : groups) b += g.bytes;
+        return b;
+    }
+
+    void ensure_slots(int need) {
+        if (need <= slot_cap) return;
+        int cap = std::max(need, std::max(8, slot_cap * 2));
+        cpm::RepDev *nr = nullptr;
+        cpm::RepDyn *nd = nullptr;
+        hip_check(hipMalloc(&nr, (size_t)cap * sizeof(cpm::RepDev)), "hipMalloc reps");
+        hip_check(hipMalloc(&nd, (size_t)cap * sizeof(cpm::RepDyn)), "hipMalloc dyn");
+        hip_check(hipMemsetAsync(nr, 0, (size_t)cap * sizeof(cpm::RepDev), stream), "memset");
+        hip_check(hipMemsetAsync(nd, 0, (size_t)cap * sizeof(cpm::RepDyn), stream), "memset");
+        if (slot_cap) {
+            hip_check(hipMemcpyAsync(nr, d_reps, (size_t)slot_cap * sizeof(cpm::RepDev),
+                                     hipMemcpyDeviceToDevice, stream), "copy reps");
+            hip_check(hipMemcpyAsync(nd, d_dyn, (size_t)slot_cap * sizeof(cpm::RepDyn),
+                                     hipMemcpyDeviceToDevice, stream), "copy dyn");
+        }
+        hip_check(hipStreamSynchronize(stream), "sync");
+        dev_free(d_reps);
+        dev_free(d_dyn);
+        d_reps = nr;
+        d_dyn = nd;
+        slot_cap = cap;
+        reps.resize((size_t)cap, nullptr);
+        drop_graph();
+    }
+
+    void upload_rep(int slot) {
+        hip_check(hipStreamSynchronize(stream), "sync");
+        hip_check(hipMemcpy(d_reps + slot, &reps[slot]->dev, sizeof(cpm::RepDev), hipMemcpyHostToDevice),
+                  "upload replica");
+        metrics[2]++;
+    }
+
+    void set_dyn(int slot, uint64_t mcs, uint64_t target, uint32_t in_use, uint32_t mask) {
+        hip_check(cpm_internal_launch_set_dyn(d_dyn, slot, mcs, target, in_use, mask, stream), "set_dyn");
+        metrics[0]++;
+    }
+
+    int add(const cpm_model_desc &desc) {
+        if (desc.algorithm != algorithm) fail(CPM_E_INVALID, "replica algorithm differs from the batch");
+        Replica *r = new Replica();
+        try {
+            r->desc = desc;
+            compile_replica(*r);
+            if (budget && bytes() + estimate_replica_bytes(r->desc) > budget)
+                fail(CPM_E_NOMEM, "HBM budget exceeded");
+            allocate_replica(*r);
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        int slot = -1;
+        for (int s = 0; s < slot_cap; ++s)
+            if (!reps[s]) {
+                slot = s;
+                break;
+            }
+        if (slot < 0) {
+            slot = slot_cap;
+            ensure_slots(slot_cap + 1);
+        }
+        reps[slot] = r;
+        int gi = -1;
+        for (size_t k = 0; k < groups.size(); ++k)
+            if (groups[k].config == r->config) gi = (int)k;
+        if (gi < 0) {
+            Group g;
+            g.config = r->config;
+            int dim, halo;
+            cpm_internal_tile_config(g.config, &dim, &halo, &g.tx, &g.ty, &g.tz);
+            groups.push_back(g);
+            gi = (int)groups.size() - 1;
+        }
+        r->group = gi;
+        groups[gi].slots.push_back(slot);
+        groups[gi].dirty = true;
+        upload_rep(slot);
+        set_dyn(slot, 0, 0, 1, 1 | 2 | 4 | 8);
+        metrics[4] = bytes();
+        drop_graph();
+        return slot;
+    }
+
+    // Keep the last-MCS report of every replica of g before its accounting arena moves.
+    void cache_reports(Group &g) {
+        for (int s : g.slots) {
+            Replica *r = reps[s];
+            if (!r || g.dirty_arena_fresh) continue;
+            uint64_t out[12];
+            try {
+                report(s, out);
+            } catch (const Failure &f) {
+                if (f.code != CPM_E_STATE) throw;
+            }
+        }
+    }
+
+    void remove(int slot) {
+        Replica *r = check_slot(slot);
+        hip_check(hipStreamSynchronize(stream), "sync");
+        Group &g = groups[r->group];
+        cache_reports(g);
+        g.slots.erase(std::remove(g.slots.begin(), g.slots.end(), slot), g.slots.end());
+        g.dirty = true;
+        set_dyn(slot, 0, 0, 0, 1 | 2 | 4 | 8);
+        hip_check(hipStreamSynchronize(stream), "sync");
+        delete r;
+        reps[slot] = nullptr;
+        drop_graph();
+    }
+
+    Replica *check_slot(int slot) {
+        if (slot < 0 || slot >= slot_cap || !reps[slot]) fail(CPM_E_INVALID, "no replica in this slot");
+        return reps[slot];
+    }
+
+    void rebuild(Group &g) {
+        cache_reports(g);
+        std::sort(g.slots.begin(), g.slots.end());
+        uint32_t tiles = 0, claims = 0;
+        int maxr = 1;
+        std::vector<uint32_t> tile_rep;
+        for (int s : g.slots) {
+            Replica *r = reps[s];
+            cpm::RepDev &v = r->dev;
+            v.tile_base = tiles;
+            v.tile_count = (uint32_t)v.ntiles[0] * (uint32_t)v.ntiles[1] * (uint32_t)v.ntiles[2];
+            v.claim_base = claims;
+            tiles += v.tile_count;
+            claims += r->desc.capacity + 1;
+            maxr = std::max(maxr, r->rounds);
+            tile_rep.insert(tile_rep.end(), v.tile_count, (uint32_t)s);
+        }
+        g.ntiles = tiles;
+        g.total_claims = std::max(claims, 1u);
+        g.seg_cap = (uint32_t)(g.tx * g.ty * g.tz);
+        g.max_rounds = maxr;
+        const size_t need_tiles = std::max<size_t>(tiles, 1);
+        const size_t need_slots = need_tiles * (size_t)maxr * cpm::kSlotFields;
+        const size_t need_seg = need_tiles * g.seg_cap;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        if (need_tiles > g.cap_tiles || 2 * (size_t)g.total_claims > g.cap_claims || need_slots > g.cap_slots ||
+            need_seg > g.cap_seg) {
+            g.release();
+            g.d_tile_rep = dev_alloc<uint32_t>(need_tiles, g.bytes);
+            g.d_seg_count = dev_alloc<uint32_t>(need_tiles, g.bytes);
+            g.d_claims = dev_alloc<unsigned long long>(2 * (size_t)g.total_claims, g.bytes);
+            g.d_slots = dev_alloc<uint32_t>(need_slots, g.bytes);
+            g.d_seg = dev_alloc<cpm::Entry>(need_seg, g.bytes);
+            g.cap_tiles = need_tiles;
+            g.cap_claims = 2 * (size_t)g.total_claims;
+            g.cap_slots = need_slots;
+            g.cap_seg = need_seg;
+        }
+        hip_check(hipMemsetAsync(g.d_claims, 0, 2 * (size_t)g.total_claims * 8, stream), "memset claims");
+        hip_check(hipMemsetAsync(g.d_slots, 0, need_slots * 4, stream), "memset slots");
+        hip_check(hipMemsetAsync(g.d_seg_count, 0, need_tiles * 4, stream), "memset seg");
+        if (tiles)
+            hip_check(hipMemcpy(g.d_tile_rep, tile_rep.data(), tiles * 4, hipMemcpyHostToDevice), "tile map");
+        for (int s : g.slots) upload_rep(s);
+        g.pass_counter = 0;
+        g.dirty = false;
+        g.dirty_arena_fresh = true;
+        metrics[4] = bytes();
+        drop_graph();
+    }
+
+    void rebuild_dirty() {
+        for (Group &g : groups)
+            if (g.dirty) rebuild(g);
+    }
+
+    cpm::PassArgs args(const Group &g, uint32_t pass, uint32_t parity) const {
+        cpm::PassArgs a{};
+        a.reps = d_reps;
+        a.dyn = d_dyn;
+        a.tile_rep = g.d_tile_rep;
+        a.claims = g.d_claims;
+        a.seg = g.d_seg;
+        a.seg_count = g.d_seg_count;
+        a.slots = g.d_slots;
+        a.ntiles = g.ntiles;
+        a.seg_cap = g.seg_cap;
+        a.pass = pass;
+        a.parity = parity;
+        a.total_claims = g.total_claims;
+        return a;
+    }
+
+    // Enqueue one MCS for every active replica (begin + 2 launches per pass).
+    void enqueue_mcs() {
+        hip_check(cpm_internal_launch_begin(d_reps, d_dyn, slot_cap, stream), "begin_mcs");
+        metrics[0]++;
+        for (Group &g : groups) {
+            if (g.ntiles == 0) continue;
+            for (int p = 0; p < g.max_rounds; ++p) {
+                const cpm::PassArgs a = args(g, (uint32_t)p, (uint32_t)(g.pass_counter & 1));
+                hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
+                metrics[0] += 2;
+                g.pass_counter++;
+            }
+        }
+    }
+
+    bool graph_parity_ok(uint32_t mcs) const {
+        for (const Group &g : groups) {
+            if (g.ntiles == 0) continue;
+            if ((g.pass_counter & 1) != 0) return false;
+            if (((uint64_t)mcs * (uint64_t)g.max_rounds) % 2 != 0) return false;
+        }
+        return true;
+    }
+
+    void capture(uint32_t mcs) {
+        drop_graph();
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        std::vector<uint64_t> saved;
+        for (Group &g : groups) saved.push_back(g.pass_counter);
+        try {
+            for (uint32_t i = 0; i < mcs; ++i) enqueue_mcs();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &graph), "end capture");
+        hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        for (size_t k = 0; k < groups.size(); ++k) groups[k].pass_counter = saved[k];
+        graph_mcs = mcs;
+        graph_valid = true;
+    }
+
+    void step(uint64_t n) {
+        rebuild_dirty();
+        for (Group &g : groups) g.dirty_arena_fresh = false;
+        while (n > 0) {
+            if (graph_chunk > 0) {
+                uint32_t chunk = graph_chunk;
+                bool even = true;
+                for (const Group &g : groups)
+                    if (g.ntiles && ((uint64_t)chunk * g.max_rounds) % 2) even = false;
+                if (!even) chunk *= 2;
+                if (n >= chunk && graph_parity_ok(chunk)) {
+                    if (!graph_valid || graph_mcs != chunk) capture(chunk);
+                    hip_check(hipGraphLaunch(graph_exec, stream), "graph launch");
+                    metrics[5]++;
+                    for (Group &g : groups) g.pass_counter += (uint64_t)chunk * g.max_rounds;
+                    n -= chunk;
+                    continue;
+                }
+            }
+            enqueue_mcs();
+            n -= 1;
+        }
+    }
+
+    void sync() {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        metrics[1]++;
+    }
+
+    uint64_t read_mcs(int slot) {
+        cpm::RepDyn y;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        hip_check(hipMemcpy(&y, d_dyn + slot, sizeof(uint64_t) * 2 + sizeof(uint32_t) * 2, hipMemcpyDeviceToHost),
+                  "read mcs");
+        metrics[1]++;
+        metrics[3]++;
+        return y.mcs;
+    }
+
+    void report(int slot, uint64_t out[12]) {
+        Replica *r = check_slot(slot);
+        const uint64_t mcs = read_mcs(slot);
+        if (r->report_cached && r->cached_mcs == mcs) {
+            std::memcpy(out, r->cached_report, sizeof(r->cached_report));
+            return;
+        }
+        if (mcs <= r->base_mcs) fail(CPM_E_STATE, "no MCS has completed since the state was set");
+        const Group &g = groups[r->group];
+        if (g.dirty_arena_fresh) fail(CPM_E_STATE, "replica accounting was discarded by a membership change");
+        hip_check(cpm_internal_launch_report(d_reps, g.d_slots, g.ntiles, slot, d_report, stream), "report");
+        metrics[0]++;
+        hip_check(hipMemcpyAsync(h_report, d_report, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 stream), "copy report");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        metrics[3]++;
+        const unsigned long long *s = h_report;
+        const uint64_t rounds = (uint64_t)r->rounds;
+        out[0] = mcs;
+        out[1] = rounds;
+        out[2] = algorithm == CPM_ALG_LOTTERY ? (uint64_t)r->nmut * rounds : s[0];
+        out[3] = s[0];
+        out[4] = s[3];
+        out[5] = s[1];
+        out[6] = s[2];
+        out[7] = 0;
+        out[8] = s[4];
+        out[9] = 0;
+        out[10] = s[5];
+        out[11] = s[6];
+        std::memcpy(r->cached_report, out, sizeof(r->cached_report));
+        r->cached_mcs = mcs;
+        r->report_cached = true;
+    }
+
+    void upload_state(int slot, const uint8_t *tags, const uint32_t *ids, const int32_t *fv,
+                      const int32_t *mv, const int64_t *bm, const uint8_t *active, const uint32_t *ctype,
+                      const double *tv, const double *vs, const int64_t *tb, const double *bs) {
+        Replica *r = check_slot(slot);
+        if (!tags || !ids || !fv || !bm || !active || !ctype)
+            fail(CPM_E_INVALID, "ownership, volume, boundary, active and type arrays are required");
+        const size_t n = (size_t)r->n, cap = r->desc.capacity, cap1 = cap + 1;
+        std::vector<int32_t> words(n);
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t id = ids[i];
+            if (id == 0 || id > 0x7fffffffu) fail(CPM_E_INVALID, "owner identity must be positive and fit Int32");
+            if (tags[i] == 1) {
+                if (id > cap) fail(CPM_E_INVALID, "cell owner exceeds the capacity");
+                words[i] = (int32_t)id;
+            } else if (tags[i] == 2) {
+                words[i] = -(int32_t)id;
+            } else {
+                fail(CPM_E_INVALID, "owner tag must identify a cell or medium domain");
+            }
+        }
+        std::vector<int32_t> vol(cap1, 0);
+        std::vector<int64_t> bnd(cap1, 0), tbv(cap1, 0);
+        std::vector<uint32_t> ct(cap1, 0);
+        std::vector<uint8_t> act(cap1, 0);
+        std::vector<double> tvv(cap1, 0.0), vsv(cap1, 0.0), bsv(cap1, 0.0);
+        for (size_t c = 0; c < cap; ++c) {
+            vol[c + 1] = fv[c];
+            bnd[c + 1] = bm[c];
+            ct[c + 1] = ctype[c];
+            act[c + 1] = active[c];
+            tvv[c + 1] = tv ? tv[c] : 0.0;
+            vsv[c + 1] = vs ? vs[c] : 0.0;
+            tbv[c + 1] = tb ? tb[c] : 0;
+            bsv[c + 1] = bs ? bs[c] : 0.0;
+        }
+        std::vector<int32_t> media(CPM_MAX_MEDIA, 0);
+        for (int i = 0; i < r->desc.n_media; ++i) media[i] = mv ? mv[i] : 0;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        hip_check(hipMemcpy(r->d_lattice, words.data(), n * 4, hipMemcpyHostToDevice), "upload lattice");
+        hip_check(hipMemcpy(r->d_vol, vol.data(), cap1 * 4, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_bnd, bnd.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_ctype, ct.data(), cap1 * 4, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_active, act.data(), cap1, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_tv, tvv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_vs, vsv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_tb, tbv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_bs, bsv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_mvol, media.data(), CPM_MAX_MEDIA * 4, hipMemcpyHostToDevice), "upload");
+        metrics[2]++;
+        r->base_mcs = read_mcs(slot);
+        r->report_cached = false;
+    }
+
+    void download_state(int slot, uint8_t *tags, uint32_t *ids, int32_t *fv, int32_t *mv, int64_t *bm,
+                        uint8_t *active, uint32_t *ctype, double *tv, double *vs, int64_t *tb, double *bs) {
+        Replica *r = check_slot(slot);
+        const size_t n = (size_t)r->n, cap = r->desc.capacity, cap1 = cap + 1;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        metrics[3]++;
+        if (tags || ids) {
+            std::vector<int32_t> words(n);
+            hip_check(hipMemcpy(words.data(), r->d_lattice, n * 4, hipMemcpyDeviceToHost), "download lattice");
+            for (size_t i = 0; i < n; ++i) {
+                const int32_t w = words[i];
+                if (tags) tags[i] = w > 0 ? 1 : 2;
+                if (ids) ids[i] = (uint32_t)(w > 0 ? w : -w);
+            }
+        }
+        auto get = [&](auto *out, const auto *dev, size_t elem) {
+            if (!out) return;
+            std::vector<uint8_t> tmp(cap1 * elem);
+            hip_check(hipMemcpy(tmp.data(), dev, cap1 * elem, hipMemcpyDeviceToHost), "download");
+            std::memcpy(out, tmp.data() + elem, cap * elem);
+        };
+        get(fv, r->d_vol, 4);
+        get(bm, r->d_bnd, 8);
+        get(active, r->d_active, 1);
+        get(ctype, r->d_ctype, 4);
+        get(tv, r->d_tv, 8);
+        get(vs, r->d_vs, 8);
+        get(tb, r->d_tb, 8);
+        get(bs, r->d_bs, 8);
+        if (mv && r->desc.n_media > 0)
+            hip_check(hipMemcpy(mv, r->d_mvol, (size_t)r->desc.n_media * 4, hipMemcpyDeviceToHost), "download");
+    }
+
+    void set_seed(int slot, uint64_t seed) {
+        Replica *r = check_slot(slot);
+        r->desc.seed = seed;
+        const uint64_t key = seed ^ mix64(kGenerationDomain);
+        r->dev.key_lo = (uint32_t)key;
+        r->dev.key_hi = (uint32_t)(key >> 32);
+        upload_rep(slot);
+    }
+};
+
+template <typename F>
+int32_t guarded(std::string *err, F &&fn) {
+    try {
+        fn();
+        return CPM_OK;
+    } catch (const Failure &f) {
+        if (err) *err = f.message;
+        return f.code;
+    } catch (const std::bad_alloc &) {
+        if (err) *err = "host allocation failed";
+        return CPM_E_NOMEM;
+    } catch (...) {
+        if (err) *err = "unexpected failure";
+        return CPM_E_INVALID;
+    }
+}
+
+}  // namespace
+
+struct cpm_ctx {
+    Engine engine;
+    int slot = 0;
+    uint64_t host_mcs = 0;
+};
+
+struct cpm_batch {
+    Engine engine;
+};
+
+extern "C" {
+
+const char *cpm_last_error(const cpm_ctx *ctx) {
+    return ctx ? ctx->engine.err.c_str() : g_create_error.c_str();
+}
+
+int32_t cpm_abi_version(void) { return CPM_ABI_VERSION; }
+
+int32_t cpm_create(const cpm_model_desc *desc, cpm_ctx **out) {
+    if (!desc || !out) {
+        g_create_error = "null argument";
+        return CPM_E_INVALID;
+    }
+    cpm_ctx *ctx = nullptr;
+    const int32_t rc = guarded(&g_create_error, [&] {
+        ctx = new cpm_ctx();
+        ctx->engine.init(desc->algorithm);
+        ctx->slot = ctx->engine.add(*desc);
+        ctx->engine.rebuild_dirty();
+        hip_check(hipStreamSynchronize(ctx->engine.stream), "sync");
+    });
+    if (rc != CPM_OK) {
+        delete ctx;
+        *out = nullptr;
+        return rc;
+    }
+    *out = ctx;
+    return CPM_OK;
+}
+
+int32_t cpm_destroy(cpm_ctx *ctx) {
+    delete ctx;
+    return CPM_OK;
+}
+
+int32_t cpm_upload_state(cpm_ctx *ctx, const uint8_t *tags, const uint32_t *ids,
+                         const int32_t *finite_volumes, const int32_t *medium_volumes,
+                         const int64_t *boundary_measures, const uint8_t *active,
+                         const uint32_t *cell_types, const double *target_volume,
+                         const double *volume_strength, const int64_t *target_boundary,
+                         const double *boundary_strength) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        ctx->engine.upload_state(ctx->slot, tags, ids, finite_volumes, medium_volumes, boundary_measures,
+                                 active, cell_types, target_volume, volume_strength, target_boundary,
+                                 boundary_strength);
+    });
+}
+
+int32_t cpm_download_state(cpm_ctx *ctx, uint8_t *tags, uint32_t *ids, int32_t *finite_volumes,
+                           int32_t *medium_volumes, int64_t *boundary_measures, uint8_t *active,
+                           uint32_t *cell_types, double *target_volume, double *volume_strength,
+                           int64_t *target_boundary, double *boundary_strength) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        ctx->engine.download_state(ctx->slot, tags, ids, finite_volumes, medium_volumes,
+                                   boundary_measures, active, cell_types, target_volume,
+                                   volume_strength, target_boundary, boundary_strength);
+    });
+}
+
+int32_t cpm_set_seed(cpm_ctx *ctx, uint64_t seed) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] { ctx->engine.set_seed(ctx->slot, seed); });
+}
+
+int32_t cpm_set_mcs(cpm_ctx *ctx, uint64_t mcs) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        ctx->engine.set_dyn(ctx->slot, mcs, mcs, 1, 1 | 2 | 8);
+        ctx->host_mcs = mcs;
+        Replica *r = ctx->engine.check_slot(ctx->slot);
+        r->base_mcs = mcs;
+        r->report_cached = false;
+    });
+}
+
+int32_t cpm_get_mcs(const cpm_ctx *ctx, uint64_t *mcs) {
+    if (!ctx || !mcs) return CPM_E_INVALID;
+    *mcs = ctx->host_mcs;
+    return CPM_OK;
+}
+
+int32_t cpm_step(cpm_ctx *ctx, uint64_t n) {
+    if (!ctx) return CPM_E_INVALID;
+    if (n == 0) return CPM_OK;
+    return guarded(&ctx->engine.err, [&] {
+        Engine &e = ctx->engine;
+        const uint64_t target = ctx->host_mcs + n;
+        e.set_dyn(ctx->slot, 0, target, 1, 2);
+        e.step(n);
+        ctx->host_mcs = target;
+    });
+}
+
+int32_t cpm_report(cpm_ctx *ctx, uint64_t out[12]) {
+    if (!ctx || !out) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] { ctx->engine.report(ctx->slot, out); });
+}
+
+int32_t cpm_sync(cpm_ctx *ctx) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] { ctx->engine.sync(); });
+}
+
+int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        Engine &e = ctx->engine;
+        Replica *r = e.check_slot(ctx->slot);
+        if (round_count) *round_count = r->rounds;
+        if (order) {
+            cpm::RepDyn y;
+            hip_check(hipStreamSynchronize(e.stream), "sync");
+            hip_check(hipMemcpy(&y, e.d_dyn + ctx->slot, sizeof(y), hipMemcpyDeviceToHost), "copy order");
+            e.metrics[1]++;
+            e.metrics[3]++;
+            std::memcpy(order, y.order, (size_t)r->rounds * 4);
+        }
+    });
+}
+
+int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors) {
+    if (!ctx || !colors) return CPM_E_INVALID;
+    const Replica *r = ctx->engine.reps[ctx->slot];
+    if (r->desc.algorithm != CPM_ALG_CHECKERBOARD) return CPM_E_STATE;
+    std::memcpy(colors, r->colors.data(), (size_t)r->n);
+    return CPM_OK;
+}
+
+int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]) {
+    if (!ctx || !out) return CPM_E_INVALID;
+    std::memcpy(out, ctx->engine.metrics, sizeof(ctx->engine.metrics));
+    return CPM_OK;
+}
+
+int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk) {
+    if (!ctx) return CPM_E_INVALID;
+    ctx->engine.graph_chunk = chunk;
+    ctx->engine.drop_graph();
+    return CPM_OK;
+}
+
+uint64_t cpm_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t rerun) {
+    // EnsembleStream = 16, operation 1, EnsembleEntity = 3, invocation = rerun - 1
+    const uint64_t c0 = (uint64_t)((rerun - 1) & 0xffu) << 56;
+    const uint64_t c1 = (uint64_t)trajectory | (1ull << 32) | (16ull << 44) | (3ull << 52);
+    const uint64_t key = master_seed ^ mix64(kGenerationDomain);
+    uint32_t c[4] = {(uint32_t)c0, (uint32_t)(c0 >> 32), (uint32_t)c1, (uint32_t)(c1 >> 32)};
+    philox_host(c, (uint32_t)key, (uint32_t)(key >> 32));
+    return (uint64_t)c[0] | ((uint64_t)c[1] << 32);
+}
+
+// ---------------- batch ----------------
+int32_t cpm_batch_create(int32_t algorithm, uint64_t hbm_budget_bytes, cpm_batch **out) {
+    if (!out) return CPM_E_INVALID;
+    if (algorithm != CPM_ALG_CHECKERBOARD && algorithm != CPM_ALG_LOTTERY) {
+        g_create_error = "unknown algorithm";
+        return CPM_E_INVALID;
+    }
+    cpm_batch *b = nullptr;
+    const int32_t rc = guarded(&g_create_error, [&] {
+        b = new cpm_batch();
+        b->engine.init(algorithm);
+        b->engine.budget = hbm_budget_bytes;
+    });
+    if (rc != CPM_OK) {
+        delete b;
+        *out = nullptr;
+        return rc;
+    }
+    *out = b;
+    return CPM_OK;
+}
+
+int32_t cpm_batch_destroy(cpm_batch *b) {
+    delete b;
+    return CPM_OK;
+}
+
+const char *cpm_batch_last_error(const cpm_batch *b) {
+    return b ? b->engine.err.c_str() : g_create_error.c_str();
+}
+
+int32_t cpm_replica_bytes(const cpm_model_desc *desc, uint64_t *bytes) {
+    if (!desc || !bytes) return CPM_E_INVALID;
+    *bytes = estimate_replica_bytes(*desc);
+    return CPM_OK;
+}
+
+int32_t cpm_batch_add(cpm_batch *b, const cpm_model_desc *desc, int32_t *slot) {
+    if (!b || !desc || !slot) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { *slot = b->engine.add(*desc); });
+}
+
+int32_t cpm_batch_remove(cpm_batch *b, int32_t slot) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.remove(slot); });
+}
+
+int32_t cpm_batch_upload_state(cpm_batch *b, int32_t slot, const uint8_t *tags,
+                               const uint32_t *ids, const int32_t *finite_volumes,
+                               const int32_t *medium_volumes, const int64_t *boundary_measures,
+                               const uint8_t *active, const uint32_t *cell_types,
+                               const double *target_volume, const double *volume_strength,
+                               const int64_t *target_boundary, const double *boundary_strength) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] {
+        b->engine.upload_state(slot, tags, ids, finite_volumes, medium_volumes, boundary_measures, active,
+                               cell_types, target_volume, volume_strength, target_boundary,
+                               boundary_strength);
+    });
+}
+
+int32_t cpm_batch_download_state(cpm_batch *b, int32_t slot, uint8_t *tags, uint32_t *ids,
+                                 int32_t *finite_volumes, int32_t *medium_volumes,
+                                 int64_t *boundary_measures, uint8_t *active,
+                                 uint32_t *cell_types, double *target_volume,
+                                 double *volume_strength, int64_t *target_boundary,
+                                 double *boundary_strength) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] {
+        b->engine.download_state(slot, tags, ids, finite_volumes, medium_volumes, boundary_measures,
+                                 active, cell_types, target_volume, volume_strength, target_boundary,
+                                 boundary_strength);
+    });
+}
+
+int32_t cpm_batch_set_target(cpm_batch *b, int32_t slot, uint64_t target_mcs) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] {
+        b->engine.check_slot(slot);
+        b->engine.set_dyn(slot, 0, target_mcs, 1, 2);
+    });
+}
+
+int32_t cpm_batch_step(cpm_batch *b, uint64_t n) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.step(n); });
+}
+
+int32_t cpm_batch_progress(cpm_batch *b, uint64_t *out, int32_t capacity) {
+    if (!b || !out) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] {
+        Engine &e = b->engine;
+        std::vector<cpm::RepDyn> y((size_t)e.slot_cap);
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+        if (e.slot_cap)
+            hip_check(hipMemcpy(y.data(), e.d_dyn, (size_t)e.slot_cap * sizeof(cpm::RepDyn),
+                                hipMemcpyDeviceToHost), "progress");
+        e.metrics[1]++;
+        e.metrics[3]++;
+        for (int s = 0; s < capacity; ++s)
+            out[s] = (s < e.slot_cap && e.reps[s]) ? y[s].mcs : UINT64_MAX;
+    });
+}
+
+int32_t cpm_batch_capacity(const cpm_batch *b) { return b ? b->engine.slot_cap : 0; }
+
+int32_t cpm_batch_report(cpm_batch *b, int32_t slot, uint64_t out[12]) {
+    if (!b || !out) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.report(slot, out); });
+}
+
+int32_t cpm_batch_sync(cpm_batch *b) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.sync(); });
+}
+
+int32_t cpm_batch_set_graph_chunk(cpm_batch *b, uint32_t chunk) {
+    if (!b) return CPM_E_INVALID;
+    b->engine.graph_chunk = chunk;
+    b->engine.drop_graph();
+    return CPM_OK;
+}
+
+int32_t cpm_batch_bytes(const cpm_batch *b, uint64_t *bytes) {
+    if (!b || !bytes) return CPM_E_INVALID;
+    *bytes = b->engine.bytes();
+    return CPM_OK;
+}
+
+}  // extern "C"

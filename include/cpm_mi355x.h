@@ -1,0 +1,212 @@
+/*
+ * cpm_mi355x: C-ABI of the MI355X (gfx950) engine for CorePotts' data-parallel
+ * Metropolis sweeps (CheckerboardSweepCPM and LotteryCPM), their Philox4x32-10
+ * semantic RNG, copy-attempt energy evaluation and volume/boundary tracker updates.
+ *
+ * Plain pointers and sizes only. Every function returns 0 on success and a
+ * nonzero CPM_E_* code otherwise; the message is available from cpm_last_error().
+ * No C++ exception crosses this boundary. A context owns its device memory and
+ * one HIP stream; calls on one context are not thread-safe, distinct contexts are.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   cpm_create          init_scientific(state, proposal_relation, components, alg; seed)
+ *                       lib/CorePotts/src/algorithms/checkerboard.jl:197-219,
+ *                       lib/CorePotts/src/algorithms/lottery.jl:94-115 (workspace
+ *                       allocation and realized colouring/conflict graph :159-195 / :46-92)
+ *   cpm_upload_state    Adapt.adapt(backend, compile_scientific_state(...)) host->device copy
+ *                       lib/CorePotts/src/components/scientific_trackers.jl:210-228,
+ *                       lib/CorePotts/src/sciml/interface.jl:607-631
+ *   cpm_step            SciMLBase.step!(integrator, n) -> perform_scientific_mcs!
+ *                       lib/CorePotts/src/algorithms/sequential.jl:599-609,
+ *                       checkerboard.jl:564-575, lottery.jl:328-395 (asynchronous, no host sync)
+ *   cpm_report          current_mcs_report(integrator) -> ScientificMCSReport (12 x UInt64)
+ *                       sequential.jl:236-251,611-623; checkerboard.jl:577-586; lottery.jl:397-415
+ *   cpm_download_state  logical_state(integrator) / logical_snapshot(plan, state)
+ *                       sequential.jl:630-632; lib/CorePotts/src/execution/contracts.jl:420-431
+ *   cpm_sync            synchronize_observation!(plan)  contracts.jl:366-370
+ *   cpm_metrics         ExecutionMetrics counters       contracts.jl:306-316,372-396
+ *   cpm_set_seed/mcs    integrator.seed / integrator.mcs fields (sequential.jl:216-234)
+ *   cpm_schedule        workspace.color_count/color_order (checkerboard.jl:1-15),
+ *                       workspace.round_count/round_order (lottery.jl:2-17)
+ *   cpm_ensemble_seed   ensemble_seed(EnsembleSeedDerivationV1(), master, trajectory, rerun)
+ *                       lib/CorePotts/src/sciml/interface.jl:1009-1026
+ *   cpm_batch_*         no single counterpart: many independent integrators (the reference's
+ *                       EnsembleProblem trajectories, interface.jl:1059-1084) advanced by one
+ *                       fused launch per colour pass.
+ *
+ * Data layout follows the reference's compiled state: lattice arrays are column-major
+ * (x fastest), site s (1-based) has 0-based index s-1; owner tags 1 = cell, 2 = medium;
+ * per-cell arrays have `capacity` entries where entry c-1 belongs to cell id c.
+ */
+#ifndef CPM_MI355X_H
+#define CPM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPM_ABI_VERSION 1
+#define CPM_MAX_DIRS 128
+#define CPM_MAX_TYPES 16
+#define CPM_MAX_MEDIA 16
+
+enum cpm_status {
+    CPM_OK = 0,
+    CPM_E_INVALID = 1,      /* invalid argument or model */
+    CPM_E_UNSUPPORTED = 2,  /* valid in the reference but outside this engine */
+    CPM_E_DEVICE = 3,       /* HIP runtime error */
+    CPM_E_NOMEM = 4,        /* allocation failed or budget exceeded */
+    CPM_E_STATE = 5         /* call not valid in the current state */
+};
+
+enum cpm_face { CPM_FACE_PERIODIC = 0, CPM_FACE_CLOSED = 1, CPM_FACE_FIXED = 2 };
+enum cpm_term { CPM_TERM_VOLUME = 0, CPM_TERM_CONTACT = 1, CPM_TERM_BOUNDARY = 2 };
+enum cpm_algorithm { CPM_ALG_CHECKERBOARD = 1, CPM_ALG_LOTTERY = 2 };
+
+/* StaticCartesianRelation (lib/CorePotts/src/spatial/cartesian.jl:252-260), already in
+ * canonical direction order; weights hold the values of the relation's eltype. */
+typedef struct cpm_relation_desc {
+    int32_t count;
+    int32_t offsets[CPM_MAX_DIRS][3];
+    double weights[CPM_MAX_DIRS];
+} cpm_relation_desc;
+
+/* Everything init_scientific receives besides the state arrays. */
+typedef struct cpm_model_desc {
+    int32_t abi_version;           /* CPM_ABI_VERSION */
+    int32_t algorithm;             /* CPM_ALG_* */
+    int32_t ndim;                  /* 2 or 3 */
+    int32_t dims[3];
+    int32_t face_kind[3][2];       /* CPM_FACE_* per axis, [0] negative [1] positive face */
+    uint32_t face_medium[3][2];    /* medium owner id of a fixed-exterior face */
+    const uint8_t *mutable_mask;   /* NULL: every site mutable; else N bytes (1 = mutable) */
+    const uint8_t *immutable_tags; /* obstacle owner tags/ids, read where mask == 0 */
+    const uint32_t *immutable_ids;
+
+    cpm_relation_desc proposal;    /* ProposalRole relation */
+    cpm_relation_desc contact;     /* UnorderedContactHamiltonian relation */
+    cpm_relation_desc surface;     /* BoundaryMeasureTracker relation (BoundaryEdgeCount) */
+
+    int32_t n_energies;            /* energies in ScientificComponentSet tuple order */
+    int32_t energy_order[3];       /* CPM_TERM_* */
+    int32_t volume_f64;            /* arithmetic type of each term: 0 Float32, 1 Float64 */
+    int32_t contact_f64;
+    int32_t boundary_f64;
+
+    int32_t n_types;               /* contact matrix J (row-major, 1-based type ids) */
+    double J[CPM_MAX_TYPES * CPM_MAX_TYPES];
+    int32_t n_table;               /* MediumTypeTable: medium owner id -> type id */
+    uint32_t table_owner[CPM_MAX_MEDIA];
+    uint32_t table_type[CPM_MAX_MEDIA];
+    int32_t n_media;               /* state medium_ids (order of the medium volume tracker) */
+    uint32_t medium_ids[CPM_MAX_MEDIA];
+
+    float temperature;             /* Float32 algorithm temperature */
+    uint32_t capacity;             /* cell slots */
+    uint64_t seed;
+    double retire_target_volume;   /* retirement defaults of the property columns */
+    double retire_volume_strength;
+    int64_t retire_target_boundary;
+    double retire_boundary_strength;
+} cpm_model_desc;
+
+typedef struct cpm_ctx cpm_ctx;
+
+/* Message of the last failing call on ctx (or of the last failing cpm_create when ctx is NULL). */
+const char *cpm_last_error(const cpm_ctx *ctx);
+int32_t cpm_abi_version(void);
+
+/* Compile schedules (realized colouring / conflict graph) and allocate device state. */
+int32_t cpm_create(const cpm_model_desc *desc, cpm_ctx **out);
+int32_t cpm_destroy(cpm_ctx *ctx);
+
+/* Host -> device copy of the compiled state (arrays sized N, capacity or n_media). */
+int32_t cpm_upload_state(cpm_ctx *ctx, const uint8_t *tags, const uint32_t *ids,
+                         const int32_t *finite_volumes, const int32_t *medium_volumes,
+                         const int64_t *boundary_measures, const uint8_t *active,
+                         const uint32_t *cell_types, const double *target_volume,
+                         const double *volume_strength, const int64_t *target_boundary,
+                         const double *boundary_strength);
+/* Device -> host copy (synchronizes). Any output pointer may be NULL. */
+int32_t cpm_download_state(cpm_ctx *ctx, uint8_t *tags, uint32_t *ids, int32_t *finite_volumes,
+                           int32_t *medium_volumes, int64_t *boundary_measures, uint8_t *active,
+                           uint32_t *cell_types, double *target_volume, double *volume_strength,
+                           int64_t *target_boundary, double *boundary_strength);
+
+int32_t cpm_set_seed(cpm_ctx *ctx, uint64_t seed);
+int32_t cpm_set_mcs(cpm_ctx *ctx, uint64_t mcs);
+int32_t cpm_get_mcs(const cpm_ctx *ctx, uint64_t *mcs);
+
+/* Enqueue n MCS on the context stream; returns without synchronizing. */
+int32_t cpm_step(cpm_ctx *ctx, uint64_t n);
+/* Report of the last completed MCS (one launch + one synchronization). Returns
+ * CPM_E_STATE when no MCS has run since the state was set. */
+int32_t cpm_report(cpm_ctx *ctx, uint64_t out[12]);
+int32_t cpm_sync(cpm_ctx *ctx);
+
+/* Colour (checkerboard) or round (lottery) count, and the order used by the last MCS.
+ * order may be NULL; it needs round_count entries. Synchronizes. */
+int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order);
+/* Realized colour of every site (0 for immutable sites), N bytes. Checkerboard only. */
+int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors);
+
+/* ExecutionMetrics mirror: [launches, host_synchronizations, h2d_transfers,
+ * d2h_transfers, device_bytes_allocated, graph_launches]. */
+int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]);
+
+/* Capture chunks of `chunk` MCS into a HIP graph that cpm_step replays when n is a
+ * multiple of chunk (0 disables graphs). */
+int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk);
+
+/* Philox-derived replica seed (sciml/interface.jl:1009-1026). */
+uint64_t cpm_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t rerun);
+
+/* ---------------- batched replicas and served jobs ----------------
+ * A batch holds many independent replicas (possibly of different sizes, temperatures,
+ * contact matrices and algorithms of one kind) and advances every replica that has not
+ * reached its target MCS with one fused launch per colour pass. Replicas can be added
+ * and removed between steps (continuous batching); each replica's trajectory is
+ * identical to a single-context run with the same model, state and seed. */
+typedef struct cpm_batch cpm_batch;
+
+int32_t cpm_batch_create(int32_t algorithm, uint64_t hbm_budget_bytes, cpm_batch **out);
+int32_t cpm_batch_destroy(cpm_batch *b);
+const char *cpm_batch_last_error(const cpm_batch *b);
+/* Device bytes a replica of this model would hold (for admission by budget). */
+int32_t cpm_replica_bytes(const cpm_model_desc *desc, uint64_t *bytes);
+/* Add a replica; fails with CPM_E_NOMEM when the budget would be exceeded. */
+int32_t cpm_batch_add(cpm_batch *b, const cpm_model_desc *desc, int32_t *slot);
+int32_t cpm_batch_remove(cpm_batch *b, int32_t slot);
+int32_t cpm_batch_upload_state(cpm_batch *b, int32_t slot, const uint8_t *tags,
+                               const uint32_t *ids, const int32_t *finite_volumes,
+                               const int32_t *medium_volumes, const int64_t *boundary_measures,
+                               const uint8_t *active, const uint32_t *cell_types,
+                               const double *target_volume, const double *volume_strength,
+                               const int64_t *target_boundary, const double *boundary_strength);
+int32_t cpm_batch_download_state(cpm_batch *b, int32_t slot, uint8_t *tags, uint32_t *ids,
+                                 int32_t *finite_volumes, int32_t *medium_volumes,
+                                 int64_t *boundary_measures, uint8_t *active,
+                                 uint32_t *cell_types, double *target_volume,
+                                 double *volume_strength, int64_t *target_boundary,
+                                 double *boundary_strength);
+/* The replica advances until its MCS counter reaches target_mcs. */
+int32_t cpm_batch_set_target(cpm_batch *b, int32_t slot, uint64_t target_mcs);
+/* Enqueue up to n MCS for every replica below its target (asynchronous). */
+int32_t cpm_batch_step(cpm_batch *b, uint64_t n);
+/* Current MCS of every slot (synchronizes); out needs cpm_batch_capacity entries,
+ * UINT64_MAX marks free slots. */
+int32_t cpm_batch_progress(cpm_batch *b, uint64_t *out, int32_t capacity);
+int32_t cpm_batch_capacity(const cpm_batch *b);
+int32_t cpm_batch_report(cpm_batch *b, int32_t slot, uint64_t out[12]);
+int32_t cpm_batch_sync(cpm_batch *b);
+int32_t cpm_batch_set_graph_chunk(cpm_batch *b, uint32_t chunk);
+/* Resident device bytes of the batch (replica state + shared scratch). */
+int32_t cpm_batch_bytes(const cpm_batch *b, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

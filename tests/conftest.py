@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for path in (ROOT, os.path.join(ROOT, "celly.jl_amd"), os.path.join(ROOT, "tests")):
+    if path not in sys.path:
+        sys.path.insert(0, path)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
+
+
+@pytest.fixture(scope="session")
+def gpu_lib():
+    """Loads the HIP engine; GPU tests must run where a device exists."""
+    import torch  # noqa: F401  (device plumbing only)
+    from corepotts import load_library
+    lib = load_library()
+    return lib

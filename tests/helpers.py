@@ -1,0 +1,125 @@
+"""Shared builders for the parity tests.
+
+phase13_fixture() rebuilds the tiny transition fixtures of the reference's Phase 13
+evidence exactly as integration/transition/Phase13Fixtures.jl:121-190 does (Float32
+spacing, relation weights, volume columns and contact matrix; energies (volume, contact);
+BoundaryEdgeCount tracker only). scientific_fixture() restates the 4x4 / 4x4x4 fixture of
+lib/CorePotts/test/test_scientific_hamiltonians.jl:1-45.
+"""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+
+from corepotts import (AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
+                       CellOwner, ClosedBoundary, ContactRole, LogicalPottsState, MediumOwner,
+                       MediumTypeTable, PeriodicBoundary, ProposalRole, QuadraticBoundaryHamiltonian,
+                       QuadraticVolumeHamiltonian, ScientificComponentSet, SurfaceRole,
+                       UnorderedContactHamiltonian, build_spec, compile_scientific_state,
+                       first_shell_relation, required_properties, state_arrays, static_relation,
+                       von_neumann_offsets, moore_offsets)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_phase13():
+    with open(os.path.join(GOLDEN, "phase13_transition_counts.json")) as handle:
+        return json.load(handle)
+
+
+def _rational(text) -> float:
+    return float(Fraction(str(text)))
+
+
+def parse_owner_encoding(encoding: str):
+    owners = []
+    for token in encoding.split(","):
+        kind, ident = token.strip().split(":")
+        owners.append(CellOwner(int(ident)) if kind == "cell" else MediumOwner(int(ident)))
+    return owners
+
+
+def phase13_fixture(row: dict, model: dict, algorithm):
+    """Returns (compiled, proposal, components, algorithm_value, labels)."""
+    dims = tuple(row["dimensions"])
+    ndim = len(dims)
+    f32 = np.float32
+    spacing = tuple(f32(1.0) for _ in dims)
+    offsets = von_neumann_offsets(ndim) if row["topology"] == "von_neumann" else moore_offsets(ndim)
+    proposal = static_relation(ProposalRole, offsets, spacing=spacing)
+    contact_rel = static_relation(ContactRole, offsets, spacing=spacing)
+    surface = static_relation(SurfaceRole, offsets, spacing=spacing)
+    bmap = {"periodic": PeriodicBoundary, "no_flux": ClosedBoundary}
+    domain = CartesianDomain(dims, spacing=spacing,
+                             boundaries=tuple(AxisBoundary(bmap[b]()) for b in row["boundaries"]))
+    owners = parse_owner_encoding(row["source_state_encoding"])
+    labels = []
+    for o in owners:
+        if o not in labels:
+            labels.append(o)
+    cell_ids = sorted({o for o in labels if o > 0})
+    medium_ids = sorted({-o for o in labels if o < 0} | {model["medium_owner_id"]})
+    for m in medium_ids:
+        if -m not in labels:
+            labels.append(-m)
+    scale = _rational(row["energy_scale"])
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    base = np.array([_rational(v) for v in model["contact_matrix_base_row_major"]], dtype=np.float32)
+    J = base.reshape(2, 2) * np.float32(scale)
+    contact = UnorderedContactHamiltonian(J.astype(np.float32),
+                                          MediumTypeTable({m: model["medium_type_id"] for m in medium_ids}),
+                                          contact_rel)
+    state = LogicalPottsState(np.array(owners, dtype=np.int64), model["cell_capacity"],
+                              cell_types={c: model["cell_type_id"] for c in cell_ids},
+                              medium_domains=medium_ids, property_schema=required_properties(volume),
+                              dims=dims)
+    for c in cell_ids:
+        state.property_values("target_volume")[c - 1] = np.float32(_rational(model["target_volume"]))
+        state.property_values("volume_strength")[c - 1] = np.float32(scale)
+    tracker = BoundaryMeasureTracker(BoundaryEdgeCount(), surface)
+    compiled = compile_scientific_state(state, domain, tracker)
+    components = ScientificComponentSet(energies=(volume, contact))
+    alg = algorithm(np.float32(_rational(row["temperature"])))
+    return compiled, proposal, components, alg, labels
+
+
+def scientific_fixture(dims, dtype=np.float32):
+    """test_scientific_hamiltonians.jl:1-45 (volume, contact, boundary energies)."""
+    ndim = len(dims)
+    T = dtype
+    spacing = tuple(T(1.0) for _ in dims)
+    volume = QuadraticVolumeHamiltonian(number_type=T)
+    surface = first_shell_relation(SurfaceRole, ndim, spacing=spacing)
+    boundary = QuadraticBoundaryHamiltonian(BoundaryEdgeCount(), surface, number_type=T)
+    schema = {}
+    schema.update(required_properties(volume))
+    schema.update(required_properties(boundary))
+    owners = np.full(dims, MediumOwner(1), dtype=np.int64)
+    if ndim == 2:
+        owners[0, 0] = owners[1, 0] = owners[0, 1] = CellOwner(1)
+        owners[-1, -1] = owners[-2, -1] = owners[-1, -2] = CellOwner(2)
+    else:
+        owners[0, 0, 0] = owners[1, 0, 0] = owners[0, 1, 0] = CellOwner(1)
+        owners[-1, -1, -1] = owners[-2, -1, -1] = owners[-1, -2, -1] = CellOwner(2)
+    state = LogicalPottsState(owners, 2, cell_types={1: 2, 2: 2}, medium_domains=(1,),
+                              property_schema=schema)
+    state.property_values("target_volume")[:] = T(4)
+    state.property_values("volume_strength")[:] = T(1.25)
+    state.property_values("target_boundary")[:] = 8
+    state.property_values("boundary_strength")[:] = T(0.75)
+    domain = CartesianDomain(dims, spacing=spacing)
+    proposal = first_shell_relation(ProposalRole, ndim, spacing=domain.spacing)
+    contact_rel = first_shell_relation(ContactRole, ndim, spacing=domain.spacing)
+    contact = UnorderedContactHamiltonian(np.array([[0, 3], [3, 1]], dtype=T),
+                                          MediumTypeTable({1: 1}), contact_rel)
+    tracker = BoundaryMeasureTracker(BoundaryEdgeCount(), surface)
+    compiled = compile_scientific_state(state, domain, tracker)
+    components = ScientificComponentSet(energies=(volume, contact, boundary))
+    return compiled, proposal, components
+
+
+def oracle_for(compiled, proposal, components, algorithm, seed, alg_override=None):
+    from oracle.bridge import OracleSim
+    spec = build_spec(compiled, proposal, components, algorithm, seed)
+    return OracleSim(spec, state_arrays(compiled, spec), alg_override), spec
