@@ -247,6 +247,24 @@ void compile_replica(Replica &r) {
         fail(CPM_E_INVALID, "temperature must be finite and non-negative");
     if (d.capacity < 1 || d.capacity > 0x7ffffffeu) fail(CPM_E_INVALID, "capacity must be 1..2^31-2");
 
+    if (contact) {
+        // every medium a contact relation can see needs a declared type (scientific_components.jl:35-40)
+        auto typed = [&](uint32_t medium) {
+            for (int i = 0; i < d.n_table; ++i)
+                if (d.table_owner[i] == medium) return d.table_type[i] >= 1 && (int)d.table_type[i] <= d.n_types;
+            return false;
+        };
+        for (int i = 0; i < d.n_media; ++i)
+            if (!typed(d.medium_ids[i])) fail(CPM_E_INVALID, "medium owner has no declared type");
+        for (int a = 0; a < d.ndim; ++a)
+            for (int f = 0; f < 2; ++f)
+                if (d.face_kind[a][f] == CPM_FACE_FIXED && !typed(d.face_medium[a][f]))
+                    fail(CPM_E_INVALID, "fixed-exterior owner has no declared type");
+        if (d.mutable_mask)
+            for (int64_t i = 0; i < r.n; ++i)
+                if (!d.mutable_mask[i] && !typed(d.immutable_ids[i]))
+                    fail(CPM_E_INVALID, "obstacle owner has no declared type");
+    }
     r.halo = std::max(max_abs_offset(d.proposal, d.ndim), max_abs_offset(d.surface, d.ndim));
     if (contact) r.halo = std::max(r.halo, max_abs_offset(d.contact, d.ndim));
     if (r.halo > 2) fail(CPM_E_UNSUPPORTED, "relations reaching beyond distance 2 are not supported");
@@ -936,6 +954,10 @@ struct Engine {
             tbv[c + 1] = tb ? tb[c] : 0;
             bsv[c + 1] = bs ? bs[c] : 0.0;
         }
+        if (has_term(r->desc, CPM_TERM_CONTACT))
+            for (size_t c = 0; c < cap; ++c)
+                if (active[c] && (ctype[c] < 1 || ctype[c] > (uint32_t)r->desc.n_types))
+                    fail(CPM_E_INVALID, "an owner type lies outside the contact interaction matrix");
         std::vector<int32_t> media(CPM_MAX_MEDIA, 0);
         for (int i = 0; i < r->desc.n_media; ++i) media[i] = mv ? mv[i] : 0;
         hip_check(hipStreamSynchronize(stream), "sync");

@@ -492,11 +492,17 @@ __device__ __forceinline__ bool neighbour_owner(const RepDev &d, int x, int y, i
     return w != 0;
 }
 
+// Cell type or declared medium type, clamped into 1..n_types (the host rejects models
+// and states that could reach an undeclared type).
 __device__ __forceinline__ uint32_t owner_type(const RepDev &d, int32_t w) {
-    if (w > 0) return d.ctype[w];
-    for (int i = 0; i < d.n_table; ++i)
-        if (d.table_word[i] == w) return d.table_type[i];
-    return 0;
+    uint32_t t = 1;
+    if (w > 0) {
+        t = d.ctype[w];
+    } else {
+        for (int i = 0; i < d.n_table; ++i)
+            if (d.table_word[i] == w) t = d.table_type[i];
+    }
+    return min(max(t, 1u), (uint32_t)d.n_types);
 }
 
 __device__ __forceinline__ int medium_slot(const RepDev &d, int32_t w) {

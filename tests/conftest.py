@@ -16,7 +16,6 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def gpu_lib():
     """Loads the HIP engine; GPU tests must run where a device exists."""
-    import torch  # noqa: F401  (device plumbing only)
     from corepotts import load_library
     lib = load_library()
     return lib
