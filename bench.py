@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Headline benchmark: CheckerboardSweepCPM copy attempts per second on one MI355X.
+
+Workload (one "step" = one MCS of the whole lattice): 4096 x 4096 periodic Int32 lattice,
+163,840 cells of 10 x 10 seeded on a pitch-10 slot grid (rows filled in order), two cell
+types, integer contact J, volume constraint (lambda 1, target 100), boundary constraint
+(lambda 1, target 40), Moore contact relation, von Neumann proposal, T = 10, seed 1234.
+Random numbers are the reference's Philox semantic addresses, so the run is synthetic but
+fully deterministic. Timed region: `--steps` MCS after `--warmup` MCS, synchronized on both
+sides. Attempts per MCS = activated_attempts of the report (one per mutable site).
+
+Roofline: the dominant kernel is the colour-pass propose kernel. Its algorithmic traffic is
+one read of the Int32 lattice per launch (4 bytes x sites); its average duration comes from
+HIP events recorded around every kernel on the engine stream (cpm_step_timed).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "celly.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "copy attempts/sec + % HBM roofline, 1 GPU; batched vs plain; p99 job latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per propose launch from the committed PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "cb_propose_4096_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as handle:
+        return json.load(handle).get("bytes_per_launch")
+
+
+def cpu_baseline(sample_mcs: int):
+    """Single-threaded C oracle on the same workload (bounded sample)."""
+    from corepotts import build_spec, state_arrays, workloads
+    from oracle.bridge import OracleSim
+    compiled, proposal, components, alg, seed = workloads.config2_large()
+    spec = build_spec(compiled, proposal, components, alg, seed)
+    sim = OracleSim(spec, state_arrays(compiled, spec))
+    t0 = time.perf_counter()
+    sim.step(sample_mcs)
+    dt = time.perf_counter() - t0
+    rep = sim.report()
+    attempts = rep[3] * sample_mcs
+    return {"value": attempts / dt, "unit": "copy attempts/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_mcs} MCS of the 4096x4096 checkerboard workload from its initial "
+                      f"state, C oracle (single thread), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--graph-chunk", type=int, default=50)
+    ap.add_argument("--timing-mcs", type=int, default=40)
+    ap.add_argument("--lottery-steps", type=int, default=50)
+    ap.add_argument("--cpu-sample-mcs", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lottery", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this engine runs one lattice (or one batch of "
+              "replicas) per GPU with no sharding or collectives; run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from corepotts import init_scientific, workloads
+
+    log("building 4096x4096 workload")
+    compiled, proposal, components, alg, seed = workloads.config2_large()
+    n_sites = int(compiled.tags.size)
+    log("compiling schedule and uploading")
+    integ = init_scientific(compiled, proposal, components, alg, seed=seed)
+    integ.set_graph_chunk(args.graph_chunk)
+    log(f"warmup {args.warmup} MCS")
+    integ.step(args.warmup)
+    integ.synchronize()
+
+    log(f"timed {args.steps} MCS")
+    t0 = time.perf_counter()
+    integ.step(args.steps)
+    integ.synchronize()
+    elapsed = time.perf_counter() - t0
+    rep = integ.current_mcs_report()
+    attempts_per_mcs = rep.activated_attempts
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = attempts_per_mcs * args.steps / elapsed
+
+    log(f"kernel timing over {args.timing_mcs} MCS")
+    timing = integ.step_timed(args.timing_mcs)
+    propose_avg_ms = timing["propose_ms"] / max(timing["propose_launches"], 1)
+    commit_avg_ms = timing["commit_ms"] / max(timing["commit_launches"], 1)
+    bytes_per_launch = 4.0 * n_sites
+    achieved = bytes_per_launch / (propose_avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    rep_t = integ.current_mcs_report()
+    passes = rep_t.internal_rounds
+    mcs_bytes = 4.0 * passes * n_sites + rep_t.accepted_copies * (4 + 16 + 32)
+
+    extra = {
+        "mcs_per_s": args.steps / elapsed,
+        "colour_passes_per_mcs": passes,
+        "propose_kernel_avg_us": propose_avg_ms * 1e3,
+        "commit_kernel_avg_us": commit_avg_ms * 1e3,
+        "event_wall_ms_per_mcs": timing["wall_ms"] / max(timing["mcs"], 1),
+        "algorithmic_bytes_per_mcs": mcs_bytes,
+        "whole_mcs_algorithmic_GBps": mcs_bytes * (args.steps / elapsed) / 1e9,
+        "last_report": rep._asdict(),
+    }
+    del integ
+
+    if not args.no_lottery:
+        log(f"lottery {args.lottery_steps} MCS")
+        compiled, proposal, components, lalg, lseed = workloads.config2_large(algorithm="lottery")
+        lot = init_scientific(compiled, proposal, components, lalg, seed=lseed)
+        lot.step(5)
+        lot.synchronize()
+        t0 = time.perf_counter()
+        lot.step(args.lottery_steps)
+        lot.synchronize()
+        dt = time.perf_counter() - t0
+        lrep = lot.current_mcs_report()
+        extra["lottery"] = {"copy_attempts_per_s": lrep.activated_attempts * args.lottery_steps / dt,
+                            "mcs_per_s": args.lottery_steps / dt, "rounds": lrep.internal_rounds,
+                            "last_report": lrep._asdict()}
+        del lot
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle)")
+        baseline = cpu_baseline(args.cpu_sample_mcs)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "copy attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+fp32",
+        "data": "synthetic (deterministic seeded layout, Philox semantic RNG)",
+        "config": {"workload": "CheckerboardSweepCPM 4096x4096 periodic, 163840 cells 10x10, "
+                               "volume+Moore contact+boundary, T=10, seed 1234",
+                   "lattice": [4096, 4096], "cells": 163840, "algorithm": "CheckerboardSweepCPM"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": baseline,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -53,7 +53,8 @@ class CpmModelDesc(C.Structure):
 EXPORTS = (
     "cpm_last_error", "cpm_abi_version", "cpm_create", "cpm_destroy", "cpm_upload_state",
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
-    "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_set_graph_chunk",
+    "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
+    "cpm_set_graph_chunk",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_step", "cpm_batch_progress",
@@ -93,6 +94,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_site_colors.argtypes = [P, P]
     lib.cpm_metrics.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.cpm_set_graph_chunk.argtypes = [P, C.c_uint32]
+    lib.cpm_step_timed.argtypes = [P, C.c_uint64, C.POINTER(C.c_double)]
     lib.cpm_ensemble_seed.restype = C.c_uint64
     lib.cpm_ensemble_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
     lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
@@ -330,6 +332,14 @@ class ScientificPottsIntegrator:
         keys = ("launches", "host_synchronizations", "host_to_device_transfers",
                 "device_to_host_transfers", "device_bytes", "graph_launches")
         return dict(zip(keys, [int(v) for v in out]))
+
+    def step_timed(self, n: int) -> dict:
+        """Runs n MCS with HIP events around every kernel (synchronizes)."""
+        out = (C.c_double * 8)()
+        self._check(self.lib.cpm_step_timed(self._ctx, int(n), out))
+        keys = ("propose_ms", "propose_launches", "commit_ms", "commit_launches", "begin_ms",
+                "begin_launches", "wall_ms", "mcs")
+        return dict(zip(keys, [float(v) for v in out]))
 
     def set_graph_chunk(self, chunk: int):
         self._check(self.lib.cpm_set_graph_chunk(self._ctx, int(chunk)))

@@ -116,6 +116,19 @@ struct PassArgs {
     uint32_t total_claims;
 };
 
+// Device buffers are reached through pointers stored in RepDev; casting them to the
+// global address space lets the compiler emit global_* (not flat_*) memory operations,
+// which do not wait on the LDS counter.
+#define CPM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ CPM_GLOBAL T *gp(T *p) {
+    return (CPM_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ const CPM_GLOBAL T *gp(const T *p) {
+    return (const CPM_GLOBAL T *)p;
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (rng/semantic.jl:136-187) and the v1 address packing (:249-261)
 // ---------------------------------------------------------------------------

@@ -17,6 +17,9 @@
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz);
 extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const cpm::PassArgs *a,
                                                hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
+                                                  hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_commit(const cpm::PassArgs *a, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
                                                 int nslots, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
@@ -800,19 +803,78 @@ struct Engine {
         return a;
     }
 
+    // Optional per-kernel event timing: events[i] precede kernel i, kinds[i] its kind.
+    std::vector<hipEvent_t> *timing_events = nullptr;
+    std::vector<int> *timing_kinds = nullptr;
+
+    void mark(int kind) {
+        if (!timing_events) return;
+        hipEvent_t ev;
+        hip_check(hipEventCreate(&ev), "hipEventCreate");
+        hip_check(hipEventRecord(ev, stream), "hipEventRecord");
+        timing_events->push_back(ev);
+        timing_kinds->push_back(kind);
+    }
+
     // Enqueue one MCS for every active replica (begin + 2 launches per pass).
     void enqueue_mcs() {
+        mark(2);
         hip_check(cpm_internal_launch_begin(d_reps, d_dyn, slot_cap, stream), "begin_mcs");
         metrics[0]++;
         for (Group &g : groups) {
             if (g.ntiles == 0) continue;
             for (int p = 0; p < g.max_rounds; ++p) {
                 const cpm::PassArgs a = args(g, (uint32_t)p, (uint32_t)(g.pass_counter & 1));
-                hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
+                if (timing_events) {
+                    mark(0);
+                    hip_check(cpm_internal_launch_propose(g.config, algorithm, &a, stream), "propose");
+                    mark(1);
+                    hip_check(cpm_internal_launch_commit(&a, stream), "commit");
+                } else {
+                    hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
+                }
                 metrics[0] += 2;
                 g.pass_counter++;
             }
         }
+    }
+
+    void step_timed(uint64_t n, double out[8]) {
+        rebuild_dirty();
+        for (Group &g : groups) g.dirty_arena_fresh = false;
+        std::vector<hipEvent_t> events;
+        std::vector<int> kinds;
+        timing_events = &events;
+        timing_kinds = &kinds;
+        try {
+            for (uint64_t i = 0; i < n; ++i) enqueue_mcs();
+            mark(3);
+        } catch (...) {
+            timing_events = nullptr;
+            timing_kinds = nullptr;
+            for (hipEvent_t e : events) (void)hipEventDestroy(e);
+            throw;
+        }
+        timing_events = nullptr;
+        timing_kinds = nullptr;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        for (int k = 0; k < 8; ++k) out[k] = 0.0;
+        for (size_t i = 0; i + 1 < events.size(); ++i) {
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, events[i], events[i + 1]), "hipEventElapsedTime");
+            const int kind = kinds[i];
+            const int slot = kind == 0 ? 0 : kind == 1 ? 2 : 4;
+            out[slot] += ms;
+            out[slot + 1] += 1.0;
+        }
+        if (events.size() > 1) {
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, events.front(), events.back()), "hipEventElapsedTime");
+            out[6] = ms;
+        }
+        out[7] = (double)n;
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
 
     bool graph_parity_ok(uint32_t mcs) const {
@@ -1183,6 +1245,17 @@ int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]) {
     if (!ctx || !out) return CPM_E_INVALID;
     std::memcpy(out, ctx->engine.metrics, sizeof(ctx->engine.metrics));
     return CPM_OK;
+}
+
+int32_t cpm_step_timed(cpm_ctx *ctx, uint64_t n, double out[8]) {
+    if (!ctx || !out) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        Engine &e = ctx->engine;
+        const uint64_t target = ctx->host_mcs + n;
+        e.set_dyn(ctx->slot, 0, target, 1, 2);
+        e.step_timed(n, out);
+        ctx->host_mcs = target;
+    });
 }
 
 int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk) {

@@ -22,14 +22,21 @@ namespace cpm {
 
 template <int DIM, int H, int TX, int TY, int TZ>
 struct Geom {
+    static constexpr int PX = 4;  // x padding (>= halo) keeping LDS rows 16-byte aligned
     static constexpr int HZ = DIM == 3 ? H : 0;
-    static constexpr int LX = TX + 2 * H;
+    static constexpr int LX = TX + 2 * PX;
     static constexpr int LY = TY + 2 * H;
     static constexpr int LZ = TZ + 2 * HZ;
     static constexpr int LN = LX * LY * LZ;
     static constexpr int TN = TX * TY * TZ;
+    static constexpr int LV = LX / 4;           // int4 per LDS row
+    static constexpr int NV = LV * LY * LZ;     // int4 per tile
     static_assert(DIM == 3 || TZ == 1, "2D tiles have TZ == 1");
+    static_assert(TX % 4 == 0 && H <= PX, "tile rows must be whole int4");
     static_assert(LN < 65536, "LDS positions must fit 16 bits");
+    static __device__ __forceinline__ int pos(int ix, int iy, int iz) {
+        return (ix + PX) + LX * ((iy + H) + LY * (iz + HZ));
+    }
 };
 
 __device__ __forceinline__ int wrap_axis(int g, int n) {
@@ -46,21 +53,24 @@ __device__ __forceinline__ int wrap_axis(int g, int n) {
 // once (lib/CorePotts/src/spatial/cartesian.jl:445-525). Returns the owner word seen
 // by contact/surface relations (0 for absent or invalid) and whether the position is
 // a mutable site (the only donors a proposal relation realizes).
-__device__ inline int32_t realize_position(const RepDev &d, int gx, int gy, int gz,
-                                           uint8_t *mutable_out, uint32_t *site_out) {
-    int c[3] = {gx, gy, gz};
+struct Realized {
+    int32_t word;   // owner seen by contact/surface relations, 0 when absent or invalid
+    uint32_t site;  // 1-based site when the position is inside the domain, else 0
+    uint8_t mut;    // mutable site (the only donors a proposal relation realizes)
+};
+
+__device__ inline Realized realize_position(const RepDev &d, int gx, int gy, int gz) {
     bool closed = false, fixed = false, invalid = false;
     int32_t fixed_word = 0;
-    for (int a = 0; a < d.ndim; ++a) {
+    auto axis = [&](int a, int &v) {
         const int n = d.dims[a];
-        int v = c[a];
         int face;
         if (v < 0) face = 0;
         else if (v >= n) face = 1;
-        else continue;
+        else return;
         const int kind = d.face_kind[a][face];
         if (kind == CPM_FACE_PERIODIC) {
-            c[a] = wrap_axis(v, n);
+            v = wrap_axis(v, n);
         } else if (kind == CPM_FACE_CLOSED) {
             invalid |= fixed;
             closed = true;
@@ -71,16 +81,21 @@ __device__ inline int32_t realize_position(const RepDev &d, int gx, int gy, int 
             fixed = true;
             fixed_word = w;
         }
+    };
+    axis(0, gx);
+    axis(1, gy);
+    if (d.ndim == 3) axis(2, gz);
+    Realized r{0, 0, 0};
+    if (invalid || closed) return r;
+    if (fixed) {
+        r.word = fixed_word;
+        return r;
     }
-    *site_out = 0;
-    *mutable_out = 0;
-    if (invalid || closed) return 0;
-    if (fixed) return fixed_word;
-    const uint32_t lin = (uint32_t)c[0] + (uint32_t)d.dims[0] *
-                                               ((uint32_t)c[1] + (uint32_t)d.dims[1] * (uint32_t)c[2]);
-    *site_out = lin + 1;
-    *mutable_out = d.mask ? d.mask[lin] : (uint8_t)1;
-    return d.lattice[lin];
+    const uint32_t lin = (uint32_t)gx + (uint32_t)d.dims[0] * ((uint32_t)gy + (uint32_t)d.dims[1] * (uint32_t)gz);
+    r.site = lin + 1;
+    r.mut = d.mask ? gp(d.mask)[lin] : (uint8_t)1;
+    r.word = gp(d.lattice)[lin];
+    return r;
 }
 
 template <int DIM, int H, int TX, int TY, int TZ>
@@ -88,416 +103,121 @@ __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_
                                  uint8_t *s_mut, uint32_t *s_sid) {
     using G = Geom<DIM, H, TX, TY, TZ>;
     const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
-    const bool interior = x0 >= H && x0 + TX + H <= X && y0 >= H && y0 + TY + H <= Y &&
-                          (DIM == 2 || (z0 >= G::HZ && z0 + TZ + G::HZ <= Z));
-    const int32_t *lat = d.lattice;
-    const uint8_t *mask = d.mask;
-    for (int i = threadIdx.x; i < G::LN; i += kBlock) {
-        const int lx = i % G::LX;
-        const int ly = (i / G::LX) % G::LY;
-        const int lz = i / (G::LX * G::LY);
-        int gx = x0 - H + lx, gy = y0 - H + ly, gz = z0 - G::HZ + lz;
-        int32_t w;
-        uint8_t mu;
-        uint32_t sid;
-        if (interior || d.simple) {
-            if (!interior) {
-                gx = wrap_axis(gx, X);
-                gy = wrap_axis(gy, Y);
-                if (DIM == 3) gz = wrap_axis(gz, Z);
+    const bool inner_yz = y0 >= H && y0 + TY + H <= Y && (DIM == 2 || (z0 >= G::HZ && z0 + TZ + G::HZ <= Z));
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    if (d.simple && (X & 3) == 0 && x0 >= G::PX && x0 + TX + G::PX <= X && inner_yz) {
+        // interior tile of a periodic, obstacle-free lattice: 16-byte loads, all in flight
+        constexpr int NL = (G::NV + kBlock - 1) / kBlock;
+        typedef int32_t v4i __attribute__((ext_vector_type(4)));
+        v4i v[NL];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            const int jj = (int)threadIdx.x + j * kBlock;
+            if (jj < G::NV) {
+                const int row = jj / G::LV, c4 = jj % G::LV;
+                const int gx = x0 - G::PX + 4 * c4, gy = y0 - H + row % G::LY, gz = z0 - G::HZ + row / G::LY;
+                v[j] = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz));
             }
-            const uint32_t lin = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
-            w = lat[lin];
-            mu = mask ? mask[lin] : (uint8_t)1;
-            sid = lin + 1;
-        } else {
-            w = realize_position(d, gx, gy, gz, &mu, &sid);
         }
-        s_own[i] = w;
-        s_mut[i] = mu;
-        if (s_sid) s_sid[i] = sid;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            const int jj = (int)threadIdx.x + j * kBlock;
+            if (jj < G::NV) {
+                const int row = jj / G::LV, c4 = jj % G::LV;
+                *reinterpret_cast<v4i *>(s_own + row * G::LX + 4 * c4) = v[j];
+                if (s_sid) {
+                    const int gx = x0 - G::PX + 4 * c4, gy = y0 - H + row % G::LY, gz = z0 - G::HZ + row / G::LY;
+                    const uint32_t lin = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
+                    for (int q = 0; q < 4; ++q) s_sid[row * G::LX + 4 * c4 + q] = lin + 1 + q;
+                }
+            }
+        }
+        if (s_sid)  // lottery kernels read the mutable flags of every position
+            for (int i = threadIdx.x; i < G::LN; i += kBlock) s_mut[i] = 1;
+        return;
+    }
+    constexpr int NL = (G::LN + kBlock - 1) / kBlock;
+    const bool interior = x0 >= G::PX && x0 + TX + G::PX <= X && inner_yz;
+    const CPM_GLOBAL uint8_t *mask = d.mask ? gp(d.mask) : nullptr;
+    int32_t w[NL];
+    // every load of the tile is issued before the first LDS store
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const int i = (int)threadIdx.x + j * kBlock;
+        w[j] = 0;
+        if (i < G::LN) {
+            const int lx = i % G::LX;
+            const int ly = (i / G::LX) % G::LY;
+            const int lz = i / (G::LX * G::LY);
+            int gx = x0 - G::PX + lx, gy = y0 - H + ly, gz = z0 - G::HZ + lz;
+            if (interior || d.simple) {
+                if (!interior) {
+                    gx = wrap_axis(gx, X);
+                    gy = wrap_axis(gy, Y);
+                    if (DIM == 3) gz = wrap_axis(gz, Z);
+                }
+                const uint32_t lin = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
+                w[j] = lat[lin];
+                if (mask || s_sid) {
+                    s_mut[i] = mask ? mask[lin] : (uint8_t)1;
+                    if (s_sid) s_sid[i] = lin + 1;
+                }
+            } else {
+                const Realized rz = realize_position(d, gx, gy, gz);
+                w[j] = rz.word;
+                s_mut[i] = rz.mut;
+                if (s_sid) s_sid[i] = rz.site;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const int i = (int)threadIdx.x + j * kBlock;
+        if (i < G::LN) s_own[i] = w[j];
     }
 }
 
-// Append lanes with `flag` set to an LDS list; returns nothing, writes value.
+// Append lanes with `flag` set to an LDS list of 16-bit tile indices.
 __device__ __forceinline__ void ballot_push(bool flag, uint16_t value, uint32_t *counter,
                                             uint16_t *list) {
     const uint64_t m = __ballot(flag);
     if (m == 0) return;
-    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
     uint32_t base = 0;
-    if (lane == (uint32_t)__ffsll((unsigned long long)m) - 1) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
     if (flag) list[base + __popcll(m & __lanemask_lt())] = value;
 }
 
-__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *s_red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+// Block-wide sum of four counters (one LDS round).
+__device__ __forceinline__ void block_sum4(uint32_t v[4], uint32_t (*s_red)[kBlock / 64]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o);
     const int wave = threadIdx.x >> 6;
     __syncthreads();
-    if (__lane_id() == 0) s_red[wave] = v;
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_red[k][wave] = v[k];
     __syncthreads();
-    uint32_t t = 0;
-    for (int k = 0; k < kBlock / 64; ++k) t += s_red[k];
-    return t;
-}
-
-// Stage one actionable attempt: publish the claim key and append to the tile segment.
-__device__ __forceinline__ void stage_attempt(bool act, const Entry &ent, const RepDev &d,
-                                              unsigned long long *claims, uint32_t *s_nseg,
-                                              Entry *seg) {
-    if (act) {
-        const unsigned long long key = ((unsigned long long)ent.priority << 32) | (uint32_t)(~ent.site);
-        if (ent.losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)ent.losing], key);
-        if (ent.gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)ent.gaining], key);
-    }
-    const uint64_t m = __ballot(act);
-    if (m == 0) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(s_nseg, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (act) {
-        const uint32_t idx = base + (uint32_t)__popcll(m & __lanemask_lt());
-        uint4 v;
-        v.x = ent.site;
-        v.y = ent.priority;
-        v.z = (uint32_t)ent.losing;
-        v.w = (uint32_t)ent.gaining;
-        reinterpret_cast<uint4 *>(seg)[idx] = v;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Checkerboard colour pass, propose + claim (checkerboard.jl:244-304)
-// ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
-    using G = Geom<DIM, H, TX, TY, TZ>;
-    __shared__ int32_t s_own[G::LN];
-    __shared__ uint8_t s_mut[G::LN];
-    __shared__ uint16_t s_work[G::TN];
-    __shared__ int32_t s_ploff[CPM_MAX_DIRS];
-    __shared__ uint8_t s_res[TX + TY + TZ];
-    __shared__ uint32_t s_nwork, s_nseg;
-    __shared__ uint32_t s_red[kBlock / 64];
-
-    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
-    const uint32_t r = a.tile_rep[tile];
-    const RepDev &d = a.reps[r];
-    const RepDyn &y = a.dyn[r];
-    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
-    const uint64_t mcs = y.mcs;
-    const uint32_t colour = y.order[a.pass];
-
-    const uint32_t lt = tile - d.tile_base;
-    const int ntx = d.ntiles[0], nty = d.ntiles[1];
-    const int x0 = (int)(lt % ntx) * TX;
-    const int y0 = (int)((lt / ntx) % nty) * TY;
-    const int z0 = (int)(lt / (ntx * nty)) * TZ;
-    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
-    const int K = d.prop.n;
-    const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
-    const bool by_pattern = d.site_color == nullptr;
-
-    if (threadIdx.x == 0) {
-        s_nwork = 0;
-        s_nseg = 0;
-    }
-    for (int k = threadIdx.x; k < K; k += kBlock)
-        s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
-    if (by_pattern) {
-        for (int i = threadIdx.x; i < TX + TY + TZ; i += kBlock) {
-            int v;
-            if (i < TX) v = (x0 + i) % px;
-            else if (i < TX + TY) v = (y0 + i - TX) % py;
-            else v = (z0 + i - TX - TY) % pz;
-            s_res[i] = (uint8_t)v;
-        }
-    }
-    load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, nullptr);
-    __syncthreads();
-
-    uint32_t c_cand = 0, c_same = 0, c_bnd = 0, c_real = 0;
-    for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
-        const int i = i0 + (int)threadIdx.x;
-        bool need = false;
-        if (i < G::TN) {
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
-            if (gx < X && gy < Y && gz < Z) {
-                const int pos = (ix + H) + G::LX * ((iy + H) + G::LY * (iz + G::HZ));
-                if (s_mut[pos]) {
-                    uint32_t c;
-                    if (by_pattern)
-                        c = d.pattern[s_res[ix] + px * (s_res[TX + iy] + py * s_res[TX + TY + iz])];
-                    else
-                        c = d.site_color[(uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)];
-                    if (c == colour) {
-                        ++c_cand;
-                        const int32_t own = s_own[pos];
-                        bool interior = true;
-                        for (int k = 0; k < K; ++k) {
-                            const int q = pos + s_ploff[k];
-                            if (!s_mut[q] || s_own[q] != own) {
-                                interior = false;
-                                break;
-                            }
-                        }
-                        if (interior) ++c_same;
-                        else need = true;
-                    }
-                }
-            }
-        }
-        ballot_push(need, (uint16_t)i, &s_nwork, s_work);
-    }
-    __syncthreads();
-
-    const uint32_t nwork = s_nwork;
-    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    Entry *seg = a.seg + (size_t)tile * a.seg_cap;
-    for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
-        const uint32_t e = e0 + threadIdx.x;
-        bool act = false;
-        Entry ent{0, 0, 0, 0};
-        if (e < nwork) {
-            const int i = s_work[e];
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int pos = (ix + H) + G::LX * ((iy + H) + G::LY * (iz + G::HZ));
-            const uint32_t site = 1u + (uint32_t)(x0 + ix) +
-                                  (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
-            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0,
-                                              kSiteEntity, site, 0, (uint32_t)K);
-            const int q = pos + s_ploff[dir];
-            if (!s_mut[q]) {
-                ++c_bnd;
-            } else {
-                const int32_t own = s_own[pos], gain = s_own[q];
-                if (gain == own) {
-                    ++c_same;
-                } else {
-                    ++c_real;
-                    const Words w = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs,
-                                              0, 0, kSiteEntity, site, 0, 0);
-                    ent = Entry{site, w.w0, own, gain};
-                    act = true;
-                }
-            }
-        }
-        stage_attempt(act, ent, d, claims, &s_nseg, seg);
-    }
-
-    c_cand = block_sum(c_cand, s_red);
-    c_same = block_sum(c_same, s_red);
-    c_bnd = block_sum(c_bnd, s_red);
-    c_real = block_sum(c_real, s_red);
-    if (threadIdx.x == 0) {
-        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-        slot[0] = c_cand;
-        slot[1] = c_same;
-        slot[2] = c_bnd;
-        slot[3] = c_real;
-        a.seg_count[tile] = s_nseg;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Lottery round, propose + claim (lottery.jl:153-256)
-// ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
-    using G = Geom<DIM, H, TX, TY, TZ>;
-    __shared__ int32_t s_own[G::LN];
-    __shared__ uint8_t s_mut[G::LN];
-    __shared__ uint32_t s_sid[G::LN];
-    __shared__ uint32_t s_tk[G::LN];
-    __shared__ uint16_t s_work[G::TN];
-    __shared__ int32_t s_ploff[CPM_MAX_DIRS];
-    __shared__ int32_t s_cloff[CPM_MAX_DIRS];
-    __shared__ uint32_t s_nwork, s_nseg;
-    __shared__ uint32_t s_red[kBlock / 64];
-
-    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
-    const uint32_t r = a.tile_rep[tile];
-    const RepDev &d = a.reps[r];
-    const RepDyn &y = a.dyn[r];
-    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
-    const uint64_t mcs = y.mcs;
-    const uint32_t sub = y.order[a.pass] - 1;
-    const uint32_t R = (uint32_t)d.rounds;
-
-    const uint32_t lt = tile - d.tile_base;
-    const int ntx = d.ntiles[0], nty = d.ntiles[1];
-    const int x0 = (int)(lt % ntx) * TX;
-    const int y0 = (int)((lt / ntx) % nty) * TY;
-    const int z0 = (int)(lt / (ntx * nty)) * TZ;
-    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
-    const int K = d.prop.n;
-    const int KC = d.conf.n;
-
-    if (threadIdx.x == 0) {
-        s_nwork = 0;
-        s_nseg = 0;
-    }
-    for (int k = threadIdx.x; k < K; k += kBlock)
-        s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
-    for (int k = threadIdx.x; k < KC; k += kBlock)
-        s_cloff[k] = d.conf.off[k][0] + G::LX * (d.conf.off[k][1] + G::LY * d.conf.off[k][2]);
-    load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, s_sid);
-    __syncthreads();
-    // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1)
-    for (int i = threadIdx.x; i < G::LN; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
         uint32_t t = 0;
-        if (s_mut[i])
-            t = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
-                          s_sid[i], 0, 0).w0;
-        s_tk[i] = t;
-    }
-    __syncthreads();
-
-    uint32_t c_act = 0, c_same = 0, c_bnd = 0, c_real = 0;
-    for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
-        const int i = i0 + (int)threadIdx.x;
-        bool need = false;
-        if (i < G::TN) {
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
-            if (gx < X && gy < Y && gz < Z) {
-                const int pos = (ix + H) + G::LX * ((iy + H) + G::LY * (iz + G::HZ));
-                if (s_mut[pos]) {
-                    const uint32_t site = s_sid[pos];
-                    const uint32_t mine = s_tk[pos];
-                    bool wins = true;
-                    for (int k = 0; k < KC && wins; ++k) {
-                        const int q = pos + s_cloff[k];
-                        if (!s_mut[q]) continue;
-                        const uint32_t nsite = s_sid[q];
-                        if (nsite == site) continue;
-                        const uint32_t other = s_tk[q];
-                        if (other > mine) {
-                            wins = false;
-                        } else if (other == mine) {
-                            // full lexicographic ticket comparison (lottery.jl:168-176)
-                            const Words a4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
-                                                       mcs, sub, 0, kSiteEntity, nsite, 0, 0);
-                            const Words b4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
-                                                       mcs, sub, 0, kSiteEntity, site, 0, 0);
-                            bool better;
-                            if (a4.w1 != b4.w1) better = a4.w1 > b4.w1;
-                            else if (a4.w2 != b4.w2) better = a4.w2 > b4.w2;
-                            else if (a4.w3 != b4.w3) better = a4.w3 > b4.w3;
-                            else better = nsite < site;
-                            if (better) wins = false;
-                        }
-                    }
-                    if (wins) {
-                        bool eligible = true;
-                        if (!d.uniform_degree) {
-                            const uint32_t degree = d.site_degree[site - 1];
-                            eligible = bounded_uint(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs,
-                                                    sub, 0, kSiteEntity, site, 0, R) < degree + 1;
-                        }
-                        if (eligible) {
-                            ++c_act;
-                            const int32_t own = s_own[pos];
-                            bool interior = true;
-                            for (int k = 0; k < K; ++k) {
-                                const int q = pos + s_ploff[k];
-                                if (!s_mut[q] || s_own[q] != own) {
-                                    interior = false;
-                                    break;
-                                }
-                            }
-                            if (interior) ++c_same;
-                            else need = true;
-                        }
-                    }
-                }
-            }
-        }
-        ballot_push(need, (uint16_t)i, &s_nwork, s_work);
-    }
-    __syncthreads();
-
-    const uint32_t nwork = s_nwork;
-    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    Entry *seg = a.seg + (size_t)tile * a.seg_cap;
-    for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
-        const uint32_t e = e0 + threadIdx.x;
-        bool act = false;
-        Entry ent{0, 0, 0, 0};
-        if (e < nwork) {
-            const int i = s_work[e];
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int pos = (ix + H) + G::LX * ((iy + H) + G::LY * (iz + G::HZ));
-            const uint32_t site = s_sid[pos];
-            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
-                                              kSiteEntity, site, 0, (uint32_t)K);
-            const int q = pos + s_ploff[dir];
-            if (!s_mut[q]) {
-                ++c_bnd;
-            } else {
-                const int32_t own = s_own[pos], gain = s_own[q];
-                if (gain == own) {
-                    ++c_same;
-                } else {
-                    ++c_real;
-                    ent = Entry{site, s_tk[pos], own, gain};
-                    act = true;
-                }
-            }
-        }
-        stage_attempt(act, ent, d, claims, &s_nseg, seg);
-    }
-
-    c_act = block_sum(c_act, s_red);
-    c_same = block_sum(c_same, s_red);
-    c_bnd = block_sum(c_bnd, s_red);
-    c_real = block_sum(c_real, s_red);
-    if (threadIdx.x == 0) {
-        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-        slot[0] = c_act;
-        slot[1] = c_same;
-        slot[2] = c_bnd;
-        slot[3] = c_real;
-        a.seg_count[tile] = s_nseg;
+        for (int q = 0; q < kBlock / 64; ++q) t += s_red[k][q];
+        v[k] = t;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Claim resolution, energy change, acceptance and commit
+// Energy terms (scientific_components.jl, scientific_trackers.jl)
 // ---------------------------------------------------------------------------
-
-// Owner word seen by a contact/surface relation at site (x,y,z) + offset; false when
-// the neighbour is absent or invalid (cartesian.jl:494-525).
-__device__ __forceinline__ bool neighbour_owner(const RepDev &d, int x, int y, int z,
-                                                const int8_t *o, int32_t *word) {
-    if (d.simple) {
-        const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
-        int nx = x + o[0], ny = y + o[1], nz = z + o[2];
-        nx += nx < 0 ? X : 0;
-        nx -= nx >= X ? X : 0;
-        ny += ny < 0 ? Y : 0;
-        ny -= ny >= Y ? Y : 0;
-        nz += nz < 0 ? Z : 0;
-        nz -= nz >= Z ? Z : 0;
-        *word = d.lattice[(uint32_t)nx + (uint32_t)X * ((uint32_t)ny + (uint32_t)Y * (uint32_t)nz)];
-        return true;
-    }
-    uint8_t mu;
-    uint32_t sid;
-    const int32_t w = realize_position(d, x + o[0], y + o[1], z + o[2], &mu, &sid);
-    *word = w;
-    return w != 0;
-}
 
 // Cell type or declared medium type, clamped into 1..n_types (the host rejects models
 // and states that could reach an undeclared type).
 __device__ __forceinline__ uint32_t owner_type(const RepDev &d, int32_t w) {
     uint32_t t = 1;
     if (w > 0) {
-        t = d.ctype[w];
+        t = gp(d.ctype)[w];
     } else {
         for (int i = 0; i < d.n_table; ++i)
             if (d.table_word[i] == w) t = d.table_type[i];
@@ -546,38 +266,452 @@ __device__ __forceinline__ double bnd_change_f64(double s, double t, double m, i
     return __dsub_rn(__dmul_rn(s, __dmul_rn(d1, d1)), old);
 }
 
-struct EvalCounts {
-    uint32_t conflicts = 0, rejected = 0, accepted = 0;
-};
+// ---------------------------------------------------------------------------
+// Propose side
+// ---------------------------------------------------------------------------
 
-__device__ inline void resolve_entry(const RepDev &d, uint64_t mcs, uint32_t sub,
-                                     const unsigned long long *claims, const Entry &e,
-                                     EvalCounts &cnt) {
-    const unsigned long long key = ((unsigned long long)e.priority << 32) | (uint32_t)(~e.site);
-    const int32_t l = e.losing, g = e.gaining;
-    bool win = true;
-    if (l > 0) win = win && claims[d.claim_base + (uint32_t)l] == key;
-    if (g > 0) win = win && claims[d.claim_base + (uint32_t)g] == key;
-    if (!win) {
-        ++cnt.conflicts;
-        return;
+// Publish the claim key of an actionable attempt on both cells (one 64-bit atomic max
+// per cell replaces the max-priority and min-identity claims, checkerboard.jl:281-335)
+// and append the attempt to the tile segment.
+__device__ __forceinline__ void stage_claim(bool act, const Entry &ent, const RepDev &d,
+                                            unsigned long long *claims, uint32_t *s_nseg,
+                                            Entry *seg) {
+    if (act) {
+        const unsigned long long key = ((unsigned long long)ent.priority << 32) | (uint32_t)(~ent.site);
+        if (ent.losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)ent.losing], key);
+        if (ent.gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)ent.gaining], key);
     }
+    const uint64_t m = __ballot(act);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(s_nseg, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (act) {
+        uint4 v;
+        v.x = ent.site;
+        v.y = ent.priority;
+        v.z = (uint32_t)ent.losing;
+        v.w = (uint32_t)ent.gaining;
+        reinterpret_cast<uint4 *>(seg)[base + __popcll(m & __lanemask_lt())] = v;
+    }
+}
+
+__device__ __forceinline__ int ilog2_exact(int v) { return 31 - __clz(v); }
+
+// Enumerates the tile sites of one colour. Pattern colourings visit only the residue
+// classes of that colour; per-site colourings scan the tile. fn(i, ix, iy, iz) is called
+// by every lane of the workgroup in lock-step (i < 0 for idle lanes).
+template <int DIM, int TX, int TY, int TZ, typename F>
+__device__ inline void for_colour_sites(const RepDev &d, const uint8_t *s_pattern, uint32_t colour,
+                                        int x0, int y0, int z0, F &&fn) {
+    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
+    if (d.site_color == nullptr) {
+        const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
+        const int np = px * py * pz;
+        // periods dividing the tile sides give power-of-two class extents: shifts, no division
+        const bool pow2 = (TX % px == 0) && (TY % py == 0) && (TZ % pz == 0) && ((TX / px) & (TX / px - 1)) == 0 &&
+                          ((TY / py) & (TY / py - 1)) == 0 && ((TZ / pz) & (TZ / pz - 1)) == 0;
+        for (int r = 0; r < np; ++r) {
+            if (s_pattern[r] != colour) continue;
+            const int rx = r % px, ry = (r / px) % py, rz = r / (px * py);
+            const int fx = (rx - x0 % px + px) % px;
+            const int fy = (ry - y0 % py + py) % py;
+            const int fz = (rz - z0 % pz + pz) % pz;
+            const int nx = fx < TX ? (TX - fx + px - 1) / px : 0;
+            const int ny = fy < TY ? (TY - fy + py - 1) / py : 0;
+            const int nz = fz < TZ ? (TZ - fz + pz - 1) / pz : 0;
+            const int n = nx * ny * nz;
+            const int sx = pow2 ? ilog2_exact(max(nx, 1)) : 0, sy = pow2 ? ilog2_exact(max(ny, 1)) : 0;
+            for (int j0 = 0; j0 < n; j0 += kBlock) {
+                const int j = j0 + (int)threadIdx.x;
+                int i = -1, ix = 0, iy = 0, iz = 0;
+                if (j < n) {
+                    int kx, ky, kz;
+                    if (pow2) {
+                        kx = j & (nx - 1);
+                        ky = (j >> sx) & (ny - 1);
+                        kz = j >> (sx + sy);
+                    } else {
+                        kx = j % nx;
+                        ky = (j / nx) % ny;
+                        kz = j / (nx * ny);
+                    }
+                    ix = fx + px * kx;
+                    iy = fy + py * ky;
+                    iz = fz + pz * kz;
+                    if (x0 + ix < X && y0 + iy < Y && z0 + iz < Z) i = ix + TX * (iy + TY * iz);
+                }
+                fn(i, ix, iy, iz);
+            }
+        }
+    } else {
+        constexpr int TN = TX * TY * TZ;
+        for (int i0 = 0; i0 < TN; i0 += kBlock) {
+            int i = i0 + (int)threadIdx.x;
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            if (i < TN) {
+                const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
+                if (gx >= X || gy >= Y || gz >= Z ||
+                    gp(d.site_color)[(uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)] != colour)
+                    i = -1;
+            } else {
+                i = -1;
+            }
+            fn(i, ix, iy, iz);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Checkerboard colour pass, propose + claim (checkerboard.jl:244-304)
+// ---------------------------------------------------------------------------
+template <int DIM, int H, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
+    using G = Geom<DIM, H, TX, TY, TZ>;
+    __shared__ __align__(16) int32_t s_own[G::LN];
+    __shared__ uint8_t s_mut[G::LN];
+    __shared__ uint16_t s_work[G::TN];
+    __shared__ uint16_t s_act[G::TN];
+    __shared__ uint8_t s_dir[G::TN];
+    __shared__ int32_t s_ploff[CPM_MAX_DIRS];
+    __shared__ uint8_t s_pattern[64];
+    __shared__ uint32_t s_nwork, s_nact, s_nseg;
+    __shared__ uint32_t s_red[4][kBlock / 64];
+
+    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
+    const uint32_t r = a.tile_rep[tile];
+    const RepDev &d = a.reps[r];
+    const RepDyn &y = a.dyn[r];
+    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
+    const uint64_t mcs = y.mcs;
+    const uint32_t colour = y.order[a.pass];
+
+    const uint32_t lt = tile - d.tile_base;
+    const int ntx = d.ntiles[0], nty = d.ntiles[1];
+    const int x0 = (int)(lt % ntx) * TX;
+    const int y0 = (int)((lt / ntx) % nty) * TY;
+    const int z0 = (int)(lt / (ntx * nty)) * TZ;
+    const int X = d.dims[0], Y = d.dims[1];
+    const int K = d.prop.n;
+    const bool simple = d.simple;
+
+    if (threadIdx.x == 0) {
+        s_nwork = 0;
+        s_nact = 0;
+        s_nseg = 0;
+    }
+    for (int k = threadIdx.x; k < K; k += kBlock)
+        s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
+    if (threadIdx.x < 64) s_pattern[threadIdx.x] = d.pattern[threadIdx.x];
+    load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, nullptr);
+    __syncthreads();
+
+    // 1. colour sites; interior sites (every proposal neighbour a mutable site of the
+    //    same owner) are same-owner no-ops whatever the direction draw
+    uint32_t cnt[4] = {0, 0, 0, 0};  // candidates, same owner, boundary null, realized
+    for_colour_sites<DIM, TX, TY, TZ>(d, s_pattern, colour, x0, y0, z0, [&](int i, int ix, int iy, int iz) {
+        bool need = false;
+        if (i >= 0) {
+            const int pos = G::pos(ix, iy, iz);
+            if (simple || s_mut[pos]) {
+                ++cnt[0];
+                const int32_t own = s_own[pos];
+                bool interior = true;
+                for (int k = 0; k < K; ++k) {
+                    const int q = pos + s_ploff[k];
+                    if ((!simple && !s_mut[q]) || s_own[q] != own) {
+                        interior = false;
+                        break;
+                    }
+                }
+                if (interior) ++cnt[1];
+                else need = true;
+            }
+        }
+        ballot_push(need, (uint16_t)(i < 0 ? 0 : i), &s_nwork, s_work);
+    });
+    __syncthreads();
+
+    // 2. direction draws of the boundary sites (ProposalDirectionStream), attempt outcome
+    const uint32_t nwork = s_nwork;
+    for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
+        const uint32_t e = e0 + threadIdx.x;
+        bool act = false;
+        int i = 0;
+        if (e < nwork) {
+            i = s_work[e];
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            const int pos = G::pos(ix, iy, iz);
+            const uint32_t site = 1u + (uint32_t)(x0 + ix) +
+                                  (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
+            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0,
+                                              kSiteEntity, site, 0, (uint32_t)K);
+            const int q = pos + s_ploff[dir];
+            if (!simple && !s_mut[q]) {
+                ++cnt[2];
+            } else if (s_own[q] == s_own[pos]) {
+                ++cnt[1];
+            } else {
+                ++cnt[3];
+                act = true;
+                i |= (int)(dir << 16);
+            }
+        }
+        const uint64_t m = __ballot(act);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&s_nact, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (act) {
+                const uint32_t idx = base + __popcll(m & __lanemask_lt());
+                s_act[idx] = (uint16_t)(i & 0xffff);
+                s_dir[idx] = (uint8_t)(i >> 16);
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. priority draws and claims of the actionable attempts (compacted)
+    const uint32_t nact = s_nact;
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
+    Entry *seg = a.seg + (size_t)tile * a.seg_cap;
+    for (uint32_t e0 = 0; e0 < nact; e0 += kBlock) {
+        const uint32_t e = e0 + threadIdx.x;
+        bool act = false;
+        Entry ent{0, 0, 0, 0};
+        if (e < nact) {
+            const int i = s_act[e];
+            const int dir = s_dir[e];
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            const int pos = G::pos(ix, iy, iz);
+            const uint32_t site = 1u + (uint32_t)(x0 + ix) +
+                                  (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
+            const Words w = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
+                                      kSiteEntity, site, 0, 0);
+            ent = Entry{site, w.w0, s_own[pos], s_own[pos + s_ploff[dir]]};
+            act = true;
+        }
+        stage_claim(act, ent, d, claims, &s_nseg, seg);
+    }
+
+    block_sum4(cnt, s_red);
+    if (threadIdx.x == 0) {
+        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+        slot[0] = cnt[0];
+        slot[1] = cnt[1];
+        slot[2] = cnt[2];
+        slot[3] = cnt[3];
+        a.seg_count[tile] = s_nseg;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lottery round, propose + claim (lottery.jl:153-256)
+// ---------------------------------------------------------------------------
+template <int DIM, int H, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
+    using G = Geom<DIM, H, TX, TY, TZ>;
+    __shared__ __align__(16) int32_t s_own[G::LN];
+    __shared__ uint8_t s_mut[G::LN];
+    __shared__ uint32_t s_sid[G::LN];
+    __shared__ uint32_t s_tk[G::LN];
+    __shared__ uint16_t s_work[G::TN];
+    __shared__ int32_t s_ploff[CPM_MAX_DIRS];
+    __shared__ int32_t s_cloff[CPM_MAX_DIRS];
+    __shared__ uint32_t s_nwork, s_nseg;
+    __shared__ uint32_t s_red[4][kBlock / 64];
+
+    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
+    const uint32_t r = a.tile_rep[tile];
+    const RepDev &d = a.reps[r];
+    const RepDyn &y = a.dyn[r];
+    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
+    const uint64_t mcs = y.mcs;
+    const uint32_t sub = y.order[a.pass] - 1;
+    const uint32_t R = (uint32_t)d.rounds;
+
+    const uint32_t lt = tile - d.tile_base;
+    const int ntx = d.ntiles[0], nty = d.ntiles[1];
+    const int x0 = (int)(lt % ntx) * TX;
+    const int y0 = (int)((lt / ntx) % nty) * TY;
+    const int z0 = (int)(lt / (ntx * nty)) * TZ;
+    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
+    const int K = d.prop.n;
+    const int KC = d.conf.n;
+
+    if (threadIdx.x == 0) {
+        s_nwork = 0;
+        s_nseg = 0;
+    }
+    for (int k = threadIdx.x; k < K; k += kBlock)
+        s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
+    for (int k = threadIdx.x; k < KC; k += kBlock)
+        s_cloff[k] = d.conf.off[k][0] + G::LX * (d.conf.off[k][1] + G::LY * d.conf.off[k][2]);
+    load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, s_sid);
+    __syncthreads();
+    // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1)
+    for (int i = threadIdx.x; i < G::LN; i += kBlock) {
+        uint32_t t = 0;
+        if (s_mut[i])
+            t = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                          s_sid[i], 0, 0).w0;
+        s_tk[i] = t;
+    }
+    __syncthreads();
+
+    uint32_t cnt[4] = {0, 0, 0, 0};  // activated, same owner, boundary null, realized
+    for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
+        const int i = i0 + (int)threadIdx.x;
+        bool need = false;
+        if (i < G::TN) {
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
+            if (gx < X && gy < Y && gz < Z) {
+                const int pos = G::pos(ix, iy, iz);
+                if (s_mut[pos]) {
+                    const uint32_t site = s_sid[pos];
+                    const uint32_t mine = s_tk[pos];
+                    bool wins = true;
+                    for (int k = 0; k < KC && wins; ++k) {
+                        const int q = pos + s_cloff[k];
+                        if (!s_mut[q]) continue;
+                        const uint32_t nsite = s_sid[q];
+                        if (nsite == site) continue;
+                        const uint32_t other = s_tk[q];
+                        if (other > mine) {
+                            wins = false;
+                        } else if (other == mine) {
+                            // full lexicographic ticket comparison (lottery.jl:168-176)
+                            const Words a4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
+                                                       mcs, sub, 0, kSiteEntity, nsite, 0, 0);
+                            const Words b4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
+                                                       mcs, sub, 0, kSiteEntity, site, 0, 0);
+                            bool better;
+                            if (a4.w1 != b4.w1) better = a4.w1 > b4.w1;
+                            else if (a4.w2 != b4.w2) better = a4.w2 > b4.w2;
+                            else if (a4.w3 != b4.w3) better = a4.w3 > b4.w3;
+                            else better = nsite < site;
+                            if (better) wins = false;
+                        }
+                    }
+                    if (wins) {
+                        bool eligible = true;
+                        if (!d.uniform_degree) {
+                            const uint32_t degree = gp(d.site_degree)[site - 1];
+                            eligible = bounded_uint(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs,
+                                                    sub, 0, kSiteEntity, site, 0, R) < degree + 1;
+                        }
+                        if (eligible) {
+                            ++cnt[0];
+                            const int32_t own = s_own[pos];
+                            bool interior = true;
+                            for (int k = 0; k < K; ++k) {
+                                const int q = pos + s_ploff[k];
+                                if (!s_mut[q] || s_own[q] != own) {
+                                    interior = false;
+                                    break;
+                                }
+                            }
+                            if (interior) ++cnt[1];
+                            else need = true;
+                        }
+                    }
+                }
+            }
+        }
+        ballot_push(need, (uint16_t)(i < G::TN ? i : 0), &s_nwork, s_work);
+    }
+    __syncthreads();
+
+    const uint32_t nwork = s_nwork;
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
+    Entry *seg = a.seg + (size_t)tile * a.seg_cap;
+    for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
+        const uint32_t e = e0 + threadIdx.x;
+        bool act = false;
+        Entry ent{0, 0, 0, 0};
+        if (e < nwork) {
+            const int i = s_work[e];
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            const int pos = G::pos(ix, iy, iz);
+            const uint32_t site = s_sid[pos];
+            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
+                                              kSiteEntity, site, 0, (uint32_t)K);
+            const int q = pos + s_ploff[dir];
+            if (!s_mut[q]) {
+                ++cnt[2];
+            } else {
+                const int32_t own = s_own[pos], gain = s_own[q];
+                if (gain == own) {
+                    ++cnt[1];
+                } else {
+                    ++cnt[3];
+                    ent = Entry{site, s_tk[pos], own, gain};
+                    act = true;
+                }
+            }
+        }
+        stage_claim(act, ent, d, claims, &s_nseg, seg);
+    }
+
+    block_sum4(cnt, s_red);
+    if (threadIdx.x == 0) {
+        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+        slot[0] = cnt[0];
+        slot[1] = cnt[1];
+        slot[2] = cnt[2];
+        slot[3] = cnt[3];
+        a.seg_count[tile] = s_nseg;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Claim resolution, evaluation, acceptance and commit (checkerboard.jl:337-457)
+// ---------------------------------------------------------------------------
+
+// Owner word seen by a contact/surface relation at site (x,y,z) + offset; 0 when the
+// neighbour is absent or invalid (cartesian.jl:494-525).
+__device__ __forceinline__ int32_t neighbour_word(const RepDev &d, int x, int y, int z, const int8_t *o) {
+    if (d.simple) {
+        const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
+        int nx = x + o[0], ny = y + o[1], nz = z + o[2];
+        nx += nx < 0 ? X : 0;
+        nx -= nx >= X ? X : 0;
+        ny += ny < 0 ? Y : 0;
+        ny -= ny >= Y ? Y : 0;
+        nz += nz < 0 ? Z : 0;
+        nz -= nz >= Z ? Z : 0;
+        return gp(d.lattice)[(uint32_t)nx + (uint32_t)X * ((uint32_t)ny + (uint32_t)Y * (uint32_t)nz)];
+    }
+    return realize_position(d, x + o[0], y + o[1], z + o[2]).word;
+}
+
+// evaluate_copy + ConventionalMetropolis + acceptance draw of one winning attempt against
+// the pass snapshot (checkerboard.jl:371-408; scientific_inner_loop.jl:213-269;
+// protocols/scientific.jl:372-385). Winners touch disjoint cells and non-adjacent sites,
+// so other winners' commits never change what this evaluation reads.
+__device__ inline bool evaluate_winner(const RepDev &d, const Entry &e, uint64_t mcs, uint32_t sub,
+                                       int32_t *dl_out, int32_t *dg_out) {
+    const int32_t l = e.losing, g = e.gaining;
     const uint32_t idx0 = e.site - 1;
     const uint32_t X = (uint32_t)d.dims[0], Y = (uint32_t)d.dims[1];
     const int x = (int)(idx0 % X);
     const int y = (int)((idx0 / X) % Y);
     const int z = (int)(idx0 / (X * Y));
-
+    const int32_t vl = l > 0 ? gp(d.vol)[l] : 0;
+    const int32_t vg = g > 0 ? gp(d.vol)[g] : 0;
     // boundary tracker deltas (BoundaryEdgeCount, scientific_components.jl:466-481)
     int32_t dl = 0, dg = 0;
     for (int k = 0; k < d.surf.n; ++k) {
-        int32_t w;
-        if (!neighbour_owner(d, x, y, z, d.surf.off[k], &w)) continue;
+        const int32_t w = neighbour_word(d, x, y, z, d.surf.off[k]);
+        if (w == 0) continue;
         dl += (w == l) ? 1 : -1;
         dg += (w == g) ? -1 : 1;
     }
-
-    // energy fold in component order (scientific_inner_loop.jl:213-217,257-269)
+    *dl_out = dl;
+    *dg_out = dg;
     double acc = 0.0;
     bool acc64 = false;
     for (int t = 0; t < d.n_energies; ++t) {
@@ -588,58 +722,70 @@ __device__ inline void resolve_entry(const RepDev &d, uint64_t mcs, uint32_t sub
             v64 = d.volume_f64;
             if (v64) {
                 double delta = 0.0;
-                if (l > 0) delta = __dadd_rn(delta, vol_change_f64(d.vs[l], d.tv[l], d.vol[l], -1));
-                if (g > 0) delta = __dadd_rn(delta, vol_change_f64(d.vs[g], d.tv[g], d.vol[g], 1));
+                if (l > 0) delta = __dadd_rn(delta, vol_change_f64(gp(d.vs)[l], gp(d.tv)[l], vl, -1));
+                if (g > 0) delta = __dadd_rn(delta, vol_change_f64(gp(d.vs)[g], gp(d.tv)[g], vg, 1));
                 v = delta;
             } else {
                 float delta = 0.0f;
-                if (l > 0)
-                    delta = __fadd_rn(delta, vol_change_f32((float)d.vs[l], (float)d.tv[l], d.vol[l], -1));
-                if (g > 0)
-                    delta = __fadd_rn(delta, vol_change_f32((float)d.vs[g], (float)d.tv[g], d.vol[g], 1));
+                if (l > 0) delta = __fadd_rn(delta, vol_change_f32((float)gp(d.vs)[l], (float)gp(d.tv)[l], vl, -1));
+                if (g > 0) delta = __fadd_rn(delta, vol_change_f32((float)gp(d.vs)[g], (float)gp(d.tv)[g], vg, 1));
                 v = (double)delta;
             }
         } else if (term == CPM_TERM_CONTACT) {
+            // scientific_components.jl:289-308
             v64 = d.contact_f64;
             const uint32_t nt = (uint32_t)d.n_types;
             const uint32_t lt = owner_type(d, l), gt = owner_type(d, g);
             double dd = 0.0;
             float df = 0.0f;
-            for (int k = 0; k < d.cont.n; ++k) {
-                int32_t w;
-                if (!neighbour_owner(d, x, y, z, d.cont.off[k], &w)) continue;
-                const uint32_t wt = owner_type(d, w);
-                const uint32_t il = (lt - 1) * nt + (wt - 1), ig = (gt - 1) * nt + (wt - 1);
-                if (v64) {
-                    const double wk = d.cont.wd[k];
-                    if (l != w) dd = __dsub_rn(dd, __dmul_rn(wk, d.Jd[il]));
-                    if (g != w) dd = __dadd_rn(dd, __dmul_rn(wk, d.Jd[ig]));
-                } else {
-                    const float wk = d.cont.wf[k];
-                    if (l != w) df = __fsub_rn(df, __fmul_rn(wk, d.Jf[il]));
-                    if (g != w) df = __fadd_rn(df, __fmul_rn(wk, d.Jf[ig]));
+            const int nc = d.cont.n;
+            constexpr int B = 4;
+            for (int k0 = 0; k0 < nc; k0 += B) {
+                int32_t wb[B];
+                uint32_t tb[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u) wb[u] = k0 + u < nc ? neighbour_word(d, x, y, z, d.cont.off[k0 + u]) : 0;
+#pragma unroll
+                for (int u = 0; u < B; ++u) tb[u] = gp(d.ctype)[wb[u] > 0 ? wb[u] : 0];  // slot 0 is a spare
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const int32_t w = wb[u];
+                    if (w == 0) continue;
+                    const int k = k0 + u;
+                    uint32_t wt;
+                    if (w == l) wt = lt;
+                    else if (w == g) wt = gt;
+                    else if (w > 0) wt = min(max(tb[u], 1u), nt);
+                    else wt = owner_type(d, w);
+                    const uint32_t il = (lt - 1) * nt + (wt - 1), ig = (gt - 1) * nt + (wt - 1);
+                    if (v64) {
+                        const double wk = d.cont.wd[k];
+                        if (l != w) dd = __dsub_rn(dd, __dmul_rn(wk, d.Jd[il]));
+                        if (g != w) dd = __dadd_rn(dd, __dmul_rn(wk, d.Jd[ig]));
+                    } else {
+                        const float wk = d.cont.wf[k];
+                        if (l != w) df = __fsub_rn(df, __fmul_rn(wk, d.Jf[il]));
+                        if (g != w) df = __fadd_rn(df, __fmul_rn(wk, d.Jf[ig]));
+                    }
                 }
             }
             v = v64 ? dd : (double)df;
         } else {
+            // scientific_trackers.jl:144-172
             v64 = d.boundary_f64;
             if (v64) {
                 double r = 0.0;
                 if (l > 0)
-                    r = __dadd_rn(r, bnd_change_f64(d.bs[l], (double)d.tb[l], (double)d.bnd[l], d.vol[l],
-                                                    (double)dl, -1));
+                    r = __dadd_rn(r, bnd_change_f64(gp(d.bs)[l], (double)gp(d.tb)[l], (double)gp(d.bnd)[l], vl, (double)dl, -1));
                 if (g > 0)
-                    r = __dadd_rn(r, bnd_change_f64(d.bs[g], (double)d.tb[g], (double)d.bnd[g], d.vol[g],
-                                                    (double)dg, 1));
+                    r = __dadd_rn(r, bnd_change_f64(gp(d.bs)[g], (double)gp(d.tb)[g], (double)gp(d.bnd)[g], vg, (double)dg, 1));
                 v = r;
             } else {
                 float r = 0.0f;
                 if (l > 0)
-                    r = __fadd_rn(r, bnd_change_f32((float)d.bs[l], (float)d.tb[l], (float)d.bnd[l],
-                                                    d.vol[l], (float)dl, -1));
+                    r = __fadd_rn(r, bnd_change_f32((float)gp(d.bs)[l], (float)gp(d.tb)[l], (float)gp(d.bnd)[l], vl, (float)dl, -1));
                 if (g > 0)
-                    r = __fadd_rn(r, bnd_change_f32((float)d.bs[g], (float)d.tb[g], (float)d.bnd[g],
-                                                    d.vol[g], (float)dg, 1));
+                    r = __fadd_rn(r, bnd_change_f32((float)gp(d.bs)[g], (float)gp(d.tb)[g], (float)gp(d.bnd)[g], vg, (float)dg, 1));
                 v = (double)r;
             }
         }
@@ -651,8 +797,6 @@ __device__ inline void resolve_entry(const RepDev &d, uint64_t mcs, uint32_t sub
         }
     }
     const float dh = (float)acc;
-
-    // ConventionalMetropolis (protocols/scientific.jl:372-385), zero drive/kinetic/yield
     const float T = d.temperature;
     const float dd = __fadd_rn(dh, 0.0f);
     float p;
@@ -664,70 +808,142 @@ __device__ inline void resolve_entry(const RepDev &d, uint64_t mcs, uint32_t sub
         p = lr >= 0.0f ? 1.0f : (float)exp((double)lr);
     }
     const Words w = rng_words(d.key_lo, d.key_hi, kStreamAcceptance, mcs, sub, 0, kSiteEntity, e.site, 0, 0);
-    if (!(uniform_open01(w.w0) < p)) {
-        ++cnt.rejected;
-        return;
-    }
-    ++cnt.accepted;
-    // commit (checkerboard.jl:410-440)
-    d.lattice[idx0] = g;
+    return uniform_open01(w.w0) < p;
+}
+
+// Commit of one accepted winning attempt: ownership, volumes, boundary measures,
+// retirement, medium volumes (checkerboard.jl:410-440).
+__device__ inline void commit_attempt(const RepDev &d, const Entry &e, int32_t dl, int32_t dg) {
+    const int32_t l = e.losing, g = e.gaining;
+    gp(d.lattice)[e.site - 1] = g;
     if (l > 0) {
-        const int32_t nv = d.vol[l] - 1;
-        d.vol[l] = nv;
-        d.bnd[l] += dl;
+        const int32_t nv = gp(d.vol)[l] - 1;
+        gp(d.vol)[l] = nv;
+        gp(d.bnd)[l] += dl;
         if (nv == 0) {
-            d.active[l] = 0;
-            d.ctype[l] = 0;
-            d.tv[l] = d.retire_tv;
-            d.vs[l] = d.retire_vs;
-            d.tb[l] = d.retire_tb;
-            d.bs[l] = d.retire_bs;
+            gp(d.active)[l] = 0;
+            gp(d.ctype)[l] = 0;
+            gp(d.tv)[l] = d.retire_tv;
+            gp(d.vs)[l] = d.retire_vs;
+            gp(d.tb)[l] = d.retire_tb;
+            gp(d.bs)[l] = d.retire_bs;
         }
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
     }
     if (g > 0) {
-        d.vol[g] += 1;
-        d.bnd[g] += dg;
+        gp(d.vol)[g] += 1;
+        gp(d.bnd)[g] += dg;
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
     }
 }
 
+constexpr int kCommitTiles = 8;     // tile segments per commit workgroup
+constexpr int kCommitChunk = 1024;  // staged attempts per claim-check round
+
 __global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
-    __shared__ uint32_t s_red[kBlock / 64];
-    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
+    __shared__ uint32_t s_base[kCommitTiles + 1];
+    __shared__ uint32_t s_cnt[kCommitTiles][3];
+    __shared__ uint32_t s_win[kCommitChunk];
+    __shared__ uint32_t s_nwin;
+    const uint32_t nwg = gridDim.x;
+    const uint32_t b = xcd_tile(blockIdx.x, nwg);
     // clear the claim buffer of the next pass (its previous reader finished earlier)
     {
         unsigned long long *next = a.claims + (size_t)(a.parity ^ 1u) * a.total_claims;
-        const uint32_t per = (a.total_claims + gridDim.x - 1) / gridDim.x;
-        const uint32_t lo = blockIdx.x * per;
+        const uint32_t per = (a.total_claims + nwg - 1) / nwg;
+        const uint32_t lo = b * per;
         const uint32_t hi = min(lo + per, a.total_claims);
         for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) next[j] = 0ull;
     }
-    const uint32_t r = a.tile_rep[tile];
-    const RepDev &d = a.reps[r];
-    const RepDyn &y = a.dyn[r];
-    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
-    const uint64_t mcs = y.mcs;
-    const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
-    const unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    const uint32_t n = a.seg_count[tile];
-    const Entry *seg = a.seg + (size_t)tile * a.seg_cap;
-    EvalCounts cnt;
-    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
-        const uint4 v = reinterpret_cast<const uint4 *>(seg)[e];
-        const Entry ent{v.x, v.y, (int32_t)v.z, (int32_t)v.w};
-        resolve_entry(d, mcs, sub, claims, ent, cnt);
+    if (threadIdx.x < kCommitTiles) {
+        const int k = threadIdx.x;
+        const uint32_t tile = b * kCommitTiles + k;
+        uint32_t n = 0;
+        if (tile < a.ntiles) {
+            const uint32_t r = a.tile_rep[tile];
+            if (a.dyn[r].active && a.pass < (uint32_t)a.reps[r].rounds) n = a.seg_count[tile];
+        }
+        s_base[k + 1] = n;
+        s_cnt[k][0] = s_cnt[k][1] = s_cnt[k][2] = 0;
     }
-    const uint32_t c0 = block_sum(cnt.conflicts, s_red);
-    const uint32_t c1 = block_sum(cnt.rejected, s_red);
-    const uint32_t c2 = block_sum(cnt.accepted, s_red);
+    if (threadIdx.x == 0) s_nwin = 0;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-        slot[4] = c0;
-        slot[5] = c1;
-        slot[6] = c2;
+        s_base[0] = 0;
+        for (int k = 1; k <= kCommitTiles; ++k) s_base[k] += s_base[k - 1];
+    }
+    __syncthreads();
+    const uint32_t total = s_base[kCommitTiles];
+    const unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
+    for (uint32_t c0 = 0; c0 < total; c0 += kCommitChunk) {
+        const uint32_t c1 = min(c0 + kCommitChunk, total);
+        // A. claim check of the staged attempts; winners are compacted into LDS
+        for (uint32_t e0 = c0; e0 < c1; e0 += kBlock) {
+            const uint32_t e = e0 + threadIdx.x;
+            bool win = false;
+            uint32_t code = 0;
+            if (e < c1) {
+                int k = 0;
+                while (k + 1 < kCommitTiles && e >= s_base[k + 1]) ++k;
+                const uint32_t tile = b * kCommitTiles + k;
+                const uint32_t i = e - s_base[k];
+                const uint4 v = reinterpret_cast<const uint4 *>(a.seg)[(size_t)tile * a.seg_cap + i];
+                const RepDev &d = a.reps[a.tile_rep[tile]];
+                const unsigned long long key = ((unsigned long long)v.y << 32) | (uint32_t)(~v.x);
+                win = true;
+                if ((int32_t)v.z > 0) win = claims[d.claim_base + v.z] == key;
+                if ((int32_t)v.w > 0) win = win && claims[d.claim_base + v.w] == key;
+                if (!win) atomicAdd(&s_cnt[k][0], 1u);
+                code = ((uint32_t)k << 16) | i;
+            }
+            const uint64_t m = __ballot(win);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                uint32_t base = 0;
+                if ((int)__lane_id() == leader) base = atomicAdd(&s_nwin, (uint32_t)__popcll(m));
+                base = __shfl(base, leader);
+                if (win) s_win[base + __popcll(m & __lanemask_lt())] = code;
+            }
+        }
+        __syncthreads();
+        // B. winners: energy change, acceptance and commit
+        const uint32_t nwin = s_nwin;
+        for (uint32_t e = threadIdx.x; e < nwin; e += kBlock) {
+            const uint32_t code = s_win[e];
+            const int k = (int)(code >> 16);
+            const uint32_t tile = b * kCommitTiles + k;
+            const uint4 v = reinterpret_cast<const uint4 *>(a.seg)[(size_t)tile * a.seg_cap + (code & 0xffffu)];
+            const Entry ent{v.x, v.y, (int32_t)v.z, (int32_t)v.w};
+            const uint32_t r = a.tile_rep[tile];
+            const RepDev &d = a.reps[r];
+            const RepDyn &y = a.dyn[r];
+            const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
+            int32_t dl, dg;
+            if (evaluate_winner(d, ent, y.mcs, sub, &dl, &dg)) {
+                atomicAdd(&s_cnt[k][2], 1u);
+                commit_attempt(d, ent, dl, dg);
+            } else {
+                atomicAdd(&s_cnt[k][1], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_nwin = 0;
+        __syncthreads();
+    }
+    if (threadIdx.x < kCommitTiles) {
+        const int k = threadIdx.x;
+        const uint32_t tile = b * kCommitTiles + k;
+        if (tile < a.ntiles) {
+            const uint32_t r = a.tile_rep[tile];
+            if (a.dyn[r].active && a.pass < (uint32_t)a.reps[r].rounds) {
+                uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+                slot[4] = s_cnt[k][0];
+                slot[5] = s_cnt[k][1];
+                slot[6] = s_cnt[k][2];
+            }
+        }
     }
 }
 
@@ -788,16 +1004,19 @@ __global__ __launch_bounds__(kBlock) void report_kernel(const RepDev *reps, cons
 // Launchers
 // ---------------------------------------------------------------------------
 template <int DIM, int H, int TX, int TY, int TZ>
-static hipError_t launch_pass_t(int algorithm, const PassArgs &a, hipStream_t stream) {
+static hipError_t launch_propose_t(int algorithm, const PassArgs &a, hipStream_t stream) {
     if (algorithm == CPM_ALG_LOTTERY)
         hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ>), dim3(a.ntiles), dim3(kBlock), 0,
                            stream, a);
     else
         hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ>), dim3(a.ntiles), dim3(kBlock), 0,
                            stream, a);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(commit_kernel, dim3(a.ntiles), dim3(kBlock), 0, stream, a);
+    return hipGetLastError();
+}
+
+static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
+    const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
+    hipLaunchKernelGGL(commit_kernel, dim3(wgs), dim3(kBlock), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -819,20 +1038,31 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
     return 1;
 }
 
-extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const cpm::PassArgs *a,
-                                               hipStream_t stream) {
+extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
+                                                  hipStream_t stream) {
     using namespace cpm;
     switch (config) {
-    case 0: return launch_pass_t<2, 1, 128, 16, 1>(algorithm, *a, stream);
-    case 1: return launch_pass_t<2, 2, 128, 16, 1>(algorithm, *a, stream);
-    case 2: return launch_pass_t<3, 1, 32, 8, 8>(algorithm, *a, stream);
-    case 3: return launch_pass_t<3, 2, 32, 8, 8>(algorithm, *a, stream);
-    case 4: return launch_pass_t<2, 1, 64, 32, 1>(algorithm, *a, stream);
-    case 5: return launch_pass_t<2, 1, 256, 8, 1>(algorithm, *a, stream);
-    case 6: return launch_pass_t<2, 1, 64, 16, 1>(algorithm, *a, stream);
-    case 7: return launch_pass_t<2, 1, 32, 32, 1>(algorithm, *a, stream);
+    case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, *a, stream);
+    case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, *a, stream);
+    case 2: return launch_propose_t<3, 1, 32, 8, 8>(algorithm, *a, stream);
+    case 3: return launch_propose_t<3, 2, 32, 8, 8>(algorithm, *a, stream);
+    case 4: return launch_propose_t<2, 1, 64, 32, 1>(algorithm, *a, stream);
+    case 5: return launch_propose_t<2, 1, 256, 8, 1>(algorithm, *a, stream);
+    case 6: return launch_propose_t<2, 1, 64, 16, 1>(algorithm, *a, stream);
+    case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, *a, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+extern "C" hipError_t cpm_internal_launch_commit(const cpm::PassArgs *a, hipStream_t stream) {
+    return cpm::launch_commit(*a, stream);
+}
+
+extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const cpm::PassArgs *a,
+                                               hipStream_t stream) {
+    hipError_t err = cpm_internal_launch_propose(config, algorithm, a, stream);
+    if (err != hipSuccess) return err;
+    return cpm::launch_commit(*a, stream);
 }
 
 extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,

@@ -156,6 +156,13 @@ int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors);
  * d2h_transfers, device_bytes_allocated, graph_launches]. */
 int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]);
 
+/* Like cpm_step, but records HIP events on the context stream around every kernel and
+ * synchronizes at the end. out[0..7] = propose-kernel total ms, propose launches,
+ * commit-kernel total ms, commit launches, begin-kernel total ms, begin launches,
+ * wall ms from the first to the last event, MCS run. (No counterpart in the reference:
+ * the measurement hook behind the per-kernel roofline.) */
+int32_t cpm_step_timed(cpm_ctx *ctx, uint64_t n, double out[8]);
+
 /* Capture chunks of `chunk` MCS into a HIP graph that cpm_step replays when n is a
  * multiple of chunk (0 disables graphs). */
 int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk);
