@@ -28,6 +28,19 @@ enum : uint32_t {
 };
 enum : uint32_t { kGlobalEntity = 0, kSiteEntity = 1 };
 
+// Division by a runtime-uniform divisor via a precomputed reciprocal: q = umulhi(n, m)
+// with m = floor(2^32 / d) undershoots the quotient by at most one.
+struct FastDiv {
+    uint32_t d;
+    uint32_t m;
+};
+
+// Bounded draw constants: rejection threshold (2^32 - bound) mod bound and divisor.
+struct Bound {
+    uint32_t threshold;
+    FastDiv div;
+};
+
 // Relation offsets compiled for the device (canonical direction order kept).
 struct RelDev {
     int32_t n;
@@ -53,7 +66,12 @@ struct RepDev {
     float temperature;
     uint32_t key_lo, key_hi;     // Philox key = seed ^ mix64(generation domain)
     int32_t pat[3];              // periodic colour pattern periods (0: per-site array)
+    int32_t pat_aligned;         // every period divides the tile side: class offsets = residues
     uint8_t pattern[64];
+    uint8_t pat_coord[64][3];    // residue coordinates of each pattern index
+    Bound prop_bound;            // direction draw over the proposal relation
+    Bound round_bound;           // lottery activation draw over the round count
+    FastDiv div_x, div_xy;       // site -> coordinates
     int32_t n_energies;
     int32_t energy_order[3];
     int32_t volume_f64, contact_f64, boundary_f64;
@@ -101,10 +119,16 @@ struct Entry {
     int32_t gaining;    // owner word of the donor
 };
 
+// Per-tile launch data: replica slot and tile origin.
+struct TileInfo {
+    uint32_t rep;
+    int32_t x0, y0, z0;
+};
+
 struct PassArgs {
     const RepDev *reps;
     RepDyn *dyn;
-    const uint32_t *tile_rep;   // group tile -> replica slot
+    const TileInfo *tiles;      // group tile -> replica slot and origin
     unsigned long long *claims; // two buffers of total_claims each
     Entry *seg;                 // per-tile segments of seg_cap entries
     uint32_t *seg_count;
@@ -168,6 +192,32 @@ __device__ __forceinline__ Words rng_words(uint32_t key_lo, uint32_t key_hi, uin
     const uint32_t c2 = entity;
     const uint32_t c3 = operation | (stream << 12) | (kind << 20) | (draw << 22);
     return philox10(c0, c1, c2, c3, key_lo, key_hi);
+}
+
+__device__ __forceinline__ uint32_t fast_mod(uint32_t n, FastDiv f) {
+    if (f.d == 1) return 0;
+    const uint32_t q = __umulhi(n, f.m);
+    uint32_t r = n - q * f.d;
+    return r >= f.d ? r - f.d : r;
+}
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
+    if (f.d == 1) return n;
+    uint32_t q = __umulhi(n, f.m);
+    return n - q * f.d >= f.d ? q + 1 : q;
+}
+
+// bounded_uint with host-precomputed threshold and divisor (rng/semantic.jl:298-310).
+__device__ __forceinline__ uint32_t bounded_draw(uint32_t key_lo, uint32_t key_hi, uint32_t stream,
+                                                 uint64_t mcs, uint32_t subround, uint32_t operation,
+                                                 uint32_t kind, uint32_t entity, uint32_t draw,
+                                                 Bound b) {
+    uint32_t invocation = 0;
+    while (true) {
+        const Words w = rng_words(key_lo, key_hi, stream, mcs, subround, operation, kind,
+                                  entity, invocation, draw);
+        if (w.w0 >= b.threshold || invocation == 255u) return fast_mod(w.w0, b.div);
+        ++invocation;
+    }
 }
 
 // bounded_uint (rng/semantic.jl:298-310): reject below (2^32 - bound) mod bound,

@@ -72,6 +72,20 @@ void philox_host(uint32_t c[4], uint32_t k0, uint32_t k1) {
 
 constexpr uint64_t kGenerationDomain = 0xd2b74407b1ce6e93ull;
 
+cpm::FastDiv make_div(uint32_t d) {
+    cpm::FastDiv f;
+    f.d = d;
+    f.m = d <= 1 ? 0u : (uint32_t)((1ull << 32) / d);
+    return f;
+}
+
+cpm::Bound make_bound(uint32_t bound) {
+    cpm::Bound b;
+    b.threshold = (uint32_t)(0u - bound) % bound;
+    b.div = make_div(bound);
+    return b;
+}
+
 template <typename T>
 T *dev_alloc(size_t count, uint64_t &bytes) {
     if (count == 0) count = 1;
@@ -434,6 +448,16 @@ void compile_replica(Replica &r) {
     } else {
         v.pat[0] = v.pat[1] = v.pat[2] = 1;
     }
+    for (int i = 0; i < v.pat[0] * v.pat[1] * v.pat[2]; ++i) {
+        v.pat_coord[i][0] = (uint8_t)(i % v.pat[0]);
+        v.pat_coord[i][1] = (uint8_t)((i / v.pat[0]) % v.pat[1]);
+        v.pat_coord[i][2] = (uint8_t)(i / (v.pat[0] * v.pat[1]));
+    }
+    v.pat_aligned = (tx % v.pat[0] == 0 && ty % v.pat[1] == 0 && tz % v.pat[2] == 0) ? 1 : 0;
+    v.prop_bound = make_bound((uint32_t)d.proposal.count);
+    v.round_bound = make_bound((uint32_t)std::max(r.rounds, 1));
+    v.div_x = make_div((uint32_t)v.dims[0]);
+    v.div_xy = make_div((uint32_t)v.dims[0] * (uint32_t)v.dims[1]);
     v.n_energies = d.n_energies;
     for (int t = 0; t < 3; ++t) v.energy_order[t] = d.energy_order[t];
     v.volume_f64 = d.volume_f64 ? 1 : 0;
@@ -539,7 +563,7 @@ struct Group {
     std::vector<int> slots;
     uint32_t ntiles = 0, total_claims = 0, seg_cap = 0;
     int max_rounds = 0;
-    uint32_t *d_tile_rep = nullptr;
+    cpm::TileInfo *d_tiles = nullptr;
     unsigned long long *d_claims = nullptr;
     cpm::Entry *d_seg = nullptr;
     uint32_t *d_seg_count = nullptr;
@@ -551,9 +575,9 @@ struct Group {
     bool dirty_arena_fresh = true;  // no accounting written since the last rebuild
 
     void release() {
-        dev_free(d_tile_rep); dev_free(d_claims); dev_free(d_seg); dev_free(d_seg_count);
+        dev_free(d_tiles); dev_free(d_claims); dev_free(d_seg); dev_free(d_seg_count);
         dev_free(d_slots);
-        d_tile_rep = nullptr; d_claims = nullptr; d_seg = nullptr; d_seg_count = nullptr;
+        d_tiles = nullptr; d_claims = nullptr; d_seg = nullptr; d_seg_count = nullptr;
         d_slots = nullptr;
         cap_tiles = cap_claims = cap_slots = cap_seg = 0;
         bytes = 0;
@@ -735,7 +759,7 @@ struct Engine {
         std::sort(g.slots.begin(), g.slots.end());
         uint32_t tiles = 0, claims = 0;
         int maxr = 1;
-        std::vector<uint32_t> tile_rep;
+        std::vector<cpm::TileInfo> tile_info;
         for (int s : g.slots) {
             Replica *r = reps[s];
             cpm::RepDev &v = r->dev;
@@ -745,7 +769,13 @@ struct Engine {
             tiles += v.tile_count;
             claims += r->desc.capacity + 1;
             maxr = std::max(maxr, r->rounds);
-            tile_rep.insert(tile_rep.end(), v.tile_count, (uint32_t)s);
+            for (uint32_t t = 0; t < v.tile_count; ++t) {
+                const uint32_t tx = t % (uint32_t)v.ntiles[0];
+                const uint32_t ty = (t / (uint32_t)v.ntiles[0]) % (uint32_t)v.ntiles[1];
+                const uint32_t tz = t / ((uint32_t)v.ntiles[0] * (uint32_t)v.ntiles[1]);
+                tile_info.push_back(cpm::TileInfo{(uint32_t)s, (int32_t)(tx * g.tx), (int32_t)(ty * g.ty),
+                                                  (int32_t)(tz * g.tz)});
+            }
         }
         g.ntiles = tiles;
         g.total_claims = std::max(claims, 1u);
@@ -758,7 +788,7 @@ struct Engine {
         if (need_tiles > g.cap_tiles || 2 * (size_t)g.total_claims > g.cap_claims || need_slots > g.cap_slots ||
             need_seg > g.cap_seg) {
             g.release();
-            g.d_tile_rep = dev_alloc<uint32_t>(need_tiles, g.bytes);
+            g.d_tiles = dev_alloc<cpm::TileInfo>(need_tiles, g.bytes);
             g.d_seg_count = dev_alloc<uint32_t>(need_tiles, g.bytes);
             g.d_claims = dev_alloc<unsigned long long>(2 * (size_t)g.total_claims, g.bytes);
             g.d_slots = dev_alloc<uint32_t>(need_slots, g.bytes);
@@ -772,7 +802,8 @@ struct Engine {
         hip_check(hipMemsetAsync(g.d_slots, 0, need_slots * 4, stream), "memset slots");
         hip_check(hipMemsetAsync(g.d_seg_count, 0, need_tiles * 4, stream), "memset seg");
         if (tiles)
-            hip_check(hipMemcpy(g.d_tile_rep, tile_rep.data(), tiles * 4, hipMemcpyHostToDevice), "tile map");
+            hip_check(hipMemcpy(g.d_tiles, tile_info.data(), tiles * sizeof(cpm::TileInfo), hipMemcpyHostToDevice),
+                      "tile map");
         for (int s : g.slots) upload_rep(s);
         g.pass_counter = 0;
         g.dirty = false;
@@ -790,7 +821,7 @@ struct Engine {
         cpm::PassArgs a{};
         a.reps = d_reps;
         a.dyn = d_dyn;
-        a.tile_rep = g.d_tile_rep;
+        a.tiles = g.d_tiles;
         a.claims = g.d_claims;
         a.seg = g.d_seg;
         a.seg_count = g.d_seg_count;

@@ -314,10 +314,13 @@ __device__ inline void for_colour_sites(const RepDev &d, const uint8_t *s_patter
                           ((TY / py) & (TY / py - 1)) == 0 && ((TZ / pz) & (TZ / pz - 1)) == 0;
         for (int r = 0; r < np; ++r) {
             if (s_pattern[r] != colour) continue;
-            const int rx = r % px, ry = (r / px) % py, rz = r / (px * py);
-            const int fx = (rx - x0 % px + px) % px;
-            const int fy = (ry - y0 % py + py) % py;
-            const int fz = (rz - z0 % pz + pz) % pz;
+            const int rx = d.pat_coord[r][0], ry = d.pat_coord[r][1], rz = d.pat_coord[r][2];
+            int fx = rx, fy = ry, fz = rz;
+            if (!d.pat_aligned) {
+                fx = (rx - x0 % px + px) % px;
+                fy = (ry - y0 % py + py) % py;
+                fz = (rz - z0 % pz + pz) % pz;
+            }
             const int nx = fx < TX ? (TX - fx + px - 1) / px : 0;
             const int ny = fy < TY ? (TY - fy + py - 1) / py : 0;
             const int nz = fz < TZ ? (TZ - fz + pz - 1) / pz : 0;
@@ -363,6 +366,42 @@ __device__ inline void for_colour_sites(const RepDev &d, const uint8_t *s_patter
     }
 }
 
+// Up to kRegDirs proposal offsets kept in (scalar) registers for the interior test.
+constexpr int kRegDirs = 8;
+
+template <int LX, int LY>
+__device__ __forceinline__ void reg_offsets(const RelDev &rel, int32_t (&po)[kRegDirs]) {
+#pragma unroll
+    for (int k = 0; k < kRegDirs; ++k)
+        po[k] = k < rel.n ? rel.off[k][0] + LX * (rel.off[k][1] + LY * rel.off[k][2]) : 0;
+}
+
+// True when every proposal neighbour of pos is a mutable site owned by `own`: any
+// direction draw would then yield a same-owner no-op. Branch-free, independent LDS reads.
+template <bool SIMPLE>
+__device__ __forceinline__ bool interior_site(const int32_t *s_own, const uint8_t *s_mut, int pos,
+                                              int32_t own, const int32_t (&po)[kRegDirs], int K,
+                                              const int32_t *s_ploff) {
+    bool interior = true;
+    if (K <= kRegDirs) {
+#pragma unroll
+        for (int k = 0; k < kRegDirs; ++k) {
+            if (k < K) {
+                const int q = pos + po[k];
+                interior &= s_own[q] == own;
+                if (!SIMPLE) interior &= s_mut[q] != 0;
+            }
+        }
+    } else {
+        for (int k = 0; k < K; ++k) {
+            const int q = pos + s_ploff[k];
+            interior &= s_own[q] == own;
+            if (!SIMPLE) interior &= s_mut[q] != 0;
+        }
+    }
+    return interior;
+}
+
 // ---------------------------------------------------------------------------
 // Checkerboard colour pass, propose + claim (checkerboard.jl:244-304)
 // ---------------------------------------------------------------------------
@@ -380,18 +419,15 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     __shared__ uint32_t s_red[4][kBlock / 64];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
-    const uint32_t r = a.tile_rep[tile];
+    const TileInfo ti = a.tiles[tile];
+    const uint32_t r = ti.rep;
     const RepDev &d = a.reps[r];
     const RepDyn &y = a.dyn[r];
     if (!y.active || a.pass >= (uint32_t)d.rounds) return;
     const uint64_t mcs = y.mcs;
     const uint32_t colour = y.order[a.pass];
 
-    const uint32_t lt = tile - d.tile_base;
-    const int ntx = d.ntiles[0], nty = d.ntiles[1];
-    const int x0 = (int)(lt % ntx) * TX;
-    const int y0 = (int)((lt / ntx) % nty) * TY;
-    const int z0 = (int)(lt / (ntx * nty)) * TZ;
+    const int x0 = ti.x0, y0 = ti.y0, z0 = ti.z0;
     const int X = d.dims[0], Y = d.dims[1];
     const int K = d.prop.n;
     const bool simple = d.simple;
@@ -404,6 +440,8 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     for (int k = threadIdx.x; k < K; k += kBlock)
         s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
     if (threadIdx.x < 64) s_pattern[threadIdx.x] = d.pattern[threadIdx.x];
+    int32_t po[kRegDirs];
+    reg_offsets<G::LX, G::LY>(d.prop, po);
     load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, nullptr);
     __syncthreads();
 
@@ -417,14 +455,8 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
             if (simple || s_mut[pos]) {
                 ++cnt[0];
                 const int32_t own = s_own[pos];
-                bool interior = true;
-                for (int k = 0; k < K; ++k) {
-                    const int q = pos + s_ploff[k];
-                    if ((!simple && !s_mut[q]) || s_own[q] != own) {
-                        interior = false;
-                        break;
-                    }
-                }
+                const bool interior = simple ? interior_site<true>(s_own, s_mut, pos, own, po, K, s_ploff)
+                                             : interior_site<false>(s_own, s_mut, pos, own, po, K, s_ploff);
                 if (interior) ++cnt[1];
                 else need = true;
             }
@@ -445,9 +477,16 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
             const int pos = G::pos(ix, iy, iz);
             const uint32_t site = 1u + (uint32_t)(x0 + ix) +
                                   (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
-            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0,
-                                              kSiteEntity, site, 0, (uint32_t)K);
-            const int q = pos + s_ploff[dir];
+            const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0,
+                                              kSiteEntity, site, 0, d.prop_bound);
+            int off = 0;
+            if (K <= kRegDirs) {
+#pragma unroll
+                for (int k = 0; k < kRegDirs; ++k) off = dir == (uint32_t)k ? po[k] : off;
+            } else {
+                off = s_ploff[dir];
+            }
+            const int q = pos + off;
             if (!simple && !s_mut[q]) {
                 ++cnt[2];
             } else if (s_own[q] == s_own[pos]) {
@@ -524,7 +563,8 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     __shared__ uint32_t s_red[4][kBlock / 64];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
-    const uint32_t r = a.tile_rep[tile];
+    const TileInfo ti = a.tiles[tile];
+    const uint32_t r = ti.rep;
     const RepDev &d = a.reps[r];
     const RepDyn &y = a.dyn[r];
     if (!y.active || a.pass >= (uint32_t)d.rounds) return;
@@ -532,11 +572,7 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     const uint32_t sub = y.order[a.pass] - 1;
     const uint32_t R = (uint32_t)d.rounds;
 
-    const uint32_t lt = tile - d.tile_base;
-    const int ntx = d.ntiles[0], nty = d.ntiles[1];
-    const int x0 = (int)(lt % ntx) * TX;
-    const int y0 = (int)((lt / ntx) % nty) * TY;
-    const int z0 = (int)(lt / (ntx * nty)) * TZ;
+    const int x0 = ti.x0, y0 = ti.y0, z0 = ti.z0;
     const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
     const int K = d.prop.n;
     const int KC = d.conf.n;
@@ -549,6 +585,8 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
         s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
     for (int k = threadIdx.x; k < KC; k += kBlock)
         s_cloff[k] = d.conf.off[k][0] + G::LX * (d.conf.off[k][1] + G::LY * d.conf.off[k][2]);
+    int32_t po[kRegDirs];
+    reg_offsets<G::LX, G::LY>(d.prop, po);
     load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, s_sid);
     __syncthreads();
     // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1)
@@ -600,21 +638,13 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
                         bool eligible = true;
                         if (!d.uniform_degree) {
                             const uint32_t degree = gp(d.site_degree)[site - 1];
-                            eligible = bounded_uint(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs,
-                                                    sub, 0, kSiteEntity, site, 0, R) < degree + 1;
+                            eligible = bounded_draw(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs,
+                                                    sub, 0, kSiteEntity, site, 0, d.round_bound) < degree + 1;
                         }
                         if (eligible) {
                             ++cnt[0];
                             const int32_t own = s_own[pos];
-                            bool interior = true;
-                            for (int k = 0; k < K; ++k) {
-                                const int q = pos + s_ploff[k];
-                                if (!s_mut[q] || s_own[q] != own) {
-                                    interior = false;
-                                    break;
-                                }
-                            }
-                            if (interior) ++cnt[1];
+                            if (interior_site<false>(s_own, s_mut, pos, own, po, K, s_ploff)) ++cnt[1];
                             else need = true;
                         }
                     }
@@ -637,8 +667,8 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
             const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
             const int pos = G::pos(ix, iy, iz);
             const uint32_t site = s_sid[pos];
-            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
-                                              kSiteEntity, site, 0, (uint32_t)K);
+            const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
+                                              kSiteEntity, site, 0, d.prop_bound);
             const int q = pos + s_ploff[dir];
             if (!s_mut[q]) {
                 ++cnt[2];
@@ -696,10 +726,12 @@ __device__ inline bool evaluate_winner(const RepDev &d, const Entry &e, uint64_t
                                        int32_t *dl_out, int32_t *dg_out) {
     const int32_t l = e.losing, g = e.gaining;
     const uint32_t idx0 = e.site - 1;
-    const uint32_t X = (uint32_t)d.dims[0], Y = (uint32_t)d.dims[1];
-    const int x = (int)(idx0 % X);
-    const int y = (int)((idx0 / X) % Y);
-    const int z = (int)(idx0 / (X * Y));
+    const uint32_t zq = fast_div(idx0, d.div_xy);
+    const uint32_t rem = idx0 - zq * d.div_xy.d;
+    const uint32_t yq = fast_div(rem, d.div_x);
+    const int x = (int)(rem - yq * d.div_x.d);
+    const int y = (int)yq;
+    const int z = (int)zq;
     const int32_t vl = l > 0 ? gp(d.vol)[l] : 0;
     const int32_t vg = g > 0 ? gp(d.vol)[g] : 0;
     // boundary tracker deltas (BoundaryEdgeCount, scientific_components.jl:466-481)
@@ -862,7 +894,7 @@ __global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
         const uint32_t tile = b * kCommitTiles + k;
         uint32_t n = 0;
         if (tile < a.ntiles) {
-            const uint32_t r = a.tile_rep[tile];
+            const uint32_t r = a.tiles[tile].rep;
             if (a.dyn[r].active && a.pass < (uint32_t)a.reps[r].rounds) n = a.seg_count[tile];
         }
         s_base[k + 1] = n;
@@ -890,7 +922,7 @@ __global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
                 const uint32_t tile = b * kCommitTiles + k;
                 const uint32_t i = e - s_base[k];
                 const uint4 v = reinterpret_cast<const uint4 *>(a.seg)[(size_t)tile * a.seg_cap + i];
-                const RepDev &d = a.reps[a.tile_rep[tile]];
+                const RepDev &d = a.reps[a.tiles[tile].rep];
                 const unsigned long long key = ((unsigned long long)v.y << 32) | (uint32_t)(~v.x);
                 win = true;
                 if ((int32_t)v.z > 0) win = claims[d.claim_base + v.z] == key;
@@ -910,13 +942,14 @@ __global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
         __syncthreads();
         // B. winners: energy change, acceptance and commit
         const uint32_t nwin = s_nwin;
+        // winners are packed densely into as few waves as possible
         for (uint32_t e = threadIdx.x; e < nwin; e += kBlock) {
             const uint32_t code = s_win[e];
             const int k = (int)(code >> 16);
             const uint32_t tile = b * kCommitTiles + k;
             const uint4 v = reinterpret_cast<const uint4 *>(a.seg)[(size_t)tile * a.seg_cap + (code & 0xffffu)];
             const Entry ent{v.x, v.y, (int32_t)v.z, (int32_t)v.w};
-            const uint32_t r = a.tile_rep[tile];
+            const uint32_t r = a.tiles[tile].rep;
             const RepDev &d = a.reps[r];
             const RepDyn &y = a.dyn[r];
             const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
@@ -936,7 +969,7 @@ __global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
         const int k = threadIdx.x;
         const uint32_t tile = b * kCommitTiles + k;
         if (tile < a.ntiles) {
-            const uint32_t r = a.tile_rep[tile];
+            const uint32_t r = a.tiles[tile].rep;
             if (a.dyn[r].active && a.pass < (uint32_t)a.reps[r].rounds) {
                 uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
                 slot[4] = s_cnt[k][0];
