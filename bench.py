@@ -11,7 +11,10 @@ sides. Attempts per MCS = activated_attempts of the report (one per mutable site
 
 Roofline: the dominant kernel is the colour-pass propose kernel. Its algorithmic traffic is
 one read of the Int32 lattice per launch (4 bytes x sites); its average duration comes from
-HIP events recorded around every kernel on the engine stream (cpm_step_timed).
+HIP events recorded around every kernel on the engine stream (cpm_step_timed). `traffic` is
+the measured HBM bytes per propose launch (FETCH_SIZE + WRITE_SIZE hardware counters, each
+scaled by what the counter reads for the library's plain copy kernel of known size), taken
+from profiles/cb_propose_4096_traffic.json when that file is present.
 """
 import argparse
 import json
@@ -33,12 +36,12 @@ def log(msg):
 
 
 def traffic_from_profiles():
-    """HBM bytes per propose launch from the committed PMC summary, if present."""
+    """Measured HBM bytes per propose launch from the committed counter summary, if present."""
     path = os.path.join(ROOT, "profiles", "cb_propose_4096_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as handle:
-        return json.load(handle).get("bytes_per_launch")
+        return json.load(handle).get("propose", {}).get("bytes_per_launch")
 
 
 def cpu_baseline(sample_mcs: int):
@@ -69,6 +72,7 @@ def main():
     ap.add_argument("--cpu-sample-mcs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lottery", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=256)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -77,7 +81,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from corepotts import init_scientific, workloads
+    from corepotts import copy_probe, init_scientific, workloads
+
+    # plain device copy of known size: achievable-bandwidth figure and the calibration
+    # kernel for the traffic counters of a profiled run
+    copy_bytes = args.copy_probe_mib << 20
+    copy_ms = copy_probe(copy_bytes, 10)
 
     log("building 4096x4096 workload")
     compiled, proposal, components, alg, seed = workloads.config2_large()
@@ -112,6 +121,8 @@ def main():
 
     extra = {
         "mcs_per_s": args.steps / elapsed,
+        "copy_probe": {"bytes": copy_bytes, "avg_ms": copy_ms,
+                       "GBps": 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9},
         "colour_passes_per_mcs": passes,
         "propose_kernel_avg_us": propose_avg_ms * 1e3,
         "commit_kernel_avg_us": commit_avg_ms * 1e3,

@@ -15,7 +15,7 @@ from .model import (AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, Car
                     required_properties, static_relation, tracker_conformance_errors,
                     validate_relation_domain, von_neumann_offsets)
 from .spec import ModelSpec, build_spec, state_arrays
-from .native import (EngineError, ScientificMCSReport, ScientificPottsIntegrator, ensemble_seed,
+from .native import (EngineError, ScientificMCSReport, ScientificPottsIntegrator, copy_probe, ensemble_seed,
                      init_scientific, load_library)
 
 

@@ -54,7 +54,7 @@ EXPORTS = (
     "cpm_last_error", "cpm_abi_version", "cpm_create", "cpm_destroy", "cpm_upload_state",
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
     "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
-    "cpm_set_graph_chunk",
+    "cpm_set_graph_chunk", "cpm_copy_probe",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_step", "cpm_batch_progress",
@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_metrics.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.cpm_set_graph_chunk.argtypes = [P, C.c_uint32]
     lib.cpm_step_timed.argtypes = [P, C.c_uint64, C.POINTER(C.c_double)]
+    lib.cpm_copy_probe.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
     lib.cpm_ensemble_seed.restype = C.c_uint64
     lib.cpm_ensemble_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
     lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
@@ -361,3 +362,13 @@ def ensemble_seed(master_seed: int, trajectory: int, rerun: int = 1) -> int:
     if not 1 <= rerun <= 256:
         raise ValueError("ensemble rerun identity exceeds the v1 address domain")
     return int(load_library().cpm_ensemble_seed(int(master_seed), int(trajectory), int(rerun)))
+
+
+def copy_probe(nbytes: int, reps: int = 10) -> float:
+    """Mean duration in ms of a plain device copy of `nbytes` bytes (calibration kernel)."""
+    lib = load_library()
+    out = C.c_double()
+    rc = lib.cpm_copy_probe(int(nbytes), int(reps), C.byref(out))
+    if rc != 0:
+        raise EngineError(rc, lib.cpm_last_error(None).decode())
+    return out.value

@@ -119,6 +119,9 @@ struct Entry {
     int32_t gaining;    // owner word of the donor
 };
 
+// Four packed int32 lanes: one 16-byte global load or store.
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+
 // Per-tile launch data: replica slot and tile origin.
 struct TileInfo {
     uint32_t rep;

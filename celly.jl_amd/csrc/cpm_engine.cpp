@@ -25,6 +25,8 @@ extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::Re
 extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
                                                   uint64_t target, uint32_t in_use, uint32_t mask,
                                                   hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_copy_probe(const void *src, void *dst, size_t n4,
+                                                     hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_report(const cpm::RepDev *reps, const uint32_t *slots,
                                                  uint32_t ntiles, int slot,
                                                  unsigned long long *out, hipStream_t stream);
@@ -1304,6 +1306,40 @@ uint64_t cpm_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t r
     uint32_t c[4] = {(uint32_t)c0, (uint32_t)(c0 >> 32), (uint32_t)c1, (uint32_t)(c1 >> 32)};
     philox_host(c, (uint32_t)key, (uint32_t)(key >> 32));
     return (uint64_t)c[0] | ((uint64_t)c[1] << 32);
+}
+
+int32_t cpm_copy_probe(uint64_t bytes, int32_t reps, double *out_ms) {
+    if (!out_ms || reps < 1 || bytes < 16 || bytes % 16) {
+        g_create_error = "cpm_copy_probe: bytes must be a positive multiple of 16, reps >= 1";
+        return CPM_E_INVALID;
+    }
+    void *src = nullptr, *dst = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    const int32_t rc = guarded(&g_create_error, [&] {
+        hip_check(hipMalloc(&src, bytes), "copy probe alloc");
+        hip_check(hipMalloc(&dst, bytes), "copy probe alloc");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        hip_check(hipEventCreate(&ev[0]), "event");
+        hip_check(hipEventCreate(&ev[1]), "event");
+        hip_check(hipMemsetAsync(src, 0x5a, bytes, stream), "memset");
+        // one untimed launch, then `reps` timed launches
+        hip_check(cpm_internal_launch_copy_probe(src, dst, bytes / 16, stream), "copy probe");
+        hip_check(hipEventRecord(ev[0], stream), "event");
+        for (int32_t i = 0; i < reps; ++i)
+            hip_check(cpm_internal_launch_copy_probe(src, dst, bytes / 16, stream), "copy probe");
+        hip_check(hipEventRecord(ev[1], stream), "event");
+        hip_check(hipEventSynchronize(ev[1]), "sync");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
+        *out_ms = (double)ms / reps;
+    });
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return rc;
 }
 
 // ---------------- batch ----------------

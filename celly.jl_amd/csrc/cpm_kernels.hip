@@ -108,7 +108,6 @@ __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_
     if (d.simple && (X & 3) == 0 && x0 >= G::PX && x0 + TX + G::PX <= X && inner_yz) {
         // interior tile of a periodic, obstacle-free lattice: 16-byte loads, all in flight
         constexpr int NL = (G::NV + kBlock - 1) / kBlock;
-        typedef int32_t v4i __attribute__((ext_vector_type(4)));
         v4i v[NL];
 #pragma unroll
         for (int j = 0; j < NL; ++j) {
@@ -570,7 +569,6 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     if (!y.active || a.pass >= (uint32_t)d.rounds) return;
     const uint64_t mcs = y.mcs;
     const uint32_t sub = y.order[a.pass] - 1;
-    const uint32_t R = (uint32_t)d.rounds;
 
     const int x0 = ti.x0, y0 = ti.y0, z0 = ti.z0;
     const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
@@ -1110,6 +1108,26 @@ extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, ui
                                                   hipStream_t stream) {
     hipLaunchKernelGGL(cpm::set_dyn_kernel, dim3(1), dim3(1), 0, stream, dyn, slot, mcs, target,
                        in_use, mask);
+    return hipGetLastError();
+}
+
+namespace cpm {
+// Plain 16-byte-per-lane device copy: the calibration kernel of known traffic
+// (reads and writes n4 * 16 bytes) for hardware traffic counters and bandwidth probes.
+__global__ __launch_bounds__(kBlock) void copy_probe_kernel(const v4i *__restrict__ src,
+                                                            v4i *__restrict__ dst, size_t n4) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += stride)
+        gp(dst)[i] = gp(src)[i];
+}
+}  // namespace cpm
+
+extern "C" hipError_t cpm_internal_launch_copy_probe(const void *src, void *dst, size_t n4,
+                                                     hipStream_t stream) {
+    const size_t want = (n4 + cpm::kBlock - 1) / cpm::kBlock;
+    const unsigned blocks = (unsigned)(want < 8192 ? (want ? want : 1) : 8192);
+    hipLaunchKernelGGL(cpm::copy_probe_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream,
+                       (const cpm::v4i *)src, (cpm::v4i *)dst, n4);
     return hipGetLastError();
 }
 

@@ -163,6 +163,13 @@ int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]);
  * the measurement hook behind the per-kernel roofline.) */
 int32_t cpm_step_timed(cpm_ctx *ctx, uint64_t n, double out[8]);
 
+/* Device-to-device copy of `bytes` bytes (a multiple of 16) with a plain 16-byte-per-lane
+ * kernel, launched once untimed and `reps` times timed; *out_ms = mean launch time. Errors
+ * are reported through cpm_last_error(NULL). (No counterpart in the reference: the
+ * calibration kernel of known traffic for hardware counters and the achievable-bandwidth
+ * figure beside the roofline.) */
+int32_t cpm_copy_probe(uint64_t bytes, int32_t reps, double *out_ms);
+
 /* Capture chunks of `chunk` MCS into a HIP graph that cpm_step replays when n is a
  * multiple of chunk (0 disables graphs). */
 int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk);
