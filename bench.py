@@ -61,6 +61,82 @@ def cpu_baseline(sample_mcs: int):
                       f"state, C oracle (single thread), {dt:.1f} s"}
 
 
+def state_digest(state: dict) -> str:
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for key in ("tags", "ids", "finite_volumes", "medium_volumes", "boundary_measures"):
+        h.update(state[key].tobytes())
+    return h.hexdigest()
+
+
+def batched_leg(replicas: int, mcs: int, graph_chunk: int):
+    """Batched ensemble of `replicas` 2D replicas (sides log-uniform in [64, 1024], T, J and
+    seed per replica) for `mcs` MCS in one ReplicaBatch, against the same replicas run one
+    after another through the plain single-lattice path on the same GPU."""
+    from corepotts import ReplicaBatch, init_scientific, workloads
+    from corepotts.spec import ALG_CHECKERBOARD
+    log(f"batched: building {replicas} replicas")
+    t0 = time.perf_counter()
+    models = [workloads.ensemble_replica(r) for r in range(replicas)]
+    sites = [int(m[0].tags.size) for m in models]
+    log(f"batched: built in {time.perf_counter() - t0:.1f} s, {sum(sites) / 1e6:.1f} M sites")
+
+    batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=graph_chunk)
+    slots = [batch.add_model(*m, target_mcs=mcs) for m in models]
+    batch.synchronize()
+    log(f"batched: {mcs} MCS")
+    t0 = time.perf_counter()
+    batch.step(mcs)
+    batch.synchronize()
+    batched_s = time.perf_counter() - t0
+    digests = [state_digest(batch.download(s)) for s in slots]
+    device_bytes = batch.device_bytes()
+    batch.close()
+
+    log("plain: replicas one after another")
+    plain_s = 0.0
+    same = 0
+    for m, d in zip(models, digests):
+        integ = init_scientific(*m)
+        integ.set_graph_chunk(graph_chunk)
+        integ.synchronize()
+        t0 = time.perf_counter()
+        integ.step(mcs)
+        integ.synchronize()
+        plain_s += time.perf_counter() - t0
+        same += state_digest(integ.download()) == d
+        integ.close()
+    attempts = sum(sites) * mcs
+    return {"replicas": replicas, "mcs": mcs, "sites": sum(sites), "graph_chunk": graph_chunk,
+            "batched_attempts_per_s": attempts / batched_s, "batched_s": batched_s,
+            "plain_sequential_attempts_per_s": attempts / plain_s, "plain_sequential_s": plain_s,
+            "speedup": plain_s / batched_s, "replicas_identical_to_plain": same,
+            "device_bytes": device_bytes}
+
+
+def serving_leg(jobs: int, chunk: int, budget_gb: float):
+    """Served stream: the first `jobs` Poisson arrivals at 200 jobs/s (80 % 2D side
+    log-uniform [128, 2048], 20 % 3D side log-uniform [32, 128], 100-2000 MCS), admitted under
+    an HBM budget and advanced by continuous batching; requests are built before the clock
+    starts, admission (compile, allocation, upload) and result download count."""
+    from corepotts import JobServer, latency_summary, make_request, workloads
+    log(f"serving: building {jobs} requests")
+    t0 = time.perf_counter()
+    stream = workloads.serving_jobs(jobs)
+    reqs = [make_request(j.index, j.arrival_s, *workloads.serving_model(j), mcs=j.mcs) for j in stream]
+    log(f"serving: built in {time.perf_counter() - t0:.1f} s")
+    server = JobServer(int(budget_gb * 1e9), chunk_mcs=chunk)
+    t0 = time.perf_counter()
+    done = server.serve(reqs)
+    wall = time.perf_counter() - t0
+    summary = latency_summary(done)
+    summary.update({"wall_s": wall, "chunk_mcs": chunk, "budget_bytes": int(budget_gb * 1e9),
+                    "peak_jobs_resident": server.peak_jobs, "peak_bytes": server.peak_used,
+                    "arrival_rate_per_s": 200.0, "jobs_3d": sum(j.ndim == 3 for j in stream)})
+    server.batch.close()
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +149,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lottery", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-replicas", type=int, default=512)
+    ap.add_argument("--batch-mcs", type=int, default=500)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-chunk", type=int, default=20)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -148,6 +230,11 @@ def main():
                             "mcs_per_s": args.lottery_steps / dt, "rounds": lrep.internal_rounds,
                             "last_report": lrep._asdict()}
         del lot
+
+    if not args.no_batch:
+        extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
+    if not args.no_serve:
+        extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0)
 
     baseline = None
     if not args.no_cpu_baseline:

@@ -15,8 +15,10 @@ from .model import (AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, Car
                     required_properties, static_relation, tracker_conformance_errors,
                     validate_relation_domain, von_neumann_offsets)
 from .spec import ModelSpec, build_spec, state_arrays
-from .native import (EngineError, ScientificMCSReport, ScientificPottsIntegrator, copy_probe, ensemble_seed,
-                     init_scientific, load_library)
+from .native import (EngineError, ScientificMCSReport, ScientificPottsIntegrator, copy_probe,
+                     ensemble_seed, init_scientific, load_library)
+from .batch import ReplicaBatch, replica_bytes
+from .serve import Completion, JobServer, Request, latency_summary, make_request
 
 
 def step(integrator, n: int = 1):

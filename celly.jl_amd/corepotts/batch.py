@@ -1,0 +1,153 @@
+"""Many independent replicas on one GPU, advanced together (cpm_batch_* of the C-ABI).
+
+A batch plays the role of the reference's EnsembleProblem trajectories
+(lib/CorePotts/src/sciml/interface.jl:1059-1084): every replica is an independent
+ScientificPottsIntegrator with its own lattice size, temperature, contact matrix and seed,
+and its trajectory equals the one a single integrator produces from the same state and seed.
+On the device the replicas of one tile shape share each colour-pass launch, so a step of
+the batch costs one begin launch plus two launches per colour pass, whatever the replica
+count. Replicas may be added and removed between steps (continuous batching).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+
+from .model import CompiledScientificState
+from .native import (EngineError, ScientificMCSReport, load_library, make_desc, snapshot,
+                     state_buffers, state_pointers)
+from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ModelSpec, build_spec, state_arrays
+
+
+def replica_bytes(spec: ModelSpec) -> int:
+    """Device bytes one replica of this model holds (cpm_replica_bytes), for admission."""
+    lib = load_library()
+    keep: list = []
+    desc = make_desc(spec, keep)
+    out = C.c_uint64()
+    rc = lib.cpm_replica_bytes(C.byref(desc), C.byref(out))
+    if rc != 0:
+        raise EngineError(rc, "cpm_replica_bytes failed")
+    return int(out.value)
+
+
+class ReplicaBatch:
+    """Device-resident replicas of one algorithm (CheckerboardSweepCPM or LotteryCPM)."""
+
+    def __init__(self, algorithm: int = ALG_CHECKERBOARD, hbm_budget_bytes: int = 0,
+                 graph_chunk: int = 0):
+        if algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY):
+            raise NotImplementedError("the MI355X engine batches CheckerboardSweepCPM and LotteryCPM")
+        self.lib = load_library()
+        self.algorithm = algorithm
+        handle = C.c_void_p()
+        rc = self.lib.cpm_batch_create(int(algorithm), int(hbm_budget_bytes), C.byref(handle))
+        if rc != 0:
+            raise EngineError(rc, self.lib.cpm_batch_last_error(None).decode())
+        self._b = handle
+        self.specs: Dict[int, ModelSpec] = {}
+        self.compiled: Dict[int, CompiledScientificState] = {}
+        if graph_chunk:
+            self.set_graph_chunk(graph_chunk)
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_b", None):
+            self.lib.cpm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise EngineError(rc, self.lib.cpm_batch_last_error(self._b).decode())
+
+    # -- membership --------------------------------------------------------
+    def add(self, compiled: CompiledScientificState, spec: ModelSpec,
+            target_mcs: Optional[int] = None) -> int:
+        """Compile, allocate and upload one replica; returns its slot. Raises EngineError
+        with code CPM_E_NOMEM (2) when the batch's HBM budget would be exceeded."""
+        if spec.algorithm != self.algorithm:
+            raise ValueError("replica algorithm differs from the batch")
+        keep: list = []
+        desc = make_desc(spec, keep)
+        slot = C.c_int32()
+        self._check(self.lib.cpm_batch_add(self._b, C.byref(desc), C.byref(slot)))
+        s = int(slot.value)
+        try:
+            a = state_arrays(compiled, spec)
+            self._check(self.lib.cpm_batch_upload_state(self._b, s, *state_pointers(a)))
+            if target_mcs is not None:
+                self.set_target(s, target_mcs)
+        except Exception:
+            self.lib.cpm_batch_remove(self._b, s)
+            raise
+        self.specs[s] = spec
+        self.compiled[s] = compiled
+        return s
+
+    def add_model(self, state, proposal_relation, components, algorithm, seed: int = 0,
+                  target_mcs: Optional[int] = None) -> int:
+        """init_scientific's arguments (checkerboard.jl:197-219) as one more replica."""
+        spec = build_spec(state, proposal_relation, components, algorithm, seed)
+        return self.add(state, spec, target_mcs)
+
+    def remove(self, slot: int):
+        self._check(self.lib.cpm_batch_remove(self._b, int(slot)))
+        self.specs.pop(slot, None)
+        self.compiled.pop(slot, None)
+
+    @property
+    def slots(self):
+        return sorted(self.specs)
+
+    # -- stepping ----------------------------------------------------------
+    def set_target(self, slot: int, target_mcs: int):
+        """The replica advances (in later steps) until its MCS counter reaches target_mcs."""
+        self._check(self.lib.cpm_batch_set_target(self._b, int(slot), int(target_mcs)))
+
+    def step(self, n: int = 1):
+        """Enqueue n MCS; each replica below its target performs min(n, remaining) of them."""
+        if n < 0:
+            raise ValueError("step count must be non-negative")
+        self._check(self.lib.cpm_batch_step(self._b, int(n)))
+
+    def set_graph_chunk(self, chunk: int):
+        self._check(self.lib.cpm_batch_set_graph_chunk(self._b, int(chunk)))
+
+    def synchronize(self):
+        self._check(self.lib.cpm_batch_sync(self._b))
+
+    def progress(self) -> Dict[int, int]:
+        """Current MCS of every replica (synchronizes)."""
+        cap = max(int(self.lib.cpm_batch_capacity(self._b)), 1)
+        out = np.zeros(cap, dtype=np.uint64)
+        self._check(self.lib.cpm_batch_progress(self._b, out.ctypes.data, cap))
+        return {s: int(out[s]) for s in self.specs}
+
+    # -- observation -------------------------------------------------------
+    def download(self, slot: int) -> dict:
+        spec = self.specs[slot]
+        out = state_buffers(spec)
+        self._check(self.lib.cpm_batch_download_state(self._b, int(slot), *state_pointers(out)))
+        out["medium_volumes"] = out["medium_volumes"][:len(spec.medium_ids)]
+        return out
+
+    def logical_state(self, slot: int) -> CompiledScientificState:
+        return snapshot(self.compiled[slot], self.download(slot))
+
+    def report(self, slot: int) -> ScientificMCSReport:
+        out = (C.c_uint64 * 12)()
+        self._check(self.lib.cpm_batch_report(self._b, int(slot), out))
+        return ScientificMCSReport(*[int(v) for v in out])
+
+    def device_bytes(self) -> int:
+        out = C.c_uint64()
+        self._check(self.lib.cpm_batch_bytes(self._b, C.byref(out)))
+        return int(out.value)

@@ -493,6 +493,13 @@ def _realize_all(domain: CartesianDomain, offset: Sequence[int], owners: np.ndar
     Returns (valid mask, neighbour owner words) following cartesian.jl:445-525."""
     dims = domain.dims
     n = domain.n_sites
+    if all(isinstance(b.negative, PeriodicBoundary) and isinstance(b.positive, PeriodicBoundary)
+           for b in domain.boundaries):
+        # every axis wraps: the neighbour of x is x + offset mod dims, a cyclic shift
+        grid = owners.reshape(dims, order="F")
+        shifted = np.roll(grid, shift=[-int(offset[a]) for a in range(len(dims))],
+                          axis=tuple(range(len(dims))))
+        return np.ones(n, dtype=bool), shifted.reshape(-1, order="F")
     idx = np.arange(n, dtype=np.int64)
     coords = []
     rem = idx

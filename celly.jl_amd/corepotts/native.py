@@ -22,6 +22,9 @@ MAX_TYPES = 16
 MAX_MEDIA = 16
 ABI_VERSION = 1
 
+# status codes of the C-ABI (cpm_mi355x.h)
+CPM_OK, CPM_E_INVALID, CPM_E_UNSUPPORTED, CPM_E_DEVICE, CPM_E_NOMEM, CPM_E_STATE = range(6)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcpm_mi355x.so")
 
@@ -190,6 +193,45 @@ def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data
 
 
+STATE_FIELDS = ("tags", "ids", "finite_volumes", "medium_volumes", "boundary_measures", "active",
+                "cell_types", "target_volume", "volume_strength", "target_boundary",
+                "boundary_strength")
+
+
+def state_pointers(a: dict) -> list:
+    """Pointers of the eleven state arrays in cpm_upload_state / cpm_download_state order."""
+    return [_ptr(a.get(k)) for k in STATE_FIELDS]
+
+
+def state_buffers(spec: ModelSpec) -> dict:
+    """Host buffers sized for a download of one replica's state."""
+    n = int(np.prod(spec.dims))
+    cap = spec.capacity
+    return dict(
+        tags=np.zeros(n, np.uint8), ids=np.zeros(n, np.uint32),
+        finite_volumes=np.zeros(cap, np.int32),
+        medium_volumes=np.zeros(max(len(spec.medium_ids), 1), np.int32),
+        boundary_measures=np.zeros(cap, np.int64), active=np.zeros(cap, np.uint8),
+        cell_types=np.zeros(cap, np.uint32), target_volume=np.zeros(cap, np.float64),
+        volume_strength=np.zeros(cap, np.float64), target_boundary=np.zeros(cap, np.int64),
+        boundary_strength=np.zeros(cap, np.float64))
+
+
+def snapshot(compiled: CompiledScientificState, s: dict) -> CompiledScientificState:
+    """A compiled host state holding the downloaded arrays `s`."""
+    snap = compiled.copy()
+    snap.tags, snap.ids = s["tags"], s["ids"]
+    snap.finite_volumes = s["finite_volumes"]
+    snap.medium_volumes = s["medium_volumes"]
+    snap.boundary_measures = s["boundary_measures"]
+    snap.active = s["active"]
+    snap.cell_types = s["cell_types"]
+    for name in ("target_volume", "volume_strength", "target_boundary", "boundary_strength"):
+        if name in snap.properties:
+            snap.properties[name] = s[name].astype(snap.properties[name].dtype)
+    return snap
+
+
 class ScientificMCSReport(NamedTuple):
     """Accounting of one completed MCS (sequential.jl:236-251)."""
     mcs: int
@@ -244,45 +286,17 @@ class ScientificPottsIntegrator:
     # -- state -------------------------------------------------------------
     def upload(self, compiled: CompiledScientificState):
         a = state_arrays(compiled, self.spec)
-        self._check(self.lib.cpm_upload_state(
-            self._ctx, _ptr(a["tags"]), _ptr(a["ids"]), _ptr(a["finite_volumes"]),
-            _ptr(a["medium_volumes"]), _ptr(a["boundary_measures"]), _ptr(a["active"]),
-            _ptr(a["cell_types"]), _ptr(a["target_volume"]), _ptr(a["volume_strength"]),
-            _ptr(a["target_boundary"]), _ptr(a["boundary_strength"])))
+        self._check(self.lib.cpm_upload_state(self._ctx, *state_pointers(a)))
 
     def download(self) -> dict:
-        n = int(np.prod(self.spec.dims))
-        cap = self.spec.capacity
-        out = dict(
-            tags=np.zeros(n, np.uint8), ids=np.zeros(n, np.uint32),
-            finite_volumes=np.zeros(cap, np.int32),
-            medium_volumes=np.zeros(max(len(self.spec.medium_ids), 1), np.int32),
-            boundary_measures=np.zeros(cap, np.int64), active=np.zeros(cap, np.uint8),
-            cell_types=np.zeros(cap, np.uint32), target_volume=np.zeros(cap, np.float64),
-            volume_strength=np.zeros(cap, np.float64), target_boundary=np.zeros(cap, np.int64),
-            boundary_strength=np.zeros(cap, np.float64))
-        self._check(self.lib.cpm_download_state(
-            self._ctx, _ptr(out["tags"]), _ptr(out["ids"]), _ptr(out["finite_volumes"]),
-            _ptr(out["medium_volumes"]), _ptr(out["boundary_measures"]), _ptr(out["active"]),
-            _ptr(out["cell_types"]), _ptr(out["target_volume"]), _ptr(out["volume_strength"]),
-            _ptr(out["target_boundary"]), _ptr(out["boundary_strength"])))
+        out = state_buffers(self.spec)
+        self._check(self.lib.cpm_download_state(self._ctx, *state_pointers(out)))
         out["medium_volumes"] = out["medium_volumes"][:len(self.spec.medium_ids)]
         return out
 
     def logical_state(self) -> CompiledScientificState:
         """Snapshot of the device state as a compiled host state (logical_state)."""
-        s = self.download()
-        snap = self.compiled.copy()
-        snap.tags, snap.ids = s["tags"], s["ids"]
-        snap.finite_volumes = s["finite_volumes"]
-        snap.medium_volumes = s["medium_volumes"]
-        snap.boundary_measures = s["boundary_measures"]
-        snap.active = s["active"]
-        snap.cell_types = s["cell_types"]
-        for name in ("target_volume", "volume_strength", "target_boundary", "boundary_strength"):
-            if name in snap.properties:
-                snap.properties[name] = s[name].astype(snap.properties[name].dtype)
-        return snap
+        return snapshot(self.compiled, self.download())
 
     # -- stepping ----------------------------------------------------------
     @property

@@ -1,0 +1,151 @@
+"""Served simulation jobs: admission by device-memory budget and continuous batching.
+
+A job is a compiled model plus the number of MCS it asks for. Jobs arrive over time; the
+server admits the head of the queue as soon as its replica fits the HBM budget
+(cpm_replica_bytes), keeps admitted lattices and trackers resident on the device, and advances
+every admitted job together in steps of at most `chunk_mcs` MCS (each replica stops at its
+own target on the device). Finished jobs are downloaded and removed between steps, which frees
+memory for the next admissions, so jobs that arrive while others run join the running batch.
+
+Each job's trajectory depends only on its own state, seed and MCS counter, so its result
+equals a solo run of the same model (ScientificPottsIntegrator or the reference's
+init_scientific + step!).
+"""
+from __future__ import annotations
+
+import collections
+import time
+from typing import Callable, Dict, List, NamedTuple, Optional
+
+import numpy as np
+
+from .batch import ReplicaBatch, replica_bytes
+from .native import CPM_E_NOMEM, EngineError
+from .spec import ALG_CHECKERBOARD, ModelSpec, build_spec
+
+
+class Request(NamedTuple):
+    key: object             # caller's job identifier
+    arrival_s: float        # arrival time relative to the start of serve()
+    compiled: object        # CompiledScientificState
+    spec: ModelSpec
+    mcs: int                # MCS requested
+
+
+class Completion(NamedTuple):
+    key: object
+    arrival_s: float
+    admitted_s: float
+    finished_s: float
+    mcs: int
+    attempts: int           # activated attempts over the job (mutable sites x MCS)
+    state: dict             # downloaded arrays (cpm_download_state order)
+    report: tuple           # ScientificMCSReport of the job's last MCS
+
+    @property
+    def latency_s(self) -> float:
+        return self.finished_s - self.arrival_s
+
+
+def make_request(key, arrival_s: float, state, proposal_relation, components, algorithm,
+                 seed: int, mcs: int) -> Request:
+    """A job in init_scientific's terms (checkerboard.jl:197-219) plus its MCS count."""
+    if mcs < 1:
+        raise ValueError("a job asks for at least one MCS")
+    spec = build_spec(state, proposal_relation, components, algorithm, seed)
+    return Request(key, float(arrival_s), state, spec, int(mcs))
+
+
+class JobServer:
+    """Continuous-batching server over one ReplicaBatch.
+
+    `batch` is normally a ReplicaBatch; anything with the same add / remove / set_target /
+    step / progress / download / report methods works (the scheduler is tested on the host
+    with a stand-in)."""
+
+    def __init__(self, hbm_budget_bytes: int, chunk_mcs: int = 50, algorithm: int = ALG_CHECKERBOARD,
+                 batch=None, bytes_of: Callable[[ModelSpec], int] = replica_bytes,
+                 graph_chunk: int = 0, clock: Callable[[], float] = time.perf_counter,
+                 sleep: Callable[[float], None] = time.sleep):
+        if chunk_mcs < 1:
+            raise ValueError("chunk_mcs must be positive")
+        self.budget = int(hbm_budget_bytes)
+        self.chunk = int(chunk_mcs)
+        self.batch = batch if batch is not None else ReplicaBatch(algorithm, self.budget, graph_chunk)
+        self.bytes_of = bytes_of
+        self.clock = clock
+        self.sleep = sleep
+        self.used = 0
+        self.peak_used = 0
+        self.peak_jobs = 0
+        self.steps = 0
+
+    def serve(self, requests: List[Request]) -> List[Completion]:
+        """Runs the stream to completion; returns completions in finishing order."""
+        size = {id(r): self.bytes_of(r.spec) for r in requests}
+        for r in requests:
+            if size[id(r)] > self.budget:
+                raise ValueError(f"job {r.key!r} cannot fit the HBM budget on its own")
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
+        queue: collections.deque = collections.deque()
+        running: Dict[int, tuple] = {}
+        done: List[Completion] = []
+        t0 = self.clock()
+        while pending or queue or running:
+            now = self.clock() - t0
+            while pending and pending[0].arrival_s <= now:
+                queue.append(pending.popleft())
+            # admission: the head of the queue joins as soon as it fits (first come, first served)
+            while queue:
+                r = queue[0]
+                need = size[id(r)]
+                if self.used + need > self.budget:
+                    break
+                try:
+                    slot = self.batch.add(r.compiled, r.spec, target_mcs=r.mcs)
+                except EngineError as e:
+                    if e.code == CPM_E_NOMEM and running:
+                        break  # the engine's own accounting says wait for a completion
+                    raise
+                queue.popleft()
+                self.used += need
+                running[slot] = (r, self.clock() - t0, need)
+            self.peak_used = max(self.peak_used, self.used)
+            self.peak_jobs = max(self.peak_jobs, len(running))
+            if not running:
+                if pending:
+                    wait = pending[0].arrival_s - (self.clock() - t0)
+                    if wait > 0:
+                        self.sleep(wait)
+                continue
+            self.batch.step(self.chunk)
+            self.steps += 1
+            progress = self.batch.progress()
+            for slot in sorted(running):
+                r, admitted, need = running[slot]
+                if progress[slot] >= r.mcs:
+                    state = self.batch.download(slot)
+                    report = tuple(self.batch.report(slot))
+                    finished = self.clock() - t0
+                    self.batch.remove(slot)
+                    self.used -= need
+                    del running[slot]
+                    mutable = int(np.count_nonzero(r.spec.mutable_mask)) if r.spec.mutable_mask is not None \
+                        else int(np.prod(r.spec.dims))
+                    done.append(Completion(r.key, r.arrival_s, admitted, finished, r.mcs,
+                                           mutable * r.mcs, state, report))
+        return done
+
+
+def latency_summary(done: List[Completion], wall_s: Optional[float] = None) -> dict:
+    """p50 / p99 time-to-completion and aggregate attempts/s of a served stream."""
+    if not done:
+        return {"jobs": 0}
+    lat = np.array([c.latency_s for c in done])
+    first = min(c.arrival_s for c in done)
+    last = max(c.finished_s for c in done)
+    span = wall_s if wall_s is not None else last - first
+    attempts = sum(c.attempts for c in done)
+    return {"jobs": len(done), "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max()),
+            "attempts": attempts, "span_s": span, "attempts_per_s": attempts / span}

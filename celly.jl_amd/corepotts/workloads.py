@@ -11,6 +11,8 @@ for ``init_scientific``. Layout rules:
 """
 from __future__ import annotations
 
+from typing import NamedTuple
+
 import numpy as np
 
 from .model import (BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
@@ -138,3 +140,107 @@ def cubes_3d(L: int = 256, side: int = 8, *, temperature=np.float32(20.0), seed:
     compiled = compile_scientific_state(state, domain, BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
     alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
     return compiled, proposal, ScientificComponentSet(energies=energies), alg, seed
+
+
+def block_cells(L: int, ndim: int, side: int, pitch: int, *, ntypes: int = 2, J=J_TWO_TYPES,
+                temperature=np.float32(10.0), seed: int = 1234, lambda_v=np.float32(1.0),
+                algorithm="checkerboard"):
+    """L^ndim periodic lattice with side^ndim blocks on a pitch^ndim slot grid (every whole
+    slot filled, slot k in first-axis-fastest order is cell k+1 of type 2 + k mod ntypes),
+    medium id 1 (type 1) elsewhere; volume (lambda_v, V* = side^ndim) + Moore contact,
+    von Neumann proposal, no boundary constraint."""
+    per = L // pitch
+    idx = np.arange(L)
+    slot = idx // pitch
+    inside = (idx % pitch) < side
+    inside &= slot < per
+    grids = np.meshgrid(*([slot] * ndim), indexing="ij")
+    masks = np.meshgrid(*([inside] * ndim), indexing="ij")
+    cell = np.zeros([L] * ndim, dtype=np.int64)
+    stride = 1
+    for g in grids:
+        cell += g * stride
+        stride *= per
+    cell += 1
+    occupied = np.logical_and.reduce(masks)
+    owners = np.where(occupied, cell, -1).reshape(-1, order="F")
+    ncells = per ** ndim
+    types = {k: 2 + (k - 1) % ntypes for k in range(1, ncells + 1)}
+    dims = (L,) * ndim
+    domain = CartesianDomain(dims)
+    proposal = first_shell_relation(ProposalRole, ndim)
+    surface = first_shell_relation(SurfaceRole, ndim)
+    contact_h = UnorderedContactHamiltonian(np.asarray(J, np.float32), MediumTypeTable({1: 1}),
+                                            moore_relation(ContactRole, ndim))
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    energies = (volume, contact_h)
+    state = LogicalPottsState(owners, max(ncells, 1), cell_types=types, medium_domains=(1,),
+                              property_schema=_schema(*energies), dims=dims)
+    state.property_values("target_volume")[:ncells] = np.float32(side ** ndim)
+    state.property_values("volume_strength")[:ncells] = lambda_v
+    compiled = compile_scientific_state(state, domain, BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
+    alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
+    return compiled, proposal, ScientificComponentSet(energies=energies), alg, seed
+
+
+def integer_contact_matrix(rng: np.random.Generator, ntypes: int) -> np.ndarray:
+    """Symmetric (1 + ntypes)^2 Float32 J with integer entries uniform in {0, ..., 20}."""
+    upper = rng.integers(0, 21, size=(ntypes + 1, ntypes + 1)).astype(np.float32)
+    return (np.triu(upper) + np.triu(upper, 1).T).astype(np.float32)
+
+
+ENSEMBLE_LAYOUT_SEED = 20261204
+
+
+def ensemble_replica(r: int, master_seed: int = 1234, algorithm="checkerboard"):
+    """Replica r (0-based) of the batched ensemble: side L = round(exp(U[ln 64, ln 1024])),
+    10x10 cells on the pitch-10 slot grid, 2 types, T ~ U[2, 30], integer J ~ U{0..20}
+    symmetric, lambda_V = 1; seed = ensemble_seed(master_seed, r + 1)."""
+    from .native import ensemble_seed
+    rng = np.random.default_rng([ENSEMBLE_LAYOUT_SEED, r])
+    L = int(round(float(np.exp(rng.uniform(np.log(64.0), np.log(1024.0))))))
+    T = np.float32(rng.uniform(2.0, 30.0))
+    J = integer_contact_matrix(rng, 2)
+    return block_cells(L, 2, 10, 10, ntypes=2, J=J, temperature=T,
+                       seed=ensemble_seed(master_seed, r + 1), algorithm=algorithm)
+
+
+SERVING_SEED = 20261205
+
+
+class ServingJob(NamedTuple):
+    index: int
+    arrival_s: float       # seconds after the stream starts
+    ndim: int
+    side: int
+    ntypes: int
+    mcs: int
+    seed: int
+
+
+def serving_jobs(count: int, rate_per_s: float = 200.0, seed: int = SERVING_SEED,
+                 min_2d: int = 128, max_2d: int = 2048, min_3d: int = 32, max_3d: int = 128):
+    """The first `count` jobs of the served stream: Poisson arrivals at `rate_per_s`, 80 %
+    2D squares of side log-uniform [min_2d, max_2d], 20 % 3D cubes of side log-uniform
+    [min_3d, max_3d], 2-3 types, 100-2000 MCS each."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for k in range(count):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        three_d = rng.uniform() < 0.2
+        lo, hi = (min_3d, max_3d) if three_d else (min_2d, max_2d)
+        side = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        jobs.append(ServingJob(k, t, 3 if three_d else 2, side, int(rng.integers(2, 4)),
+                               int(rng.integers(100, 2001)), int(rng.integers(0, 2 ** 63))))
+    return jobs
+
+
+def serving_model(job: "ServingJob", algorithm="checkerboard"):
+    """Model of one served job: cells of 10^2 (2D) or 8^3 (3D) on slot grids of the same
+    pitch, integer J ~ U{0..20} symmetric drawn from the job seed, lambda_V = 1, T = 10."""
+    rng = np.random.default_rng([SERVING_SEED, job.index])
+    J = integer_contact_matrix(rng, job.ntypes)
+    side = 10 if job.ndim == 2 else 8
+    return block_cells(job.side, job.ndim, side, side, ntypes=job.ntypes, J=J,
+                       temperature=np.float32(10.0), seed=job.seed, algorithm=algorithm)

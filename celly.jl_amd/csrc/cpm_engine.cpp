@@ -127,6 +127,7 @@ struct Replica {
     uint32_t *d_ctype = nullptr;
     double *d_tv = nullptr, *d_vs = nullptr, *d_bs = nullptr;
     uint64_t bytes = 0;
+    uint64_t estimate = 0;      // estimate_replica_bytes at admission
     uint64_t base_mcs = 0;      // MCS counter when the state was last set
     bool report_cached = false;  // cached_report valid while mcs == cached_mcs
     uint64_t cached_mcs = 0;
@@ -548,12 +549,31 @@ void allocate_replica(Replica &r) {
     v.medium_vol = r.d_mvol;
 }
 
+// Upper bound on the device bytes a replica adds to a batch: its own buffers plus its share
+// of the group arenas (claim entries per tile site, tile table, accounting slots for at most
+// (2 halo + 1)^ndim colour passes, two claim words per cell) and its descriptor slot.
 uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
-    uint64_t n = 1;
-    for (int a = 0; a < d.ndim; ++a) n *= (uint64_t)std::max(d.dims[a], 1);
+    const int ndim = d.ndim == 3 ? 3 : 2;
+    int dims[3] = {std::max(d.dims[0], 1), std::max(d.dims[1], 1), ndim == 3 ? std::max(d.dims[2], 1) : 1};
+    uint64_t n = (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2];
+    int halo = std::max(max_abs_offset(d.proposal, ndim), max_abs_offset(d.surface, ndim));
+    if (has_term(d, CPM_TERM_CONTACT)) halo = std::max(halo, max_abs_offset(d.contact, ndim));
+    halo = std::min(std::max(halo, 1), 2);
+    int cdim, chalo, t[3];
+    cpm_internal_tile_config(ndim == 2 ? (halo == 1 ? 0 : 1) : (halo == 1 ? 2 : 3), &cdim, &chalo, &t[0],
+                             &t[1], &t[2]);
+    uint64_t tiles = 1;
+    for (int a = 0; a < 3; ++a) tiles *= (uint64_t)((dims[a] + t[a] - 1) / t[a]);
+    const uint64_t tile_sites = (uint64_t)t[0] * t[1] * t[2];
+    uint64_t rounds = 1;
+    for (int a = 0; a < ndim; ++a) rounds *= (uint64_t)(2 * halo + 1);
     const uint64_t cap1 = (uint64_t)d.capacity + 1;
-    // lattice + optional mask/colour/degree + segment share + per-cell state + claims
-    return n * 4 + n * 2 + n * 16 + cap1 * (4 + 8 + 4 + 1 + 8 * 4) + cap1 * 16 + (1u << 16);
+    const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + n /* colour or degree table */ +
+                         cap1 * (4 + 8 + 4 + 1 + 8 * 4) + CPM_MAX_MEDIA * 4;
+    const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 + sizeof(cpm::TileInfo) +
+                                     rounds * cpm::kSlotFields * 4) +
+                            2 * cap1 * 8;
+    return own + arenas + sizeof(cpm::RepDev) + sizeof(cpm::RepDyn);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,8 +704,12 @@ struct Engine {
         try {
             r->desc = desc;
             compile_replica(*r);
-            if (budget && bytes() + estimate_replica_bytes(r->desc) > budget)
-                fail(CPM_E_NOMEM, "HBM budget exceeded");
+            // admission: resident replicas' estimates plus this one stay within the budget
+            r->estimate = estimate_replica_bytes(desc);
+            uint64_t resident = 0;
+            for (const Replica *q : reps)
+                if (q) resident += q->estimate;
+            if (budget && resident + r->estimate > budget) fail(CPM_E_NOMEM, "HBM budget exceeded");
             allocate_replica(*r);
         } catch (...) {
             delete r;
@@ -787,8 +811,9 @@ struct Engine {
         const size_t need_slots = need_tiles * (size_t)maxr * cpm::kSlotFields;
         const size_t need_seg = need_tiles * g.seg_cap;
         hip_check(hipStreamSynchronize(stream), "sync");
+        // grow to fit; shrink once the claim-entry arena (the large one) is less than half used
         if (need_tiles > g.cap_tiles || 2 * (size_t)g.total_claims > g.cap_claims || need_slots > g.cap_slots ||
-            need_seg > g.cap_seg) {
+            need_seg > g.cap_seg || 2 * need_seg < g.cap_seg) {
             g.release();
             g.d_tiles = dev_alloc<cpm::TileInfo>(need_tiles, g.bytes);
             g.d_seg_count = dev_alloc<uint32_t>(need_tiles, g.bytes);

@@ -1,0 +1,135 @@
+"""Batched replicas and served jobs versus solo runs: every replica of a batch (different
+sizes, dimensions, temperatures, contact matrices, seeds and MCS targets; replicas joining
+and leaving mid-run) must end bit-exactly where the oracle's solo run of the same model
+ends, with the same last-MCS report."""
+import numpy as np
+import pytest
+
+from corepotts import (JobServer, ReplicaBatch, ensemble_seed, init_scientific, make_request,
+                       latency_summary, replica_bytes, workloads)
+from corepotts.spec import ALG_CHECKERBOARD, ALG_LOTTERY, build_spec
+from helpers import oracle_for
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("tags", "ids", "finite_volumes", "medium_volumes", "boundary_measures", "active",
+          "cell_types")
+
+
+def _models(algorithm="checkerboard"):
+    """Small replicas of uneven shape: 2D squares of several sides (odd sides need more than
+    four colours), one with a boundary constraint, and 3D cubes."""
+    rng = np.random.default_rng(77)
+    out = []
+    for k, (L, ndim, side) in enumerate([(23, 2, 10), (40, 2, 10), (31, 2, 7), (16, 3, 4),
+                                         (64, 2, 10), (9, 3, 3)]):
+        J = workloads.integer_contact_matrix(rng, 2)
+        T = np.float32(rng.uniform(2.0, 30.0))
+        out.append(workloads.block_cells(L, ndim, side, side, ntypes=2, J=J, temperature=T,
+                                         seed=ensemble_seed(99, k + 1), algorithm=algorithm))
+    out.append(workloads.scaled_2d(50, algorithm=algorithm, seed=ensemble_seed(99, 50)))
+    return out
+
+
+def _solo(model, mcs):
+    compiled, proposal, components, alg, seed = model
+    sim, _ = oracle_for(compiled, proposal, components, alg, seed)
+    sim.step(mcs)
+    return sim.store(), sim.report()
+
+
+def _check(got: dict, report, want, label):
+    state, rep = want
+    for f in FIELDS:
+        np.testing.assert_array_equal(got[f], state[f], err_msg=f"{label}: {f}")
+    assert tuple(report) == rep, f"{label}: report"
+
+
+@pytest.mark.parametrize("algorithm", ["checkerboard", "lottery"])
+@pytest.mark.parametrize("graph_chunk", [0, 4])
+def test_batch_replicas_equal_solo_runs(gpu_lib, algorithm, graph_chunk):
+    models = _models(algorithm)
+    targets = [7, 12, 3, 12, 9, 1, 12]
+    alg_id = ALG_CHECKERBOARD if algorithm == "checkerboard" else ALG_LOTTERY
+    batch = ReplicaBatch(alg_id, graph_chunk=graph_chunk)
+    slots = []
+    for model, target in zip(models, targets):
+        compiled, proposal, components, alg, seed = model
+        slots.append(batch.add_model(compiled, proposal, components, alg, seed, target_mcs=target))
+    batch.step(5)
+    batch.step(8)  # replicas stop at their own targets
+    prog = batch.progress()
+    for s, t in zip(slots, targets):
+        assert prog[s] == t
+    for k, (s, model, t) in enumerate(zip(slots, models, targets)):
+        _check(batch.download(s), batch.report(s), _solo(model, t), f"replica {k}")
+    batch.close()
+
+
+def test_batch_membership_changes_mid_run(gpu_lib):
+    models = _models()
+    batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=2)
+    a = batch.add_model(*models[0], target_mcs=10)
+    b = batch.add_model(*models[1], target_mcs=4)
+    batch.step(4)
+    c = batch.add_model(*models[3], target_mcs=6)   # joins at a's MCS 4
+    batch.step(2)
+    _check(batch.download(b), batch.report(b), _solo(models[1], 4), "b")
+    batch.remove(b)
+    d = batch.add_model(*models[4], target_mcs=5)   # reuses b's slot
+    batch.step(6)
+    for slot, model, t in ((a, models[0], 10), (c, models[3], 6), (d, models[4], 5)):
+        _check(batch.download(slot), batch.report(slot), _solo(model, t), f"slot {slot}")
+    batch.close()
+
+
+def test_batch_budget_refuses_admission(gpu_lib):
+    # resident bytes + the new replica's estimate must stay within the budget
+    model = _models()[4]
+    need = replica_bytes(build_spec(*model))
+    batch = ReplicaBatch(ALG_CHECKERBOARD, hbm_budget_bytes=2 * need - 1)
+    batch.add_model(*model, target_mcs=1)
+    from corepotts.native import CPM_E_NOMEM, EngineError
+    with pytest.raises(EngineError) as err:
+        batch.add_model(*model, target_mcs=1)
+    assert err.value.code == CPM_E_NOMEM
+    assert len(batch.slots) == 1
+    batch.step(1)  # the refused add left the batch usable
+    assert batch.progress()[batch.slots[0]] == 1
+    batch.close()
+
+
+def test_batch_equals_single_integrator(gpu_lib):
+    # the plain single-lattice path and the batch agree on a mid-size replica
+    model = workloads.ensemble_replica(3)
+    gpu = init_scientific(*model)
+    gpu.step(20)
+    batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=10)
+    s = batch.add_model(*model, target_mcs=20)
+    batch.step(20)
+    g, b = gpu.download(), batch.download(s)
+    for f in FIELDS:
+        np.testing.assert_array_equal(g[f], b[f], err_msg=f)
+    assert tuple(gpu.current_mcs_report()) == tuple(batch.report(s))
+    batch.close()
+    gpu.close()
+
+
+def test_served_jobs_equal_solo_runs(gpu_lib):
+    jobs = workloads.serving_jobs(8, rate_per_s=400.0, min_2d=20, max_2d=70, min_3d=8, max_3d=16)
+    reqs, models = [], {}
+    for j in jobs:
+        model = workloads.serving_model(j)
+        models[j.index] = model
+        mcs = 3 + j.mcs % 17
+        reqs.append(make_request(j.index, j.arrival_s, *model, mcs=mcs))
+    sizes = sorted(replica_bytes(r.spec) for r in reqs)
+    # a budget of the three largest forces queueing and staggered admissions
+    server = JobServer(sum(sizes[-3:]) + (8 << 20), chunk_mcs=3)
+    done = server.serve(reqs)
+    assert sorted(c.key for c in done) == [j.index for j in jobs]
+    assert server.peak_jobs >= 2
+    for c in done:
+        _check(c.state, c.report, _solo(models[c.key], c.mcs), f"job {c.key}")
+    s = latency_summary(done)
+    assert s["jobs"] == len(jobs) and s["p99_s"] >= s["p50_s"] > 0
