@@ -95,6 +95,8 @@ def load() -> C.CDLL:
     lib.orc_report.argtypes = [P, P]
     lib.orc_last_order.restype = C.c_int32
     lib.orc_last_order.argtypes = [P, P]
+    lib.orc_last_activations.restype = C.c_int64
+    lib.orc_last_activations.argtypes = [P, P]
     lib.orc_load_state.argtypes = [P, C.POINTER(OrcState)]
     lib.orc_store_state.argtypes = [P, C.POINTER(OrcState)]
     lib.orc_tracker_errors.restype = C.c_int32
@@ -244,6 +246,14 @@ class OracleSim:
         n = self.lib.orc_last_order(self._sim, out.ctypes.data)
         return out[:n]
 
+    def last_activations(self) -> np.ndarray:
+        """Lottery: activated opportunities per mutable site in the last MCS."""
+        out = np.zeros(int(np.prod(self.spec.dims)), np.uint32)
+        n = self.lib.orc_last_activations(self._sim, out.ctypes.data)
+        if n < 0:
+            raise ValueError("activations exist for LotteryCPM only")
+        return out[:n]
+
     def tracker_errors(self) -> int:
         return int(self.lib.orc_tracker_errors(self._sim))
 
@@ -295,3 +305,22 @@ def philox(counter, key):
     out = np.zeros(4, np.uint32)
     load().orc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
     return tuple(int(v) for v in out)
+
+
+def claims(recipient, losing, gaining, priority, capacity: int):
+    """The reference's three-phase cell-claim protocol on one colour (checkerboard.jl:273-369)
+    over actionable attempts between cells; returns (selected, dispositions, max_priority,
+    min_identity), the last two indexed by cell id."""
+    lib = load()
+    n = len(recipient)
+    r = np.ascontiguousarray(recipient, np.uint32)
+    l = np.ascontiguousarray(losing, np.uint32)
+    g = np.ascontiguousarray(gaining, np.uint32)
+    p = np.ascontiguousarray(priority, np.uint32)
+    sel = np.zeros(n, np.uint8)
+    disp = np.zeros(n, np.uint8)
+    maxp = np.zeros(capacity + 1, np.uint32)  # indexed by cell id (entry 0 unused)
+    mini = np.zeros(capacity + 1, np.uint32)
+    lib.orc_claims(n, r.ctypes.data, l.ctypes.data, g.ctypes.data, p.ctypes.data, capacity,
+                   sel.ctypes.data, disp.ctypes.data, maxp.ctypes.data, mini.ctypes.data)
+    return sel, disp, maxp, mini

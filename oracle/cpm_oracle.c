@@ -233,6 +233,8 @@ struct orc_sim {
     int64_t *att_dl, *att_dg;
     uint32_t *cell_max_priority, *cell_min_identity;
     uint32_t (*tickets)[4];
+    uint32_t *activations;   /* lottery: activated opportunities of each mutable position in the
+                                last MCS (lottery.jl accounting[1:N]) */
 };
 
 enum { OUT_ACTIONABLE = 1, OUT_SAME_OWNER = 2, OUT_BOUNDARY_NULL = 3, OUT_IMMUTABLE = 4 };
@@ -432,6 +434,7 @@ orc_sim *orc_create(const orc_model *model, int32_t algorithm, uint64_t seed) {
         s->color_count = maxdeg + 1;
         if (s->color_count > 255) { orc_destroy(s); return NULL; }
         s->tickets = (uint32_t (*)[4])xcalloc((size_t)nm, 16);
+        s->activations = (uint32_t *)xcalloc((size_t)nm, sizeof(uint32_t));
     } else {
         s->color_count = 1;
     }
@@ -455,6 +458,7 @@ void orc_destroy(orc_sim *s) {
     free(s->att_dg); free(s->cell_max_priority); free(s->cell_min_identity); free(s->order);
     free(s->color_sites); free(s->color_offsets); free(s->csr_offsets); free(s->csr_nbrs);
     free(s->tickets);
+    free(s->activations);
     free(s);
 }
 
@@ -467,6 +471,12 @@ int32_t orc_round_count(const orc_sim *s) { return s->color_count; }
 int32_t orc_last_order(const orc_sim *s, uint32_t *out) {
     for (int i = 0; i < s->color_count; ++i) out[i] = s->order[i];
     return s->color_count;
+}
+
+int64_t orc_last_activations(const orc_sim *s, uint32_t *out) {
+    if (!s->activations) return -1;
+    memcpy(out, s->activations, (size_t)s->n_mutable * sizeof(uint32_t));
+    return s->n_mutable;
 }
 
 static void copy_state(const orc_model *m, int64_t n, const orc_state *src, orc_state *dst) {
@@ -918,6 +928,7 @@ static void lottery_mcs(orc_sim *s) {
     fisher_yates(s->seed, S_LOTTERY_ACTIVATION, mcs, 1, R, s->order);
     uint64_t acc[9] = {0};
     uint8_t *use = (uint8_t *)xcalloc((size_t)nm, 1);
+    memset(s->activations, 0, (size_t)nm * sizeof(uint32_t));
     for (int32_t ordinal = 0; ordinal < R; ++ordinal) {
         uint32_t round = s->order[ordinal];
         uint32_t sub = round - 1;
@@ -944,6 +955,7 @@ static void lottery_mcs(orc_sim *s) {
             use[i] = 0;
             if (eligible && wins) {
                 acc[0]++;
+                s->activations[i]++;
                 uint32_t dir = orc_bounded_uint(s->seed, S_DIRECTION, mcs, sub, 0, K_SITE, site, 0,
                                                 (uint32_t)K) + 1;
                 uint32_t donor;

@@ -139,6 +139,10 @@ int32_t orc_round_count(const orc_sim *s); /* colours (checkerboard) or rounds (
 int32_t orc_step(orc_sim *s, uint64_t n);
 int32_t orc_report(const orc_sim *s, uint64_t out[12]);
 int32_t orc_last_order(const orc_sim *s, uint32_t *out); /* last colour/round order */
+/* Lottery: activated opportunities of every mutable site (ascending site order) in the last
+ * MCS, the reference's workspace.accounting[1:N] (lottery.jl:240-262); returns the count
+ * of mutable sites, or -1 for other algorithms. */
+int64_t orc_last_activations(const orc_sim *s, uint32_t *out);
 /* Copy the logical state in / out (arrays as in orc_state, sized by the model). */
 int32_t orc_load_state(orc_sim *s, const orc_state *src);
 int32_t orc_store_state(const orc_sim *s, orc_state *dst);
