@@ -190,6 +190,16 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = attempts_per_mcs * args.steps / elapsed
 
+    # host <-> device transfers at the boundary (outside the timed region): one state upload
+    # and one download, to state the PCIe-inclusive rate of the timed run
+    t0 = time.perf_counter()
+    integ.upload(compiled)
+    integ.synchronize()
+    upload_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    integ.download()
+    download_s = time.perf_counter() - t0
+
     log(f"kernel timing over {args.timing_mcs} MCS")
     timing = integ.step_timed(args.timing_mcs)
     propose_avg_ms = timing["propose_ms"] / max(timing["propose_launches"], 1)
@@ -203,6 +213,9 @@ def main():
 
     extra = {
         "mcs_per_s": args.steps / elapsed,
+        "pcie": {"upload_ms": upload_s * 1e3, "download_ms": download_s * 1e3,
+                 "pcie_inclusive_attempts_per_s":
+                     attempts_per_mcs * args.steps / (elapsed + upload_s + download_s)},
         "copy_probe": {"bytes": copy_bytes, "avg_ms": copy_ms,
                        "GBps": 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9},
         "colour_passes_per_mcs": passes,
