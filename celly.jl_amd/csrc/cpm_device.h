@@ -13,6 +13,8 @@
 namespace cpm {
 
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kCommitTiles = 4;    // tile segments per claim-check workgroup / apply region
 constexpr int kMaxRounds = 256;    // colour / round order capacity per replica
 constexpr int kSlotFields = 8;     // per-tile accounting fields
 
@@ -69,6 +71,17 @@ struct RepDev {
     int32_t pat_aligned;         // every period divides the tile side: class offsets = residues
     uint8_t pattern[64];
     uint8_t pat_coord[64][3];    // residue coordinates of each pattern index
+    // aligned patterns whose residue classes have power-of-two extents inside a tile:
+    // colour c owns residues col_res[col_first[c] .. + col_count[c]), each class holds
+    // 2^cls_shift tile sites laid out 2^cls_sx along x by 2^cls_sy along y
+    int32_t pat_fast;
+    int32_t cls_shift, cls_sx, cls_sy;
+    uint8_t col_first[65], col_count[65];
+    uint8_t col_res[64];
+    int32_t prop_lds[8];         // first proposal offsets as LDS position deltas of the tile shape
+    int32_t halo;                // largest |offset| component over the relations
+    int32_t surf_lin[32];        // first surface / contact offsets as linear site deltas, valid
+    int32_t cont_lin[32];        // for sites at least `halo` away from every face
     Bound prop_bound;            // direction draw over the proposal relation
     Bound round_bound;           // lottery activation draw over the round count
     FastDiv div_x, div_xy;       // site -> coordinates
@@ -122,11 +135,17 @@ struct Entry {
 // Four packed int32 lanes: one 16-byte global load or store.
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 
-// Per-tile launch data: replica slot and tile origin.
-struct TileInfo {
+// Per-tile launch data: replica slot, tile origin and the replica's first claim word.
+struct __align__(16) TileInfo {
     uint32_t rep;
     int32_t x0, y0, z0;
+    uint32_t claim_base;
+    uint32_t pad[3];
 };
+
+// Stretch count written by a propose kernel for a tile whose replica does not run this
+// pass (inactive replica, or fewer colours / rounds than the group's maximum).
+constexpr uint32_t kIdleTile = 0xffffffffu;
 
 struct PassArgs {
     const RepDev *reps;
@@ -134,7 +153,8 @@ struct PassArgs {
     const TileInfo *tiles;      // group tile -> replica slot and origin
     unsigned long long *claims; // two buffers of total_claims each
     Entry *seg;                 // per-tile segments of seg_cap entries
-    uint32_t *seg_count;
+    uint32_t *seg_count;        // [tile][kWaves] stretch lengths (kIdleTile: not running)
+    uint32_t *win_count;        // winners packed per check workgroup
     uint32_t *slots;            // [pass][tile][kSlotFields]
     uint32_t ntiles;
     uint32_t seg_cap;
@@ -147,6 +167,7 @@ struct PassArgs {
 // global address space lets the compiler emit global_* (not flat_*) memory operations,
 // which do not wait on the LDS counter.
 #define CPM_GLOBAL __attribute__((address_space(1)))
+#define CPM_CONST __attribute__((address_space(4)))
 template <typename T>
 __device__ __forceinline__ CPM_GLOBAL T *gp(T *p) {
     return (CPM_GLOBAL T *)p;

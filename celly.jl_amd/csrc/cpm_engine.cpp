@@ -419,10 +419,12 @@ void compile_replica(Replica &r) {
         }
     }
 
-    // tile configuration by dimension and halo
-    r.config = d.ndim == 2 ? (r.halo == 1 ? 0 : 1) : (r.halo == 1 ? 2 : 3);
+    // tile shape by dimension and halo; bit 3 selects the kernels for proposal relations
+    // other than 2 * ndim directions (the fixed-count kernels keep the offsets in registers)
+    const int shape = d.ndim == 2 ? (r.halo == 1 ? 0 : 1) : (r.halo == 1 ? 2 : 3);
+    r.config = shape | (d.proposal.count == 2 * d.ndim ? 0 : 8);
     int cdim, chalo, tx, ty, tz;
-    cpm_internal_tile_config(r.config, &cdim, &chalo, &tx, &ty, &tz);
+    cpm_internal_tile_config(shape, &cdim, &chalo, &tx, &ty, &tz);
 
     cpm::RepDev &v = r.dev;
     std::memset(&v, 0, sizeof(v));
@@ -457,6 +459,46 @@ void compile_replica(Replica &r) {
         v.pat_coord[i][2] = (uint8_t)(i / (v.pat[0] * v.pat[1]));
     }
     v.pat_aligned = (tx % v.pat[0] == 0 && ty % v.pat[1] == 0 && tz % v.pat[2] == 0) ? 1 : 0;
+    v.pat_fast = 0;
+    if (d.algorithm == CPM_ALG_CHECKERBOARD && !r.per_site_color && v.pat_aligned) {
+        const int nx = tx / v.pat[0], ny = ty / v.pat[1], nz = tz / v.pat[2];
+        auto pow2 = [](int q) { return q > 0 && (q & (q - 1)) == 0; };
+        auto log2i = [](int q) {
+            int l = 0;
+            while ((1 << l) < q) ++l;
+            return l;
+        };
+        if (pow2(nx) && pow2(ny) && pow2(nz) && r.rounds <= 64) {
+            v.pat_fast = 1;
+            v.cls_sx = log2i(nx);
+            v.cls_sy = log2i(ny);
+            v.cls_shift = log2i(nx * ny * nz);
+            const int np = v.pat[0] * v.pat[1] * v.pat[2];
+            int at = 0;
+            for (int c = 1; c <= r.rounds; ++c) {
+                v.col_first[c] = (uint8_t)at;
+                for (int q = 0; q < np; ++q)
+                    if (v.pattern[q] == c) v.col_res[at++] = (uint8_t)q;
+                v.col_count[c] = (uint8_t)(at - v.col_first[c]);
+            }
+        }
+    }
+    v.halo = r.halo;
+    {
+        const int64_t X = r.desc.dims[0], XY = (int64_t)r.desc.dims[0] * r.desc.dims[1];
+        for (int k = 0; k < 32 && k < d.surface.count; ++k)
+            v.surf_lin[k] = (int32_t)(d.surface.offsets[k][0] + X * d.surface.offsets[k][1] + XY * d.surface.offsets[k][2]);
+        if (contact)
+            for (int k = 0; k < 32 && k < d.contact.count; ++k)
+                v.cont_lin[k] = (int32_t)(d.contact.offsets[k][0] + X * d.contact.offsets[k][1] + XY * d.contact.offsets[k][2]);
+    }
+    {
+        // LDS layout of the tile shape: rows of tx + 8 positions (4-position x pads), ty + 2 halo rows
+        const int lx = tx + 8, ly = ty + 2 * chalo;
+        for (int k = 0; k < 8 && k < d.proposal.count; ++k)
+            v.prop_lds[k] = d.proposal.offsets[k][0] +
+                            lx * (d.proposal.offsets[k][1] + ly * d.proposal.offsets[k][2]);
+    }
     v.prop_bound = make_bound((uint32_t)d.proposal.count);
     v.round_bound = make_bound((uint32_t)std::max(r.rounds, 1));
     v.div_x = make_div((uint32_t)v.dims[0]);
@@ -570,7 +612,7 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     const uint64_t cap1 = (uint64_t)d.capacity + 1;
     const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + n /* colour or degree table */ +
                          cap1 * (4 + 8 + 4 + 1 + 8 * 4) + CPM_MAX_MEDIA * 4;
-    const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 + sizeof(cpm::TileInfo) +
+    const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + 4 + sizeof(cpm::TileInfo) +
                                      rounds * cpm::kSlotFields * 4) +
                             2 * cap1 * 8;
     return own + arenas + sizeof(cpm::RepDev) + sizeof(cpm::RepDyn);
@@ -589,6 +631,7 @@ struct Group {
     unsigned long long *d_claims = nullptr;
     cpm::Entry *d_seg = nullptr;
     uint32_t *d_seg_count = nullptr;
+    uint32_t *d_win_count = nullptr;
     uint32_t *d_slots = nullptr;
     size_t cap_tiles = 0, cap_claims = 0, cap_slots = 0, cap_seg = 0;
     uint64_t bytes = 0;
@@ -598,9 +641,9 @@ struct Group {
 
     void release() {
         dev_free(d_tiles); dev_free(d_claims); dev_free(d_seg); dev_free(d_seg_count);
-        dev_free(d_slots);
+        dev_free(d_win_count); dev_free(d_slots);
         d_tiles = nullptr; d_claims = nullptr; d_seg = nullptr; d_seg_count = nullptr;
-        d_slots = nullptr;
+        d_win_count = nullptr; d_slots = nullptr;
         cap_tiles = cap_claims = cap_slots = cap_seg = 0;
         bytes = 0;
     }
@@ -800,7 +843,7 @@ struct Engine {
                 const uint32_t ty = (t / (uint32_t)v.ntiles[0]) % (uint32_t)v.ntiles[1];
                 const uint32_t tz = t / ((uint32_t)v.ntiles[0] * (uint32_t)v.ntiles[1]);
                 tile_info.push_back(cpm::TileInfo{(uint32_t)s, (int32_t)(tx * g.tx), (int32_t)(ty * g.ty),
-                                                  (int32_t)(tz * g.tz)});
+                                                  (int32_t)(tz * g.tz), v.claim_base, {0, 0, 0}});
             }
         }
         g.ntiles = tiles;
@@ -816,7 +859,8 @@ struct Engine {
             need_seg > g.cap_seg || 2 * need_seg < g.cap_seg) {
             g.release();
             g.d_tiles = dev_alloc<cpm::TileInfo>(need_tiles, g.bytes);
-            g.d_seg_count = dev_alloc<uint32_t>(need_tiles, g.bytes);
+            g.d_seg_count = dev_alloc<uint32_t>(need_tiles * cpm::kWaves, g.bytes);
+            g.d_win_count = dev_alloc<uint32_t>(need_tiles / cpm::kCommitTiles + 1, g.bytes);
             g.d_claims = dev_alloc<unsigned long long>(2 * (size_t)g.total_claims, g.bytes);
             g.d_slots = dev_alloc<uint32_t>(need_slots, g.bytes);
             g.d_seg = dev_alloc<cpm::Entry>(need_seg, g.bytes);
@@ -827,7 +871,7 @@ struct Engine {
         }
         hip_check(hipMemsetAsync(g.d_claims, 0, 2 * (size_t)g.total_claims * 8, stream), "memset claims");
         hip_check(hipMemsetAsync(g.d_slots, 0, need_slots * 4, stream), "memset slots");
-        hip_check(hipMemsetAsync(g.d_seg_count, 0, need_tiles * 4, stream), "memset seg");
+        hip_check(hipMemsetAsync(g.d_seg_count, 0, need_tiles * cpm::kWaves * 4, stream), "memset seg");
         if (tiles)
             hip_check(hipMemcpy(g.d_tiles, tile_info.data(), tiles * sizeof(cpm::TileInfo), hipMemcpyHostToDevice),
                       "tile map");
@@ -852,6 +896,7 @@ struct Engine {
         a.claims = g.d_claims;
         a.seg = g.d_seg;
         a.seg_count = g.d_seg_count;
+        a.win_count = g.d_win_count;
         a.slots = g.d_slots;
         a.ntiles = g.ntiles;
         a.seg_cap = g.seg_cap;
@@ -891,7 +936,7 @@ struct Engine {
                 } else {
                     hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
                 }
-                metrics[0] += 2;
+                metrics[0] += 3;
                 g.pass_counter++;
             }
         }

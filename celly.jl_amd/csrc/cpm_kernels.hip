@@ -59,7 +59,8 @@ struct Realized {
     uint8_t mut;    // mutable site (the only donors a proposal relation realizes)
 };
 
-__device__ inline Realized realize_position(const RepDev &d, int gx, int gy, int gz) {
+template <class D>
+__device__ inline Realized realize_position(const D &d, int gx, int gy, int gz) {
     bool closed = false, fixed = false, invalid = false;
     int32_t fixed_word = 0;
     auto axis = [&](int a, int &v) {
@@ -213,7 +214,8 @@ __device__ __forceinline__ void block_sum4(uint32_t v[4], uint32_t (*s_red)[kBlo
 
 // Cell type or declared medium type, clamped into 1..n_types (the host rejects models
 // and states that could reach an undeclared type).
-__device__ __forceinline__ uint32_t owner_type(const RepDev &d, int32_t w) {
+template <class D>
+__device__ __forceinline__ uint32_t owner_type(const D &d, int32_t w) {
     uint32_t t = 1;
     if (w > 0) {
         t = gp(d.ctype)[w];
@@ -224,7 +226,8 @@ __device__ __forceinline__ uint32_t owner_type(const RepDev &d, int32_t w) {
     return min(max(t, 1u), (uint32_t)d.n_types);
 }
 
-__device__ __forceinline__ int medium_slot(const RepDev &d, int32_t w) {
+template <class D>
+__device__ __forceinline__ int medium_slot(const D &d, int32_t w) {
     for (int i = 0; i < d.n_media; ++i)
         if (d.medium_word[i] == w) return i;
     return 0;
@@ -296,100 +299,48 @@ __device__ __forceinline__ void stage_claim(bool act, const Entry &ent, const Re
     }
 }
 
-__device__ __forceinline__ int ilog2_exact(int v) { return 31 - __clz(v); }
+// Appends the lanes with `flag` set to a wave-private LDS list. `count` is the wave's
+// running length (uniform across the wave), so no atomics are needed.
+__device__ __forceinline__ void wave_append(bool flag, uint16_t value, uint32_t &count, uint16_t *list) {
+    const uint64_t m = __ballot(flag);
+    if (flag) list[count + __popcll(m & __lanemask_lt())] = value;
+    count += (uint32_t)__popcll(m);
+}
 
-// Enumerates the tile sites of one colour. Pattern colourings visit only the residue
-// classes of that colour; per-site colourings scan the tile. fn(i, ix, iy, iz) is called
-// by every lane of the workgroup in lock-step (i < 0 for idle lanes).
-template <int DIM, int TX, int TY, int TZ, typename F>
-__device__ inline void for_colour_sites(const RepDev &d, const uint8_t *s_pattern, uint32_t colour,
-                                        int x0, int y0, int z0, F &&fn) {
-    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
-    if (d.site_color == nullptr) {
-        const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
-        const int np = px * py * pz;
-        // periods dividing the tile sides give power-of-two class extents: shifts, no division
-        const bool pow2 = (TX % px == 0) && (TY % py == 0) && (TZ % pz == 0) && ((TX / px) & (TX / px - 1)) == 0 &&
-                          ((TY / py) & (TY / py - 1)) == 0 && ((TZ / pz) & (TZ / pz - 1)) == 0;
-        for (int r = 0; r < np; ++r) {
-            if (s_pattern[r] != colour) continue;
-            const int rx = d.pat_coord[r][0], ry = d.pat_coord[r][1], rz = d.pat_coord[r][2];
-            int fx = rx, fy = ry, fz = rz;
-            if (!d.pat_aligned) {
-                fx = (rx - x0 % px + px) % px;
-                fy = (ry - y0 % py + py) % py;
-                fz = (rz - z0 % pz + pz) % pz;
-            }
-            const int nx = fx < TX ? (TX - fx + px - 1) / px : 0;
-            const int ny = fy < TY ? (TY - fy + py - 1) / py : 0;
-            const int nz = fz < TZ ? (TZ - fz + pz - 1) / pz : 0;
-            const int n = nx * ny * nz;
-            const int sx = pow2 ? ilog2_exact(max(nx, 1)) : 0, sy = pow2 ? ilog2_exact(max(ny, 1)) : 0;
-            for (int j0 = 0; j0 < n; j0 += kBlock) {
-                const int j = j0 + (int)threadIdx.x;
-                int i = -1, ix = 0, iy = 0, iz = 0;
-                if (j < n) {
-                    int kx, ky, kz;
-                    if (pow2) {
-                        kx = j & (nx - 1);
-                        ky = (j >> sx) & (ny - 1);
-                        kz = j >> (sx + sy);
-                    } else {
-                        kx = j % nx;
-                        ky = (j / nx) % ny;
-                        kz = j / (nx * ny);
-                    }
-                    ix = fx + px * kx;
-                    iy = fy + py * ky;
-                    iz = fz + pz * kz;
-                    if (x0 + ix < X && y0 + iy < Y && z0 + iz < Z) i = ix + TX * (iy + TY * iz);
-                }
-                fn(i, ix, iy, iz);
-            }
-        }
-    } else {
-        constexpr int TN = TX * TY * TZ;
-        for (int i0 = 0; i0 < TN; i0 += kBlock) {
-            int i = i0 + (int)threadIdx.x;
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            if (i < TN) {
-                const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
-                if (gx >= X || gy >= Y || gz >= Z ||
-                    gp(d.site_color)[(uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)] != colour)
-                    i = -1;
-            } else {
-                i = -1;
-            }
-            fn(i, ix, iy, iz);
-        }
+// Entry e of the concatenation of the kWaves wave-private lists (each of capacity Q) whose
+// lengths have exclusive prefix sums p1 <= p2 <= p3.
+template <int Q>
+__device__ __forceinline__ uint32_t list_slot(uint32_t e, uint32_t p1, uint32_t p2, uint32_t p3) {
+    const uint32_t sg = (uint32_t)(e >= p1) + (uint32_t)(e >= p2) + (uint32_t)(e >= p3);
+    const uint32_t base = sg == 0 ? 0u : sg == 1 ? p1 : sg == 2 ? p2 : p3;
+    return sg * (uint32_t)Q + (e - base);
+}
+
+// Four per-tile counters packed as two 16-bit pairs (a tile holds at most 2048 sites),
+// summed over the workgroup into s_sum (zeroed before the first barrier).
+__device__ __forceinline__ void block_add_packed(uint32_t lo, uint32_t hi, uint32_t *s_sum) {
+    lo = __ockl_wfred_add_u32(lo);
+    hi = __ockl_wfred_add_u32(hi);
+    if (__lane_id() == 0) {
+        atomicAdd(&s_sum[0], lo);
+        atomicAdd(&s_sum[1], hi);
     }
 }
 
-// Up to kRegDirs proposal offsets kept in (scalar) registers for the interior test.
-constexpr int kRegDirs = 8;
-
-template <int LX, int LY>
-__device__ __forceinline__ void reg_offsets(const RelDev &rel, int32_t (&po)[kRegDirs]) {
-#pragma unroll
-    for (int k = 0; k < kRegDirs; ++k)
-        po[k] = k < rel.n ? rel.off[k][0] + LX * (rel.off[k][1] + LY * rel.off[k][2]) : 0;
-}
-
 // True when every proposal neighbour of pos is a mutable site owned by `own`: any
-// direction draw would then yield a same-owner no-op. Branch-free, independent LDS reads.
-template <bool SIMPLE>
+// direction draw would then yield a same-owner no-op. KF > 0: the relation has KF
+// directions held in registers (po); KF == 0: K directions in LDS (s_ploff).
+template <int KF, bool SIMPLE>
 __device__ __forceinline__ bool interior_site(const int32_t *s_own, const uint8_t *s_mut, int pos,
-                                              int32_t own, const int32_t (&po)[kRegDirs], int K,
+                                              int32_t own, const int32_t *po, int K,
                                               const int32_t *s_ploff) {
     bool interior = true;
-    if (K <= kRegDirs) {
+    if constexpr (KF > 0) {
 #pragma unroll
-        for (int k = 0; k < kRegDirs; ++k) {
-            if (k < K) {
-                const int q = pos + po[k];
-                interior &= s_own[q] == own;
-                if (!SIMPLE) interior &= s_mut[q] != 0;
-            }
+        for (int k = 0; k < KF; ++k) {
+            const int q = pos + po[k];
+            interior &= s_own[q] == own;
+            if (!SIMPLE) interior &= s_mut[q] != 0;
         }
     } else {
         for (int k = 0; k < K; ++k) {
@@ -401,172 +352,239 @@ __device__ __forceinline__ bool interior_site(const int32_t *s_own, const uint8_
     return interior;
 }
 
+// Offset of proposal direction dir (0-based), from the LDS copy of the offsets (a
+// register select chain would be turned into an indexed scratch load).
+template <int KF>
+__device__ __forceinline__ int direction_offset(uint32_t dir, const int32_t *, const int32_t *s_ploff) {
+    return s_ploff[dir];
+}
+
 // ---------------------------------------------------------------------------
 // Checkerboard colour pass, propose + claim (checkerboard.jl:244-304)
 // ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ>
+template <int DIM, int H, int TX, int TY, int TZ, int KF>
 __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     using G = Geom<DIM, H, TX, TY, TZ>;
+    constexpr int Q = G::TN / kWaves;  // capacity of a wave-private list
+    static_assert(G::TN <= 4096, "packed 16-bit tile counters");
     __shared__ __align__(16) int32_t s_own[G::LN];
     __shared__ uint8_t s_mut[G::LN];
     __shared__ uint16_t s_work[G::TN];
     __shared__ uint16_t s_act[G::TN];
     __shared__ uint8_t s_dir[G::TN];
-    __shared__ int32_t s_ploff[CPM_MAX_DIRS];
-    __shared__ uint8_t s_pattern[64];
-    __shared__ uint32_t s_nwork, s_nact, s_nseg;
-    __shared__ uint32_t s_red[4][kBlock / 64];
+    __shared__ int32_t s_ploff[KF > 0 ? KF : CPM_MAX_DIRS];
+    __shared__ uint32_t s_cls[64];
+    __shared__ uint32_t s_len[2][kWaves];
+    __shared__ uint32_t s_sum[2];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
     const uint32_t r = ti.rep;
     const RepDev &d = a.reps[r];
     const RepDyn &y = a.dyn[r];
-    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
+    if (!y.active || a.pass >= (uint32_t)d.rounds) {
+        if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
+        return;
+    }
     const uint64_t mcs = y.mcs;
     const uint32_t colour = y.order[a.pass];
 
     const int x0 = ti.x0, y0 = ti.y0, z0 = ti.z0;
-    const int X = d.dims[0], Y = d.dims[1];
+    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
     const int K = d.prop.n;
     const bool simple = d.simple;
+    const bool fast = d.pat_fast && d.site_color == nullptr;
+    const int wave = (int)(threadIdx.x >> 6);
 
-    if (threadIdx.x == 0) {
-        s_nwork = 0;
-        s_nact = 0;
-        s_nseg = 0;
+    if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
+    if constexpr (KF == 0) {
+        for (int k = threadIdx.x; k < K; k += kBlock)
+            s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
+    } else {
+        if (threadIdx.x < KF) s_ploff[threadIdx.x] = d.prop_lds[threadIdx.x];
     }
-    for (int k = threadIdx.x; k < K; k += kBlock)
-        s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
-    if (threadIdx.x < 64) s_pattern[threadIdx.x] = d.pattern[threadIdx.x];
-    int32_t po[kRegDirs];
-    reg_offsets<G::LX, G::LY>(d.prop, po);
+    int32_t po[KF > 0 ? KF : 1];
+#pragma unroll
+    for (int k = 0; k < (KF > 0 ? KF : 1); ++k) po[k] = d.prop_lds[k];
+    int ccount = 0;
+    if (fast) {
+        // residue classes of this colour: packed residue coordinates
+        ccount = d.col_count[colour];
+        if ((int)threadIdx.x < ccount) {
+            const int rr = d.col_res[d.col_first[colour] + threadIdx.x];
+            s_cls[threadIdx.x] = (uint32_t)d.pat_coord[rr][0] | ((uint32_t)d.pat_coord[rr][1] << 8) |
+                                 ((uint32_t)d.pat_coord[rr][2] << 16);
+        }
+    } else if (threadIdx.x < 64) {
+        s_cls[threadIdx.x] = d.pattern[threadIdx.x];
+    }
     load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, nullptr);
     __syncthreads();
 
     // 1. colour sites; interior sites (every proposal neighbour a mutable site of the
     //    same owner) are same-owner no-ops whatever the direction draw
-    uint32_t cnt[4] = {0, 0, 0, 0};  // candidates, same owner, boundary null, realized
-    for_colour_sites<DIM, TX, TY, TZ>(d, s_pattern, colour, x0, y0, z0, [&](int i, int ix, int iy, int iz) {
+    uint32_t c01 = 0, c23 = 0;  // candidates | same owner << 16, boundary null | realized << 16
+    uint32_t nw = 0;
+    uint16_t *my_work = s_work + wave * Q;
+    auto visit = [&](int i, int ix, int iy, int iz) __attribute__((always_inline)) {
         bool need = false;
         if (i >= 0) {
             const int pos = G::pos(ix, iy, iz);
             if (simple || s_mut[pos]) {
-                ++cnt[0];
+                c01 += 1u;
                 const int32_t own = s_own[pos];
-                const bool interior = simple ? interior_site<true>(s_own, s_mut, pos, own, po, K, s_ploff)
-                                             : interior_site<false>(s_own, s_mut, pos, own, po, K, s_ploff);
-                if (interior) ++cnt[1];
+                const bool interior = simple ? interior_site<KF, true>(s_own, s_mut, pos, own, po, K, s_ploff)
+                                             : interior_site<KF, false>(s_own, s_mut, pos, own, po, K, s_ploff);
+                if (interior) c01 += 1u << 16;
                 else need = true;
             }
         }
-        ballot_push(need, (uint16_t)(i < 0 ? 0 : i), &s_nwork, s_work);
-    });
+        wave_append(need, (uint16_t)(i < 0 ? 0 : i), nw, my_work);
+    };
+    if (fast) {
+        // the colour's residue classes as one flat index space: class k = j >> shift
+        const int shift = d.cls_shift, sx = d.cls_sx, sy = d.cls_sy;
+        const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
+        const int total = ccount << shift;
+        for (int j0 = 0; j0 < total; j0 += kBlock) {
+            const int j = j0 + (int)threadIdx.x;
+            int i = -1, ix = 0, iy = 0, iz = 0;
+            if (j < total) {
+                const uint32_t c = s_cls[j >> shift];
+                const int w = j & ((1 << shift) - 1);
+                const int kx = w & ((1 << sx) - 1);
+                const int ky = (w >> sx) & ((1 << sy) - 1);
+                const int kz = w >> (sx + sy);
+                ix = (int)(c & 0xff) + __mul24(px, kx);
+                iy = (int)((c >> 8) & 0xff) + __mul24(py, ky);
+                iz = (int)(c >> 16) + __mul24(pz, kz);
+                if (x0 + ix < X && y0 + iy < Y && z0 + iz < Z) i = ix + TX * (iy + TY * iz);
+            }
+            visit(i, ix, iy, iz);
+        }
+    } else {
+        // every tile site, coloured by the per-site array or the periodic pattern
+        const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
+        for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
+            int i = i0 + (int)threadIdx.x;
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
+            if (gx >= X || gy >= Y || gz >= Z) {
+                i = -1;
+            } else {
+                const uint32_t c = d.site_color
+                    ? gp(d.site_color)[(uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)]
+                    : s_cls[(gx % px) + px * ((gy % py) + py * (gz % pz))];
+                if (c != colour) i = -1;
+            }
+            visit(i, ix, iy, iz);
+        }
+    }
+    if (__lane_id() == 0) s_len[0][wave] = nw;
     __syncthreads();
 
     // 2. direction draws of the boundary sites (ProposalDirectionStream), attempt outcome
-    const uint32_t nwork = s_nwork;
+    const uint32_t p1 = s_len[0][0], p2 = p1 + s_len[0][1], p3 = p2 + s_len[0][2];
+    const uint32_t nwork = p3 + s_len[0][3];
+    uint32_t na = 0;
     for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
         const uint32_t e = e0 + threadIdx.x;
         bool act = false;
         int i = 0;
+        uint32_t dir = 0;
         if (e < nwork) {
-            i = s_work[e];
+            i = s_work[list_slot<Q>(e, p1, p2, p3)];
             const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
             const int pos = G::pos(ix, iy, iz);
             const uint32_t site = 1u + (uint32_t)(x0 + ix) +
                                   (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
-            const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0,
-                                              kSiteEntity, site, 0, d.prop_bound);
-            int off = 0;
-            if (K <= kRegDirs) {
-#pragma unroll
-                for (int k = 0; k < kRegDirs; ++k) off = dir == (uint32_t)k ? po[k] : off;
-            } else {
-                off = s_ploff[dir];
-            }
-            const int q = pos + off;
+            dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity, site, 0,
+                               d.prop_bound);
+            const int q = pos + direction_offset<KF>(dir, po, s_ploff);
             if (!simple && !s_mut[q]) {
-                ++cnt[2];
+                c23 += 1u;
             } else if (s_own[q] == s_own[pos]) {
-                ++cnt[1];
+                c01 += 1u << 16;
             } else {
-                ++cnt[3];
+                c23 += 1u << 16;
                 act = true;
-                i |= (int)(dir << 16);
             }
         }
         const uint64_t m = __ballot(act);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(&s_nact, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (act) {
-                const uint32_t idx = base + __popcll(m & __lanemask_lt());
-                s_act[idx] = (uint16_t)(i & 0xffff);
-                s_dir[idx] = (uint8_t)(i >> 16);
-            }
+        if (act) {
+            const uint32_t at = wave * Q + na + __popcll(m & __lanemask_lt());
+            s_act[at] = (uint16_t)i;
+            s_dir[at] = (uint8_t)dir;
         }
+        na += (uint32_t)__popcll(m);
     }
+    block_add_packed(c01, c23, s_sum);
+    if (__lane_id() == 0) s_len[1][wave] = na;
     __syncthreads();
 
-    // 3. priority draws and claims of the actionable attempts (compacted)
-    const uint32_t nact = s_nact;
+    // 3. priority draws and claims of the actionable attempts; the tile segment keeps the
+    //    wave-private layout of the lists (stretch w = entries [w * Q, w * Q + length w))
+    const uint32_t q1 = s_len[1][0], q2 = q1 + s_len[1][1], q3 = q2 + s_len[1][2];
+    const uint32_t nact = q3 + s_len[1][3];
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    Entry *seg = a.seg + (size_t)tile * a.seg_cap;
-    for (uint32_t e0 = 0; e0 < nact; e0 += kBlock) {
-        const uint32_t e = e0 + threadIdx.x;
-        bool act = false;
-        Entry ent{0, 0, 0, 0};
-        if (e < nact) {
-            const int i = s_act[e];
-            const int dir = s_dir[e];
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int pos = G::pos(ix, iy, iz);
-            const uint32_t site = 1u + (uint32_t)(x0 + ix) +
-                                  (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
-            const Words w = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
-                                      kSiteEntity, site, 0, 0);
-            ent = Entry{site, w.w0, s_own[pos], s_own[pos + s_ploff[dir]]};
-            act = true;
-        }
-        stage_claim(act, ent, d, claims, &s_nseg, seg);
+    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
+    for (uint32_t e = threadIdx.x; e < nact; e += kBlock) {
+        const uint32_t at = list_slot<Q>(e, q1, q2, q3);
+        const int i = s_act[at];
+        const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+        const int pos = G::pos(ix, iy, iz);
+        const uint32_t site = 1u + (uint32_t)(x0 + ix) +
+                              (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
+        const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
+                                        kSiteEntity, site, 0, 0).w0;
+        const int32_t losing = s_own[pos];
+        const int32_t gaining = s_own[pos + direction_offset<KF>(s_dir[at], po, s_ploff)];
+        // one 64-bit atomic max per cell replaces the max-priority and min-identity claims
+        // (checkerboard.jl:281-335); media are never claimed
+        const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
+        if (losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)losing], key);
+        if (gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)gaining], key);
+        seg[at] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
     }
 
-    block_sum4(cnt, s_red);
+    if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = s_len[1][threadIdx.x];
     if (threadIdx.x == 0) {
         uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-        slot[0] = cnt[0];
-        slot[1] = cnt[1];
-        slot[2] = cnt[2];
-        slot[3] = cnt[3];
-        a.seg_count[tile] = s_nseg;
+        const uint32_t lo = s_sum[0], hi = s_sum[1];
+        slot[0] = lo & 0xffffu;
+        slot[1] = lo >> 16;
+        slot[2] = hi & 0xffffu;
+        slot[3] = hi >> 16;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Lottery round, propose + claim (lottery.jl:153-256)
 // ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ>
+template <int DIM, int H, int TX, int TY, int TZ, int KF>
 __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     using G = Geom<DIM, H, TX, TY, TZ>;
+    constexpr int Q = G::TN / kWaves;
+    static_assert(G::TN <= 4096, "packed 16-bit tile counters");
     __shared__ __align__(16) int32_t s_own[G::LN];
     __shared__ uint8_t s_mut[G::LN];
     __shared__ uint32_t s_sid[G::LN];
     __shared__ uint32_t s_tk[G::LN];
     __shared__ uint16_t s_work[G::TN];
-    __shared__ int32_t s_ploff[CPM_MAX_DIRS];
+    __shared__ int32_t s_ploff[KF > 0 ? KF : CPM_MAX_DIRS];
     __shared__ int32_t s_cloff[CPM_MAX_DIRS];
-    __shared__ uint32_t s_nwork, s_nseg;
-    __shared__ uint32_t s_red[4][kBlock / 64];
+    __shared__ uint32_t s_len[kWaves];
+    __shared__ uint32_t s_sum[2];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
     const uint32_t r = ti.rep;
     const RepDev &d = a.reps[r];
     const RepDyn &y = a.dyn[r];
-    if (!y.active || a.pass >= (uint32_t)d.rounds) return;
+    if (!y.active || a.pass >= (uint32_t)d.rounds) {
+        if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
+        return;
+    }
     const uint64_t mcs = y.mcs;
     const uint32_t sub = y.order[a.pass] - 1;
 
@@ -574,17 +592,20 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
     const int K = d.prop.n;
     const int KC = d.conf.n;
+    const int wave = (int)(threadIdx.x >> 6);
 
-    if (threadIdx.x == 0) {
-        s_nwork = 0;
-        s_nseg = 0;
+    if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
+    if constexpr (KF == 0) {
+        for (int k = threadIdx.x; k < K; k += kBlock)
+            s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
+    } else {
+        if (threadIdx.x < KF) s_ploff[threadIdx.x] = d.prop_lds[threadIdx.x];
     }
-    for (int k = threadIdx.x; k < K; k += kBlock)
-        s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
     for (int k = threadIdx.x; k < KC; k += kBlock)
         s_cloff[k] = d.conf.off[k][0] + G::LX * (d.conf.off[k][1] + G::LY * d.conf.off[k][2]);
-    int32_t po[kRegDirs];
-    reg_offsets<G::LX, G::LY>(d.prop, po);
+    int32_t po[KF > 0 ? KF : 1];
+#pragma unroll
+    for (int k = 0; k < (KF > 0 ? KF : 1); ++k) po[k] = d.prop_lds[k];
     load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, s_sid);
     __syncthreads();
     // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1)
@@ -597,7 +618,9 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     }
     __syncthreads();
 
-    uint32_t cnt[4] = {0, 0, 0, 0};  // activated, same owner, boundary null, realized
+    uint32_t c01 = 0, c23 = 0;  // activated | same owner << 16, boundary null | realized << 16
+    uint32_t nw = 0;
+    uint16_t *my_work = s_work + wave * Q;
     for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
         const int i = i0 + (int)threadIdx.x;
         bool need = false;
@@ -640,58 +663,69 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
                                                     sub, 0, kSiteEntity, site, 0, d.round_bound) < degree + 1;
                         }
                         if (eligible) {
-                            ++cnt[0];
+                            c01 += 1u;
                             const int32_t own = s_own[pos];
-                            if (interior_site<false>(s_own, s_mut, pos, own, po, K, s_ploff)) ++cnt[1];
+                            if (interior_site<KF, false>(s_own, s_mut, pos, own, po, K, s_ploff)) c01 += 1u << 16;
                             else need = true;
                         }
                     }
                 }
             }
         }
-        ballot_push(need, (uint16_t)(i < G::TN ? i : 0), &s_nwork, s_work);
+        wave_append(need, (uint16_t)(i < G::TN ? i : 0), nw, my_work);
     }
+    if (__lane_id() == 0) s_len[wave] = nw;
     __syncthreads();
 
-    const uint32_t nwork = s_nwork;
+    const uint32_t p1 = s_len[0], p2 = p1 + s_len[1], p3 = p2 + s_len[2];
+    const uint32_t nwork = p3 + s_len[3];
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    Entry *seg = a.seg + (size_t)tile * a.seg_cap;
+    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
+    uint32_t na = 0;  // actionable attempts of this wave (entries e with the same position)
     for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
         const uint32_t e = e0 + threadIdx.x;
         bool act = false;
-        Entry ent{0, 0, 0, 0};
+        v4i v = v4i{0, 0, 0, 0};
         if (e < nwork) {
-            const int i = s_work[e];
+            const int i = s_work[list_slot<Q>(e, p1, p2, p3)];
             const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
             const int pos = G::pos(ix, iy, iz);
             const uint32_t site = s_sid[pos];
             const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
                                               kSiteEntity, site, 0, d.prop_bound);
-            const int q = pos + s_ploff[dir];
+            const int q = pos + direction_offset<KF>(dir, po, s_ploff);
             if (!s_mut[q]) {
-                ++cnt[2];
+                c23 += 1u;
             } else {
                 const int32_t own = s_own[pos], gain = s_own[q];
                 if (gain == own) {
-                    ++cnt[1];
+                    c01 += 1u << 16;
                 } else {
-                    ++cnt[3];
-                    ent = Entry{site, s_tk[pos], own, gain};
+                    c23 += 1u << 16;
+                    const uint32_t prio = s_tk[pos];
+                    const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
+                    if (own > 0) atomicMax(&claims[d.claim_base + (uint32_t)own], key);
+                    if (gain > 0) atomicMax(&claims[d.claim_base + (uint32_t)gain], key);
+                    v = v4i{(int32_t)site, (int32_t)prio, own, gain};
                     act = true;
                 }
             }
         }
-        stage_claim(act, ent, d, claims, &s_nseg, seg);
+        // wave-private stretches of the tile segment: wave w owns entries [w * Q, (w + 1) * Q)
+        const uint64_t m = __ballot(act);
+        if (act) seg[wave * Q + na + __popcll(m & __lanemask_lt())] = v;
+        na += (uint32_t)__popcll(m);
     }
-
-    block_sum4(cnt, s_red);
+    block_add_packed(c01, c23, s_sum);
+    if (__lane_id() == 0) a.seg_count[tile * kWaves + wave] = na;
+    __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-        slot[0] = cnt[0];
-        slot[1] = cnt[1];
-        slot[2] = cnt[2];
-        slot[3] = cnt[3];
-        a.seg_count[tile] = s_nseg;
+        const uint32_t lo = s_sum[0], hi = s_sum[1];
+        slot[0] = lo & 0xffffu;
+        slot[1] = lo >> 16;
+        slot[2] = hi & 0xffffu;
+        slot[3] = hi >> 16;
     }
 }
 
@@ -701,7 +735,8 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
 
 // Owner word seen by a contact/surface relation at site (x,y,z) + offset; 0 when the
 // neighbour is absent or invalid (cartesian.jl:494-525).
-__device__ __forceinline__ int32_t neighbour_word(const RepDev &d, int x, int y, int z, const int8_t *o) {
+template <class D, class O>
+__device__ __forceinline__ int32_t neighbour_word(const D &d, int x, int y, int z, const O *o) {
     if (d.simple) {
         const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
         int nx = x + o[0], ny = y + o[1], nz = z + o[2];
@@ -720,28 +755,65 @@ __device__ __forceinline__ int32_t neighbour_word(const RepDev &d, int x, int y,
 // the pass snapshot (checkerboard.jl:371-408; scientific_inner_loop.jl:213-269;
 // protocols/scientific.jl:372-385). Winners touch disjoint cells and non-adjacent sites,
 // so other winners' commits never change what this evaluation reads.
-__device__ inline bool evaluate_winner(const RepDev &d, const Entry &e, uint64_t mcs, uint32_t sub,
+//
+// Memory latency dominates this function, so every load that depends only on the entry
+// (both cells' trackers and parameters, up to kNb surface and kNb contact neighbour words)
+// is issued before any is used, then one batch of neighbour cell types; the arithmetic
+// keeps the reference's operation order.
+constexpr int kNb = 8;
+
+template <class D>
+__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub,
                                        int32_t *dl_out, int32_t *dg_out) {
     const int32_t l = e.losing, g = e.gaining;
     const uint32_t idx0 = e.site - 1;
-    const uint32_t zq = fast_div(idx0, d.div_xy);
+    const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
+    const uint32_t zq = fast_div(idx0, div_xy);
     const uint32_t rem = idx0 - zq * d.div_xy.d;
-    const uint32_t yq = fast_div(rem, d.div_x);
+    const uint32_t yq = fast_div(rem, div_x);
     const int x = (int)(rem - yq * d.div_x.d);
     const int y = (int)yq;
     const int z = (int)zq;
-    const int32_t vl = l > 0 ? gp(d.vol)[l] : 0;
-    const int32_t vg = g > 0 ? gp(d.vol)[g] : 0;
+    // slot 0 of every per-cell array is a spare, read in place of a medium owner
+    const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
+    const int32_t vl0 = gp(d.vol)[li], vg0 = gp(d.vol)[gi];
+    const double vs_l = gp(d.vs)[li], tv_l = gp(d.tv)[li], vs_g = gp(d.vs)[gi], tv_g = gp(d.tv)[gi];
+    const double bs_l = gp(d.bs)[li], bs_g = gp(d.bs)[gi];
+    const int64_t tb_l = gp(d.tb)[li], tb_g = gp(d.tb)[gi], bm_l = gp(d.bnd)[li], bm_g = gp(d.bnd)[gi];
+    const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
+    // sites at least `halo` from every face of a periodic, obstacle-free lattice reach
+    // their neighbours by a linear delta
+    const int h = d.halo;
+    const bool inner = d.simple && x >= h && x < d.dims[0] - h && y >= h && y < d.dims[1] - h &&
+                       (d.ndim == 2 || (z >= h && z < d.dims[2] - h));
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    auto nb = [&](const auto *o, int32_t lin) __attribute__((always_inline)) {
+        return inner ? lat[(int32_t)idx0 + lin] : neighbour_word(d, x, y, z, o);
+    };
+    int32_t sw[kNb], cw[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) sw[u] = u < ns ? nb(d.surf.off[u], d.surf_lin[u]) : 0;
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? nb(d.cont.off[u], d.cont_lin[u]) : 0;
+    uint32_t ct[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
+    const int32_t vl = l > 0 ? vl0 : 0, vg = g > 0 ? vg0 : 0;
+
     // boundary tracker deltas (BoundaryEdgeCount, scientific_components.jl:466-481)
     int32_t dl = 0, dg = 0;
-    for (int k = 0; k < d.surf.n; ++k) {
-        const int32_t w = neighbour_word(d, x, y, z, d.surf.off[k]);
-        if (w == 0) continue;
+    auto surface_word = [&](int32_t w) __attribute__((always_inline)) {
+        if (w == 0) return;
         dl += (w == l) ? 1 : -1;
         dg += (w == g) ? -1 : 1;
-    }
+    };
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) surface_word(sw[u]);
+    for (int k = kNb; k < ns; ++k)
+        surface_word(k < 32 ? nb(d.surf.off[k], d.surf_lin[k]) : neighbour_word(d, x, y, z, d.surf.off[k]));
     *dl_out = dl;
     *dg_out = dg;
+
     double acc = 0.0;
     bool acc64 = false;
     for (int t = 0; t < d.n_energies; ++t) {
@@ -752,52 +824,46 @@ __device__ inline bool evaluate_winner(const RepDev &d, const Entry &e, uint64_t
             v64 = d.volume_f64;
             if (v64) {
                 double delta = 0.0;
-                if (l > 0) delta = __dadd_rn(delta, vol_change_f64(gp(d.vs)[l], gp(d.tv)[l], vl, -1));
-                if (g > 0) delta = __dadd_rn(delta, vol_change_f64(gp(d.vs)[g], gp(d.tv)[g], vg, 1));
+                if (l > 0) delta = __dadd_rn(delta, vol_change_f64(vs_l, tv_l, vl, -1));
+                if (g > 0) delta = __dadd_rn(delta, vol_change_f64(vs_g, tv_g, vg, 1));
                 v = delta;
             } else {
                 float delta = 0.0f;
-                if (l > 0) delta = __fadd_rn(delta, vol_change_f32((float)gp(d.vs)[l], (float)gp(d.tv)[l], vl, -1));
-                if (g > 0) delta = __fadd_rn(delta, vol_change_f32((float)gp(d.vs)[g], (float)gp(d.tv)[g], vg, 1));
+                if (l > 0) delta = __fadd_rn(delta, vol_change_f32((float)vs_l, (float)tv_l, vl, -1));
+                if (g > 0) delta = __fadd_rn(delta, vol_change_f32((float)vs_g, (float)tv_g, vg, 1));
                 v = (double)delta;
             }
         } else if (term == CPM_TERM_CONTACT) {
-            // scientific_components.jl:289-308
+            // scientific_components.jl:289-308, directions in canonical order
             v64 = d.contact_f64;
             const uint32_t nt = (uint32_t)d.n_types;
             const uint32_t lt = owner_type(d, l), gt = owner_type(d, g);
             double dd = 0.0;
             float df = 0.0f;
-            const int nc = d.cont.n;
-            constexpr int B = 4;
-            for (int k0 = 0; k0 < nc; k0 += B) {
-                int32_t wb[B];
-                uint32_t tb[B];
-#pragma unroll
-                for (int u = 0; u < B; ++u) wb[u] = k0 + u < nc ? neighbour_word(d, x, y, z, d.cont.off[k0 + u]) : 0;
-#pragma unroll
-                for (int u = 0; u < B; ++u) tb[u] = gp(d.ctype)[wb[u] > 0 ? wb[u] : 0];  // slot 0 is a spare
-#pragma unroll
-                for (int u = 0; u < B; ++u) {
-                    const int32_t w = wb[u];
-                    if (w == 0) continue;
-                    const int k = k0 + u;
-                    uint32_t wt;
-                    if (w == l) wt = lt;
-                    else if (w == g) wt = gt;
-                    else if (w > 0) wt = min(max(tb[u], 1u), nt);
-                    else wt = owner_type(d, w);
-                    const uint32_t il = (lt - 1) * nt + (wt - 1), ig = (gt - 1) * nt + (wt - 1);
-                    if (v64) {
-                        const double wk = d.cont.wd[k];
-                        if (l != w) dd = __dsub_rn(dd, __dmul_rn(wk, d.Jd[il]));
-                        if (g != w) dd = __dadd_rn(dd, __dmul_rn(wk, d.Jd[ig]));
-                    } else {
-                        const float wk = d.cont.wf[k];
-                        if (l != w) df = __fsub_rn(df, __fmul_rn(wk, d.Jf[il]));
-                        if (g != w) df = __fadd_rn(df, __fmul_rn(wk, d.Jf[ig]));
-                    }
+            auto contact_term = [&](int k, int32_t w, uint32_t tw) __attribute__((always_inline)) {
+                if (w == 0) return;
+                uint32_t wt;
+                if (w == l) wt = lt;
+                else if (w == g) wt = gt;
+                else if (w > 0) wt = min(max(tw, 1u), nt);
+                else wt = owner_type(d, w);
+                const uint32_t il = (lt - 1) * nt + (wt - 1), ig = (gt - 1) * nt + (wt - 1);
+                if (v64) {
+                    const double wk = d.cont.wd[k];
+                    if (l != w) dd = __dsub_rn(dd, __dmul_rn(wk, d.Jd[il]));
+                    if (g != w) dd = __dadd_rn(dd, __dmul_rn(wk, d.Jd[ig]));
+                } else {
+                    const float wk = d.cont.wf[k];
+                    if (l != w) df = __fsub_rn(df, __fmul_rn(wk, d.Jf[il]));
+                    if (g != w) df = __fadd_rn(df, __fmul_rn(wk, d.Jf[ig]));
                 }
+            };
+#pragma unroll
+            for (int u = 0; u < kNb; ++u)
+                if (u < nc) contact_term(u, cw[u], ct[u]);
+            for (int k = kNb; k < nc; ++k) {
+                const int32_t w = k < 32 ? nb(d.cont.off[k], d.cont_lin[k]) : neighbour_word(d, x, y, z, d.cont.off[k]);
+                contact_term(k, w, gp(d.ctype)[w > 0 ? w : 0]);
             }
             v = v64 ? dd : (double)df;
         } else {
@@ -805,17 +871,13 @@ __device__ inline bool evaluate_winner(const RepDev &d, const Entry &e, uint64_t
             v64 = d.boundary_f64;
             if (v64) {
                 double r = 0.0;
-                if (l > 0)
-                    r = __dadd_rn(r, bnd_change_f64(gp(d.bs)[l], (double)gp(d.tb)[l], (double)gp(d.bnd)[l], vl, (double)dl, -1));
-                if (g > 0)
-                    r = __dadd_rn(r, bnd_change_f64(gp(d.bs)[g], (double)gp(d.tb)[g], (double)gp(d.bnd)[g], vg, (double)dg, 1));
+                if (l > 0) r = __dadd_rn(r, bnd_change_f64(bs_l, (double)tb_l, (double)bm_l, vl, (double)dl, -1));
+                if (g > 0) r = __dadd_rn(r, bnd_change_f64(bs_g, (double)tb_g, (double)bm_g, vg, (double)dg, 1));
                 v = r;
             } else {
                 float r = 0.0f;
-                if (l > 0)
-                    r = __fadd_rn(r, bnd_change_f32((float)gp(d.bs)[l], (float)gp(d.tb)[l], (float)gp(d.bnd)[l], vl, (float)dl, -1));
-                if (g > 0)
-                    r = __fadd_rn(r, bnd_change_f32((float)gp(d.bs)[g], (float)gp(d.tb)[g], (float)gp(d.bnd)[g], vg, (float)dg, 1));
+                if (l > 0) r = __fadd_rn(r, bnd_change_f32((float)bs_l, (float)tb_l, (float)bm_l, vl, (float)dl, -1));
+                if (g > 0) r = __fadd_rn(r, bnd_change_f32((float)bs_g, (float)tb_g, (float)bm_g, vg, (float)dg, 1));
                 v = (double)r;
             }
         }
@@ -843,7 +905,8 @@ __device__ inline bool evaluate_winner(const RepDev &d, const Entry &e, uint64_t
 
 // Commit of one accepted winning attempt: ownership, volumes, boundary measures,
 // retirement, medium volumes (checkerboard.jl:410-440).
-__device__ inline void commit_attempt(const RepDev &d, const Entry &e, int32_t dl, int32_t dg) {
+template <class D>
+__device__ inline void commit_attempt(const D &d, const Entry &e, int32_t dl, int32_t dg) {
     const int32_t l = e.losing, g = e.gaining;
     gp(d.lattice)[e.site - 1] = g;
     if (l > 0) {
@@ -869,13 +932,22 @@ __device__ inline void commit_attempt(const RepDev &d, const Entry &e, int32_t d
     }
 }
 
-constexpr int kCommitTiles = 8;     // tile segments per commit workgroup
 constexpr int kCommitChunk = 1024;  // staged attempts per claim-check round
 
-__global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
-    __shared__ uint32_t s_base[kCommitTiles + 1];
-    __shared__ uint32_t s_cnt[kCommitTiles][3];
-    __shared__ uint32_t s_win[kCommitChunk];
+// Claim check of one pass (checkerboard.jl:337-369). Workgroup b owns the segments of
+// tiles [8b, 8b + 8): an attempt wins iff its key is the maximum on both of its cells.
+// Winners are packed in place at the start of the workgroup's segment region as records
+// {site, tile within the region, losing, gaining} and counted in win_count[b]; every
+// packed position precedes every entry not yet read, so packing never overwrites input.
+__global__ __launch_bounds__(kBlock) void check_kernel(PassArgs a) {
+    // the workgroup's segments: kWaves stretches per tile, stretch w of a tile holding
+    // entries [w * Q, w * Q + count) with Q = seg_cap / kWaves
+    constexpr int NS = kCommitTiles * kWaves;
+    static_assert((NS & (NS - 1)) == 0 && NS <= 64, "stretch search halves a power of two");
+    __shared__ uint32_t s_base[NS + 1];
+    __shared__ uint32_t s_conf[kCommitTiles];
+    __shared__ uint32_t s_cbase[kCommitTiles];
+    __shared__ uint32_t s_live[kCommitTiles];
     __shared__ uint32_t s_nwin;
     const uint32_t nwg = gridDim.x;
     const uint32_t b = xcd_tile(blockIdx.x, nwg);
@@ -887,93 +959,133 @@ __global__ __launch_bounds__(kBlock) void commit_kernel(PassArgs a) {
         const uint32_t hi = min(lo + per, a.total_claims);
         for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) next[j] = 0ull;
     }
-    if (threadIdx.x < kCommitTiles) {
+    const uint32_t Q = a.seg_cap / kWaves;
+    if (threadIdx.x < 64) {
+        // wave 0: stretch counts (idle tiles count zero) and their inclusive scan
         const int k = threadIdx.x;
-        const uint32_t tile = b * kCommitTiles + k;
+        const uint32_t tile = b * kCommitTiles + k / kWaves;
         uint32_t n = 0;
-        if (tile < a.ntiles) {
-            const uint32_t r = a.tiles[tile].rep;
-            if (a.dyn[r].active && a.pass < (uint32_t)a.reps[r].rounds) n = a.seg_count[tile];
+        bool live = false;
+        if (k < NS && tile < a.ntiles) {
+            const uint32_t c = a.seg_count[tile * kWaves + k % kWaves];
+            live = c != kIdleTile;
+            n = live ? c : 0u;
         }
-        s_base[k + 1] = n;
-        s_cnt[k][0] = s_cnt[k][1] = s_cnt[k][2] = 0;
+        if (k < kCommitTiles && b * kCommitTiles + k < a.ntiles)
+            s_cbase[k] = a.tiles[b * kCommitTiles + k].claim_base;
+#pragma unroll
+        for (int o = 1; o < NS; o <<= 1) {
+            const uint32_t t = __shfl_up(n, o);
+            if (k >= o) n += t;
+        }
+        if (k < NS) s_base[k + 1] = n;
+        if (k == 0) {
+            s_base[0] = 0;
+            s_nwin = 0;
+        }
+        const uint64_t lm = __ballot(live && k % kWaves == 0);  // stretch 0 carries the flag
+        if (k < kCommitTiles) {
+            s_live[k] = (uint32_t)((lm >> (k * kWaves)) & 1u);
+            s_conf[k] = 0;
+        }
     }
-    if (threadIdx.x == 0) s_nwin = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        s_base[0] = 0;
-        for (int k = 1; k <= kCommitTiles; ++k) s_base[k] += s_base[k - 1];
-    }
-    __syncthreads();
-    const uint32_t total = s_base[kCommitTiles];
+    const uint32_t total = s_base[NS];
     const unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    for (uint32_t c0 = 0; c0 < total; c0 += kCommitChunk) {
-        const uint32_t c1 = min(c0 + kCommitChunk, total);
-        // A. claim check of the staged attempts; winners are compacted into LDS
-        for (uint32_t e0 = c0; e0 < c1; e0 += kBlock) {
-            const uint32_t e = e0 + threadIdx.x;
-            bool win = false;
-            uint32_t code = 0;
-            if (e < c1) {
-                int k = 0;
-                while (k + 1 < kCommitTiles && e >= s_base[k + 1]) ++k;
-                const uint32_t tile = b * kCommitTiles + k;
-                const uint32_t i = e - s_base[k];
-                const uint4 v = reinterpret_cast<const uint4 *>(a.seg)[(size_t)tile * a.seg_cap + i];
-                const RepDev &d = a.reps[a.tiles[tile].rep];
-                const unsigned long long key = ((unsigned long long)v.y << 32) | (uint32_t)(~v.x);
-                win = true;
-                if ((int32_t)v.z > 0) win = claims[d.claim_base + v.z] == key;
-                if ((int32_t)v.w > 0) win = win && claims[d.claim_base + v.w] == key;
-                if (!win) atomicAdd(&s_cnt[k][0], 1u);
-                code = ((uint32_t)k << 16) | i;
-            }
-            const uint64_t m = __ballot(win);
-            if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                uint32_t base = 0;
-                if ((int)__lane_id() == leader) base = atomicAdd(&s_nwin, (uint32_t)__popcll(m));
-                base = __shfl(base, leader);
-                if (win) s_win[base + __popcll(m & __lanemask_lt())] = code;
-            }
+    CPM_GLOBAL v4i *region = (CPM_GLOBAL v4i *)(a.seg + (size_t)b * kCommitTiles * a.seg_cap);
+    for (uint32_t e0 = 0; e0 < total; e0 += kBlock) {
+        const uint32_t e = e0 + threadIdx.x;
+        bool win = false;
+        v4i rec = v4i{0, 0, 0, 0};
+        if (e < total) {
+            // stretch holding entry e: largest k with s_base[k] <= e (binary search)
+            int k = 0;
+#pragma unroll
+            for (int step = NS / 2; step > 0; step >>= 1)
+                if (e >= s_base[k + step]) k += step;
+            const int kt = k / kWaves;
+            const uint32_t i = (uint32_t)kt * a.seg_cap + (uint32_t)(k % kWaves) * Q + (e - s_base[k]);
+            const v4i v = region[i];
+            const uint32_t cb = s_cbase[kt];
+            const unsigned long long key = ((unsigned long long)(uint32_t)v.y << 32) | (uint32_t)(~(uint32_t)v.x);
+            win = true;
+            if (v.z > 0) win = claims[cb + (uint32_t)v.z] == key;
+            if (v.w > 0) win = win && claims[cb + (uint32_t)v.w] == key;
+            if (!win) atomicAdd(&s_conf[kt], 1u);
+            rec = v4i{v.x, kt, v.z, v.w};
         }
+        // every lane has read its entry before any winner of this round is written
         __syncthreads();
-        // B. winners: energy change, acceptance and commit
-        const uint32_t nwin = s_nwin;
-        // winners are packed densely into as few waves as possible
-        for (uint32_t e = threadIdx.x; e < nwin; e += kBlock) {
-            const uint32_t code = s_win[e];
-            const int k = (int)(code >> 16);
-            const uint32_t tile = b * kCommitTiles + k;
-            const uint4 v = reinterpret_cast<const uint4 *>(a.seg)[(size_t)tile * a.seg_cap + (code & 0xffffu)];
-            const Entry ent{v.x, v.y, (int32_t)v.z, (int32_t)v.w};
-            const uint32_t r = a.tiles[tile].rep;
-            const RepDev &d = a.reps[r];
-            const RepDyn &y = a.dyn[r];
-            const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
-            int32_t dl, dg;
-            if (evaluate_winner(d, ent, y.mcs, sub, &dl, &dg)) {
-                atomicAdd(&s_cnt[k][2], 1u);
-                commit_attempt(d, ent, dl, dg);
-            } else {
-                atomicAdd(&s_cnt[k][1], 1u);
-            }
+        const uint64_t m = __ballot(win);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&s_nwin, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (win) region[base + __popcll(m & __lanemask_lt())] = rec;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) s_nwin = 0;
-        __syncthreads();
     }
+    __syncthreads();
+    if (threadIdx.x == 0) a.win_count[b] = s_nwin;
     if (threadIdx.x < kCommitTiles) {
         const int k = threadIdx.x;
         const uint32_t tile = b * kCommitTiles + k;
-        if (tile < a.ntiles) {
-            const uint32_t r = a.tiles[tile].rep;
-            if (a.dyn[r].active && a.pass < (uint32_t)a.reps[r].rounds) {
-                uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-                slot[4] = s_cnt[k][0];
-                slot[5] = s_cnt[k][1];
-                slot[6] = s_cnt[k][2];
+        if (tile < a.ntiles && s_live[k]) {
+            uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+            slot[4] = s_conf[k];
+            slot[5] = 0;
+            slot[6] = 0;
+        }
+    }
+}
+
+// Energy change, acceptance and commit of the winners of one segment region
+// (checkerboard.jl:371-457); one wave per region, so every region's latency chain runs
+// at once. The tiles of a replica are contiguous, so a region whose first and last tile
+// share a replica belongs to that replica alone (one round below, normally); otherwise one
+// round per tile. The replica index is uniform in every round, so its descriptor is read
+// through the scalar cache (constant address space: descriptors never change while a
+// kernel runs).
+__global__ __launch_bounds__(64) void apply_kernel(PassArgs a) {
+    __shared__ uint32_t s_cnt[kCommitTiles][2];
+    const uint32_t b = blockIdx.x;
+    const uint32_t nwin = a.win_count[b];
+    if (nwin == 0) return;
+    if (threadIdx.x < kCommitTiles) s_cnt[threadIdx.x][0] = s_cnt[threadIdx.x][1] = 0;
+    __syncthreads();
+    const uint32_t t0 = b * kCommitTiles;
+    const uint32_t nt = min((uint32_t)kCommitTiles, a.ntiles - t0);
+    const uint32_t r_first = a.tiles[t0].rep;
+    const bool one = a.tiles[t0 + nt - 1].rep == r_first;
+    const CPM_GLOBAL v4i *region = (const CPM_GLOBAL v4i *)(a.seg + (size_t)t0 * a.seg_cap);
+    for (uint32_t e0 = 0; e0 < nwin; e0 += 64) {
+        const uint32_t e = e0 + threadIdx.x;
+        v4i v = v4i{0, -1, 0, 0};
+        if (e < nwin) v = region[e];
+        const int kt = v.y;  // -1 for lanes without a winner
+        for (uint32_t k = 0; k < nt; ++k) {
+            const uint32_t r = one ? r_first : a.tiles[t0 + k].rep;
+            if (e < nwin && (one || kt == (int)k)) {
+                const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
+                const RepDyn &y = a.dyn[r];
+                const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
+                const Entry ent{(uint32_t)v.x, 0u, v.z, v.w};
+                int32_t dl, dg;
+                const bool accept = evaluate_winner(d, ent, y.mcs, sub, &dl, &dg);
+                if (accept) commit_attempt(d, ent, dl, dg);
+                atomicAdd(&s_cnt[kt][accept ? 1 : 0], 1u);
             }
+            if (one) break;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kCommitTiles) {
+        const int k = threadIdx.x;
+        const uint32_t c0 = s_cnt[k][0], c1 = s_cnt[k][1];
+        if (c0 + c1) {
+            uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + t0 + k) * kSlotFields;
+            slot[5] = c0;
+            slot[6] = c1;
         }
     }
 }
@@ -1034,27 +1146,44 @@ __global__ __launch_bounds__(kBlock) void report_kernel(const RepDev *reps, cons
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// generic == 0: the proposal relation has 2 * DIM directions (offsets in registers)
 template <int DIM, int H, int TX, int TY, int TZ>
-static hipError_t launch_propose_t(int algorithm, const PassArgs &a, hipStream_t stream) {
-    if (algorithm == CPM_ALG_LOTTERY)
-        hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ>), dim3(a.ntiles), dim3(kBlock), 0,
-                           stream, a);
-    else
-        hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ>), dim3(a.ntiles), dim3(kBlock), 0,
-                           stream, a);
+static hipError_t launch_propose_t(int algorithm, int generic, const PassArgs &a, hipStream_t stream) {
+    constexpr int KF = 2 * DIM;
+    if (algorithm == CPM_ALG_LOTTERY) {
+        if (generic)
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0>), dim3(a.ntiles), dim3(kBlock), 0,
+                               stream, a);
+        else
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF>), dim3(a.ntiles), dim3(kBlock), 0,
+                               stream, a);
+    } else {
+        if (generic)
+            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, 0>), dim3(a.ntiles), dim3(kBlock), 0,
+                               stream, a);
+        else
+            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, KF>), dim3(a.ntiles), dim3(kBlock), 0,
+                               stream, a);
+    }
     return hipGetLastError();
 }
 
 static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
     const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
-    hipLaunchKernelGGL(commit_kernel, dim3(wgs), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL(check_kernel, dim3(wgs), dim3(kBlock), 0, stream, a);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(apply_kernel, dim3(wgs), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 
 }  // namespace cpm
 
-// Tile configurations: id -> (dim, halo, TX, TY, TZ).
+// Tile configurations: id & 7 -> (dim, halo, TX, TY, TZ); bit 3 of id selects the kernels
+// for proposal relations whose direction count differs from 2 * dim.
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz) {
+    if (id < 0 || id > 15) return 0;
+    id &= 7;
     static const int table[][5] = {
         {2, 1, 128, 16, 1}, {2, 2, 128, 16, 1}, {3, 1, 32, 8, 8}, {3, 2, 32, 8, 8},
         {2, 1, 64, 32, 1},  {2, 1, 256, 8, 1},  {2, 1, 64, 16, 1}, {2, 1, 32, 32, 1},
@@ -1072,15 +1201,17 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream) {
     using namespace cpm;
-    switch (config) {
-    case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, *a, stream);
-    case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, *a, stream);
-    case 2: return launch_propose_t<3, 1, 32, 8, 8>(algorithm, *a, stream);
-    case 3: return launch_propose_t<3, 2, 32, 8, 8>(algorithm, *a, stream);
-    case 4: return launch_propose_t<2, 1, 64, 32, 1>(algorithm, *a, stream);
-    case 5: return launch_propose_t<2, 1, 256, 8, 1>(algorithm, *a, stream);
-    case 6: return launch_propose_t<2, 1, 64, 16, 1>(algorithm, *a, stream);
-    case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, *a, stream);
+    if (config < 0 || config > 15) return hipErrorInvalidValue;
+    const int g = config >> 3;
+    switch (config & 7) {
+    case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, g, *a, stream);
+    case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, g, *a, stream);
+    case 2: return launch_propose_t<3, 1, 32, 8, 8>(algorithm, g, *a, stream);
+    case 3: return launch_propose_t<3, 2, 32, 8, 8>(algorithm, g, *a, stream);
+    case 4: return launch_propose_t<2, 1, 64, 32, 1>(algorithm, g, *a, stream);
+    case 5: return launch_propose_t<2, 1, 256, 8, 1>(algorithm, g, *a, stream);
+    case 6: return launch_propose_t<2, 1, 64, 16, 1>(algorithm, g, *a, stream);
+    case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, g, *a, stream);
     default: return hipErrorInvalidValue;
     }
 }
