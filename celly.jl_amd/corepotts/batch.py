@@ -5,8 +5,8 @@ A batch plays the role of the reference's EnsembleProblem trajectories
 ScientificPottsIntegrator with its own lattice size, temperature, contact matrix and seed,
 and its trajectory equals the one a single integrator produces from the same state and seed.
 On the device the replicas of one tile shape share each colour-pass launch, so a step of
-the batch costs one begin launch plus two launches per colour pass, whatever the replica
-count. Replicas may be added and removed between steps (continuous batching).
+the batch costs one begin launch plus three launches (propose, claim check, apply) per
+colour pass, whatever the replica count. Replicas may be added and removed between steps (continuous batching).
 """
 from __future__ import annotations
 
@@ -69,28 +69,39 @@ class ReplicaBatch:
             raise EngineError(rc, self.lib.cpm_batch_last_error(self._b).decode())
 
     # -- membership --------------------------------------------------------
-    def add(self, compiled: CompiledScientificState, spec: ModelSpec,
-            target_mcs: Optional[int] = None) -> int:
-        """Compile, allocate and upload one replica; returns its slot. Raises EngineError
-        with code CPM_E_NOMEM (2) when the batch's HBM budget would be exceeded."""
+    def reserve(self, spec: ModelSpec) -> int:
+        """Compile the schedule of a replica and allocate its device state (cpm_batch_add);
+        returns its slot. No host synchronization: the host work overlaps queued GPU work.
+        Raises EngineError with code CPM_E_NOMEM when the HBM budget would be exceeded. The
+        replica stays idle until load() gives it a state and a target."""
         if spec.algorithm != self.algorithm:
             raise ValueError("replica algorithm differs from the batch")
         keep: list = []
         desc = make_desc(spec, keep)
         slot = C.c_int32()
         self._check(self.lib.cpm_batch_add(self._b, C.byref(desc), C.byref(slot)))
-        s = int(slot.value)
+        self.specs[int(slot.value)] = spec
+        return int(slot.value)
+
+    def load(self, slot: int, compiled: CompiledScientificState, target_mcs: Optional[int] = None):
+        """Upload the state of a reserved replica and set its target MCS."""
+        spec = self.specs[slot]
         try:
             a = state_arrays(compiled, spec)
-            self._check(self.lib.cpm_batch_upload_state(self._b, s, *state_pointers(a)))
+            self._check(self.lib.cpm_batch_upload_state(self._b, int(slot), *state_pointers(a)))
             if target_mcs is not None:
-                self.set_target(s, target_mcs)
+                self.set_target(slot, target_mcs)
         except Exception:
-            self.lib.cpm_batch_remove(self._b, s)
+            self.remove(slot)
             raise
-        self.specs[s] = spec
-        self.compiled[s] = compiled
-        return s
+        self.compiled[slot] = compiled
+
+    def add(self, compiled: CompiledScientificState, spec: ModelSpec,
+            target_mcs: Optional[int] = None) -> int:
+        """reserve() then load(): compile, allocate and upload one replica; returns its slot."""
+        slot = self.reserve(spec)
+        self.load(slot, compiled, target_mcs)
+        return slot
 
     def add_model(self, state, proposal_relation, components, algorithm, seed: int = 0,
                   target_mcs: Optional[int] = None) -> int:

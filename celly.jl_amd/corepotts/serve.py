@@ -81,17 +81,24 @@ class JobServer:
         self.steps = 0
 
     def serve(self, requests: List[Request]) -> List[Completion]:
-        """Runs the stream to completion; returns completions in finishing order."""
+        """Runs the stream to completion; returns completions in finishing order.
+
+        Each round: newly admissible jobs are reserved (schedule compile and allocation on the
+        host) while the previous step runs on the GPU; then the step is awaited, finished jobs
+        are downloaded and removed, the reserved jobs get their states, and the next step of
+        at most `chunk_mcs` MCS is enqueued for every resident job."""
         size = {id(r): self.bytes_of(r.spec) for r in requests}
         for r in requests:
             if size[id(r)] > self.budget:
                 raise ValueError(f"job {r.key!r} cannot fit the HBM budget on its own")
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
         queue: collections.deque = collections.deque()
+        reserved: List[tuple] = []
         running: Dict[int, tuple] = {}
         done: List[Completion] = []
+        inflight = False
         t0 = self.clock()
-        while pending or queue or running:
+        while pending or queue or running or reserved:
             now = self.clock() - t0
             while pending and pending[0].arrival_s <= now:
                 queue.append(pending.popleft())
@@ -102,38 +109,44 @@ class JobServer:
                 if self.used + need > self.budget:
                     break
                 try:
-                    slot = self.batch.add(r.compiled, r.spec, target_mcs=r.mcs)
+                    slot = self.batch.reserve(r.spec)
                 except EngineError as e:
-                    if e.code == CPM_E_NOMEM and running:
+                    if e.code == CPM_E_NOMEM and (running or reserved):
                         break  # the engine's own accounting says wait for a completion
                     raise
                 queue.popleft()
                 self.used += need
-                running[slot] = (r, self.clock() - t0, need)
+                reserved.append((slot, r, self.clock() - t0, need))
+            if inflight:
+                progress = self.batch.progress()
+                inflight = False
+                for slot in sorted(running):
+                    r, admitted, need = running[slot]
+                    if progress[slot] >= r.mcs:
+                        state = self.batch.download(slot)
+                        report = tuple(self.batch.report(slot))
+                        finished = self.clock() - t0
+                        self.batch.remove(slot)
+                        self.used -= need
+                        del running[slot]
+                        mutable = int(np.count_nonzero(r.spec.mutable_mask)) if r.spec.mutable_mask is not None \
+                            else int(np.prod(r.spec.dims))
+                        done.append(Completion(r.key, r.arrival_s, admitted, finished, r.mcs,
+                                               mutable * r.mcs, state, report))
+            for slot, r, admitted, need in reserved:
+                self.batch.load(slot, r.compiled, target_mcs=r.mcs)
+                running[slot] = (r, admitted, need)
+            reserved = []
             self.peak_used = max(self.peak_used, self.used)
             self.peak_jobs = max(self.peak_jobs, len(running))
-            if not running:
-                if pending:
-                    wait = pending[0].arrival_s - (self.clock() - t0)
-                    if wait > 0:
-                        self.sleep(wait)
-                continue
-            self.batch.step(self.chunk)
-            self.steps += 1
-            progress = self.batch.progress()
-            for slot in sorted(running):
-                r, admitted, need = running[slot]
-                if progress[slot] >= r.mcs:
-                    state = self.batch.download(slot)
-                    report = tuple(self.batch.report(slot))
-                    finished = self.clock() - t0
-                    self.batch.remove(slot)
-                    self.used -= need
-                    del running[slot]
-                    mutable = int(np.count_nonzero(r.spec.mutable_mask)) if r.spec.mutable_mask is not None \
-                        else int(np.prod(r.spec.dims))
-                    done.append(Completion(r.key, r.arrival_s, admitted, finished, r.mcs,
-                                           mutable * r.mcs, state, report))
+            if running:
+                self.batch.step(self.chunk)
+                self.steps += 1
+                inflight = True
+            elif pending and not queue:
+                wait = pending[0].arrival_s - (self.clock() - t0)
+                if wait > 0:
+                    self.sleep(wait)
         return done
 
 

@@ -104,7 +104,8 @@ struct RepDev {
     // device buffers
     int32_t *lattice;
     const uint8_t *mask;         // nullptr: all mutable
-    const uint8_t *site_color;   // nullptr: pattern colouring
+    const uint16_t *colour_list; // colourings without pat_fast: tile positions by tile and colour
+    const uint32_t *colour_off;  // [tile][rounds + 1] ranges into colour_list
     const uint8_t *site_degree;  // lottery, nullptr: uniform degree
     int32_t *vol;                // cap + 1, index = cell id
     int64_t *bnd;

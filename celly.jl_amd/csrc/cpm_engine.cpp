@@ -118,10 +118,16 @@ struct Replica {
     std::vector<uint8_t> degrees;   // lottery conflict degrees
     bool uniform_degree = true;
     bool per_site_color = false;
+    // checkerboard colourings without a fast periodic pattern: per tile, the tile positions
+    // of each colour (clist) and, per tile and colour, their range (coff, rounds + 1 per tile)
+    std::vector<uint16_t> clist;
+    std::vector<uint32_t> coff;
     cpm::RepDev dev{};
     // device buffers
     int32_t *d_lattice = nullptr;
     uint8_t *d_mask = nullptr, *d_color = nullptr, *d_degree = nullptr, *d_active = nullptr;
+    uint16_t *d_clist = nullptr;
+    uint32_t *d_coff = nullptr;
     int32_t *d_vol = nullptr, *d_mvol = nullptr;
     int64_t *d_bnd = nullptr, *d_tb = nullptr;
     uint32_t *d_ctype = nullptr;
@@ -136,6 +142,7 @@ struct Replica {
 
     ~Replica() {
         dev_free(d_lattice); dev_free(d_mask); dev_free(d_color); dev_free(d_degree);
+        dev_free(d_clist); dev_free(d_coff);
         dev_free(d_active); dev_free(d_vol); dev_free(d_mvol); dev_free(d_bnd); dev_free(d_tb);
         dev_free(d_ctype); dev_free(d_tv); dev_free(d_vs); dev_free(d_bs);
     }
@@ -349,27 +356,92 @@ void compile_replica(Replica &r) {
     std::vector<int64_t> nb(r.conf_off.size() / 3 + 1);
     int pattern[3] = {0, 0, 0};
     std::vector<uint8_t> pattern_table;
+    // Sites at least `halo` from every face of an obstacle-free lattice reach their conflict
+    // neighbours by linear deltas; distinct offsets within the halo are distinct sites there
+    // (dims >= 2 halo + 1), so no de-duplication is needed. Other sites use realize_site.
+    const int kconf = (int)r.conf_off.size() / 3;
+    std::vector<int64_t> lin((size_t)kconf);
+    for (int q = 0; q < kconf; ++q)
+        lin[q] = r.conf_off[3 * q] + (int64_t)g.dims[0] * (r.conf_off[3 * q + 1] + (int64_t)g.dims[1] * r.conf_off[3 * q + 2]);
+    const int hh = r.halo;
+    bool wide = !g.mask;
+    for (int a = 0; a < d.ndim; ++a) wide = wide && g.dims[a] >= 2 * hh + 1;
+    // visits every site in ascending order with its coordinates; fn(i, x, y, z, interior)
+    auto for_sites = [&](auto &&fn) {
+        int64_t i = 0;
+        for (int z = 0; z < g.dims[2]; ++z)
+            for (int y = 0; y < g.dims[1]; ++y) {
+                const bool iny = wide && y >= hh && y < g.dims[1] - hh &&
+                                 (d.ndim == 2 || (z >= hh && z < g.dims[2] - hh));
+                for (int x = 0; x < g.dims[0]; ++x, ++i)
+                    fn(i, x, y, z, iny && x >= hh && x < g.dims[0] - hh);
+            }
+    };
     if (d.algorithm == CPM_ALG_CHECKERBOARD) {
         // checkerboard.jl:57-90: greedy smallest free colour, ascending site order
         r.colors.assign((size_t)r.n, 0);
         int maxcolor = 0;
         std::vector<char> forbidden;
-        for (int64_t i = 0; i < r.n; ++i) {
-            if (g.mask && !g.mask[i]) continue;
-            const int c = conflict_neighbours(g, r.conf_off, i, nb.data());
-            forbidden.assign((size_t)c + 2, 0);
-            for (int q = 0; q < c; ++q) {
-                const int col = r.colors[nb[q]];
-                if (col > 0 && col <= c + 1) forbidden[col] = 1;
+        uint8_t *colors = r.colors.data();
+        for_sites([&](int64_t i, int, int, int, bool interior) {
+            if (g.mask && !g.mask[i]) return;
+            int col;
+            if (kconf + 1 <= 63) {
+                // colours never exceed degree + 1 <= 63: a bit mask of the taken colours
+                uint64_t taken = 1;  // colour 0 (uncoloured) is never chosen
+                if (interior) {
+                    for (int q = 0; q < kconf; ++q) taken |= 1ull << colors[i + lin[q]];
+                } else {
+                    const int c = conflict_neighbours(g, r.conf_off, i, nb.data());
+                    for (int q = 0; q < c; ++q) taken |= 1ull << colors[nb[q]];
+                }
+                col = __builtin_ctzll(~taken);
+            } else {
+                const int c = conflict_neighbours(g, r.conf_off, i, nb.data());
+                forbidden.assign((size_t)c + 2, 0);
+                for (int q = 0; q < c; ++q) {
+                    const int cq = colors[nb[q]];
+                    if (cq > 0 && cq <= c + 1) forbidden[cq] = 1;
+                }
+                col = 1;
+                while (forbidden[col]) ++col;
             }
-            int col = 1;
-            while (forbidden[col]) ++col;
             if (col > 255) fail(CPM_E_UNSUPPORTED, "realized colouring exceeds 255 colours");
-            r.colors[i] = (uint8_t)col;
+            colors[i] = (uint8_t)col;
             maxcolor = std::max(maxcolor, col);
-        }
+        });
         r.rounds = maxcolor;
-        // periodic pattern detection: colour == f(x mod px, y mod py, z mod pz)
+        // periodic pattern detection: colour == f(x mod px, y mod py, z mod pz). Each
+        // candidate is first probed on the sites within 2 of a face (where wrap seams break
+        // periodicity), then verified on every site.
+        auto matches = [&](int px, int py, int pz, std::vector<uint8_t> &table, bool probe) {
+            int rz = 0;
+            for (int z = 0; z < g.dims[2]; ++z, rz = (rz + 1 == pz ? 0 : rz + 1)) {
+                const bool ez = d.ndim == 3 && (z < 2 || z >= g.dims[2] - 2);
+                int ry = 0;
+                for (int y = 0; y < g.dims[1]; ++y, ry = (ry + 1 == py ? 0 : ry + 1)) {
+                    const bool ey = ez || y < 2 || y >= g.dims[1] - 2;
+                    const int64_t row = (int64_t)g.dims[0] * (y + (int64_t)g.dims[1] * z);
+                    const int base = px * (ry + py * rz);
+                    int rx = 0;
+                    for (int x = 0; x < g.dims[0]; ++x, rx = (rx + 1 == px ? 0 : rx + 1)) {
+                        if (probe && !ey && x >= 2 && x < g.dims[0] - 2) {
+                            // interior of the row: skip to its last two sites
+                            const int skip = g.dims[0] - 2 - x;
+                            x += skip - 1;
+                            rx = (rx + skip - 1) % px;
+                            continue;
+                        }
+                        const int64_t i = row + x;
+                        if (g.mask && !g.mask[i]) continue;
+                        uint8_t &t = table[(size_t)(base + rx)];
+                        if (t == 0) t = colors[i];
+                        else if (t != colors[i]) return false;
+                    }
+                }
+            }
+            return true;
+        };
         bool found = false;
         for (int prod = 1; prod <= 64 && !found; ++prod) {
             for (int px = 1; px <= 8 && !found; ++px)
@@ -377,17 +449,9 @@ void compile_replica(Replica &r) {
                     for (int pz = 1; pz <= (d.ndim == 3 ? 8 : 1) && !found; ++pz) {
                         if (px * py * pz != prod) continue;
                         std::vector<uint8_t> table((size_t)prod, 0);
-                        bool ok = true;
-                        for (int64_t i = 0; i < r.n && ok; ++i) {
-                            if (g.mask && !g.mask[i]) continue;
-                            const int x = (int)(i % g.dims[0]);
-                            const int y = (int)((i / g.dims[0]) % g.dims[1]);
-                            const int z = (int)(i / ((int64_t)g.dims[0] * g.dims[1]));
-                            const int key = (x % px) + px * ((y % py) + py * (z % pz));
-                            if (table[key] == 0) table[key] = r.colors[i];
-                            else ok = table[key] == r.colors[i];
-                        }
-                        if (ok) {
+                        if (!matches(px, py, pz, table, true)) continue;
+                        std::fill(table.begin(), table.end(), 0);
+                        if (matches(px, py, pz, table, false)) {
                             found = true;
                             pattern[0] = px;
                             pattern[1] = py;
@@ -401,12 +465,12 @@ void compile_replica(Replica &r) {
         // lottery.jl:46-92: rounds = maximum realized conflict degree + 1
         r.degrees.assign((size_t)r.n, 0);
         int maxdeg = 0;
-        for (int64_t i = 0; i < r.n; ++i) {
-            if (g.mask && !g.mask[i]) continue;
-            const int c = conflict_neighbours(g, r.conf_off, i, nb.data());
+        for_sites([&](int64_t i, int, int, int, bool interior) {
+            if (g.mask && !g.mask[i]) return;
+            const int c = interior ? kconf : conflict_neighbours(g, r.conf_off, i, nb.data());
             r.degrees[i] = (uint8_t)std::min(c, 255);
             maxdeg = std::max(maxdeg, c);
-        }
+        });
         if (maxdeg + 1 > 255) fail(CPM_E_UNSUPPORTED, "lottery conflict degree exceeds the subround domain");
         r.rounds = maxdeg + 1;
         r.uniform_degree = true;
@@ -492,6 +556,47 @@ void compile_replica(Replica &r) {
             for (int k = 0; k < 32 && k < d.contact.count; ++k)
                 v.cont_lin[k] = (int32_t)(d.contact.offsets[k][0] + X * d.contact.offsets[k][1] + XY * d.contact.offsets[k][2]);
     }
+    r.clist.clear();
+    r.coff.clear();
+    if (d.algorithm == CPM_ALG_CHECKERBOARD && !v.pat_fast) {
+        const int C = r.rounds;
+        const uint32_t ntx = (uint32_t)v.ntiles[0], nty = (uint32_t)v.ntiles[1], ntz = (uint32_t)v.ntiles[2];
+        const uint64_t ntiles = (uint64_t)ntx * nty * ntz;
+        r.clist.resize((size_t)r.n);
+        r.coff.assign((size_t)ntiles * (size_t)(C + 1), 0);
+        std::vector<uint32_t> cnt((size_t)C + 1);
+        uint32_t at = 0;
+        for (uint64_t t = 0; t < ntiles; ++t) {
+            const int tx0 = (int)(t % ntx) * tx, ty0 = (int)((t / ntx) % nty) * ty, tz0 = (int)(t / ((uint64_t)ntx * nty)) * tz;
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (int pass = 0; pass < 2; ++pass) {
+                // pass 0 counts the sites of each colour, pass 1 writes them in position order
+                uint32_t *off = r.coff.data() + t * (uint64_t)(C + 1);
+                if (pass == 1) {
+                    uint32_t run = at;
+                    for (int c = 1; c <= C; ++c) {
+                        off[c - 1] = run;
+                        run += cnt[c];
+                        cnt[c] = off[c - 1];
+                    }
+                    off[C] = run;
+                }
+                for (int iz = 0; iz < tz; ++iz)
+                    for (int iy = 0; iy < ty; ++iy)
+                        for (int ix = 0; ix < tx; ++ix) {
+                            const int gx = tx0 + ix, gy = ty0 + iy, gz = tz0 + iz;
+                            if (gx >= v.dims[0] || gy >= v.dims[1] || gz >= v.dims[2]) continue;
+                            const int64_t site = gx + (int64_t)v.dims[0] * (gy + (int64_t)v.dims[1] * gz);
+                            const int c = r.colors[(size_t)site];
+                            if (c == 0) continue;  // immutable site
+                            if (pass == 0) ++cnt[c];
+                            else r.clist[cnt[c]++] = (uint16_t)(ix + tx * (iy + ty * iz));
+                        }
+                if (pass == 1) at = off[C];
+            }
+        }
+        r.clist.resize(at);
+    }
     {
         // LDS layout of the tile shape: rows of tx + 8 positions (4-position x pads), ty + 2 halo rows
         const int lx = tx + 8, ly = ty + 2 * chalo;
@@ -541,20 +646,25 @@ void compile_replica(Replica &r) {
         for (int a = 0; a < 3; ++a) v.conf.off[k][a] = (int8_t)r.conf_off[3 * k + a];
 }
 
-void allocate_replica(Replica &r) {
+// Device buffers of a replica, filled in stream order (no host synchronization): copy
+// sources are host vectors the replica keeps for its lifetime.
+void allocate_replica(Replica &r, hipStream_t stream) {
     const size_t cap1 = (size_t)r.desc.capacity + 1;
     r.d_lattice = dev_alloc<int32_t>((size_t)r.n, r.bytes);
     if (r.has_mask) {
         r.d_mask = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
-        hip_check(hipMemcpy(r.d_mask, r.mask.data(), (size_t)r.n, hipMemcpyHostToDevice), "copy mask");
+        hip_check(hipMemcpyAsync(r.d_mask, r.mask.data(), (size_t)r.n, hipMemcpyHostToDevice, stream), "copy mask");
     }
-    if (r.desc.algorithm == CPM_ALG_CHECKERBOARD && r.per_site_color) {
-        r.d_color = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
-        hip_check(hipMemcpy(r.d_color, r.colors.data(), (size_t)r.n, hipMemcpyHostToDevice), "copy colours");
+    if (!r.coff.empty()) {
+        r.d_clist = dev_alloc<uint16_t>(std::max<size_t>(r.clist.size(), 1), r.bytes);
+        r.d_coff = dev_alloc<uint32_t>(r.coff.size(), r.bytes);
+        if (!r.clist.empty())
+            hip_check(hipMemcpyAsync(r.d_clist, r.clist.data(), r.clist.size() * 2, hipMemcpyHostToDevice, stream), "copy colour lists");
+        hip_check(hipMemcpyAsync(r.d_coff, r.coff.data(), r.coff.size() * 4, hipMemcpyHostToDevice, stream), "copy colour ranges");
     }
     if (r.desc.algorithm == CPM_ALG_LOTTERY && !r.uniform_degree) {
         r.d_degree = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
-        hip_check(hipMemcpy(r.d_degree, r.degrees.data(), (size_t)r.n, hipMemcpyHostToDevice), "copy degrees");
+        hip_check(hipMemcpyAsync(r.d_degree, r.degrees.data(), (size_t)r.n, hipMemcpyHostToDevice, stream), "copy degrees");
     }
     r.d_vol = dev_alloc<int32_t>(cap1, r.bytes);
     r.d_bnd = dev_alloc<int64_t>(cap1, r.bytes);
@@ -565,20 +675,21 @@ void allocate_replica(Replica &r) {
     r.d_tb = dev_alloc<int64_t>(cap1, r.bytes);
     r.d_bs = dev_alloc<double>(cap1, r.bytes);
     r.d_mvol = dev_alloc<int32_t>(CPM_MAX_MEDIA, r.bytes);
-    hip_check(hipMemset(r.d_lattice, 0, (size_t)r.n * 4), "memset");
-    hip_check(hipMemset(r.d_vol, 0, cap1 * 4), "memset");
-    hip_check(hipMemset(r.d_bnd, 0, cap1 * 8), "memset");
-    hip_check(hipMemset(r.d_ctype, 0, cap1 * 4), "memset");
-    hip_check(hipMemset(r.d_active, 0, cap1), "memset");
-    hip_check(hipMemset(r.d_tv, 0, cap1 * 8), "memset");
-    hip_check(hipMemset(r.d_vs, 0, cap1 * 8), "memset");
-    hip_check(hipMemset(r.d_tb, 0, cap1 * 8), "memset");
-    hip_check(hipMemset(r.d_bs, 0, cap1 * 8), "memset");
-    hip_check(hipMemset(r.d_mvol, 0, CPM_MAX_MEDIA * 4), "memset");
+    hip_check(hipMemsetAsync(r.d_lattice, 0, (size_t)r.n * 4, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_vol, 0, cap1 * 4, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_bnd, 0, cap1 * 8, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_ctype, 0, cap1 * 4, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_active, 0, cap1, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_tv, 0, cap1 * 8, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_vs, 0, cap1 * 8, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_tb, 0, cap1 * 8, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_bs, 0, cap1 * 8, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_mvol, 0, CPM_MAX_MEDIA * 4, stream), "memset");
     cpm::RepDev &v = r.dev;
     v.lattice = r.d_lattice;
     v.mask = r.d_mask;
-    v.site_color = r.d_color;
+    v.colour_list = r.d_clist;
+    v.colour_off = r.d_coff;
     v.site_degree = r.d_degree;
     v.vol = r.d_vol;
     v.bnd = r.d_bnd;
@@ -610,7 +721,8 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     uint64_t rounds = 1;
     for (int a = 0; a < ndim; ++a) rounds *= (uint64_t)(2 * halo + 1);
     const uint64_t cap1 = (uint64_t)d.capacity + 1;
-    const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + n /* colour or degree table */ +
+    const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + 2 * n /* colour lists or degree table */ +
+                         tiles * (rounds + 1) * 4 /* colour ranges */ +
                          cap1 * (4 + 8 + 4 + 1 + 8 * 4) + CPM_MAX_MEDIA * 4;
     const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + 4 + sizeof(cpm::TileInfo) +
                                      rounds * cpm::kSlotFields * 4) +
@@ -659,6 +771,9 @@ struct Engine {
     std::vector<Group> groups;
     unsigned long long *d_report = nullptr;
     unsigned long long *h_report = nullptr;
+    unsigned long long *d_many = nullptr;   // batched report counters (cache_reports)
+    unsigned long long *h_many = nullptr;
+    size_t many_cap = 0;
     uint64_t metrics[6] = {0, 0, 0, 0, 0, 0};
     uint64_t budget = 0;
     uint32_t graph_chunk = 0;
@@ -677,6 +792,8 @@ struct Engine {
         dev_free(d_dyn);
         dev_free(d_report);
         if (h_report) (void)hipHostFree(h_report);
+        dev_free(d_many);
+        if (h_many) (void)hipHostFree(h_many);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -729,9 +846,10 @@ struct Engine {
         drop_graph();
     }
 
+    // Stream-ordered descriptor upload: kernels enqueued earlier still see the old slot.
     void upload_rep(int slot) {
-        hip_check(hipStreamSynchronize(stream), "sync");
-        hip_check(hipMemcpy(d_reps + slot, &reps[slot]->dev, sizeof(cpm::RepDev), hipMemcpyHostToDevice),
+        hip_check(hipMemcpyAsync(d_reps + slot, &reps[slot]->dev, sizeof(cpm::RepDev), hipMemcpyHostToDevice,
+                                 stream),
                   "upload replica");
         metrics[2]++;
     }
@@ -753,7 +871,7 @@ struct Engine {
             for (const Replica *q : reps)
                 if (q) resident += q->estimate;
             if (budget && resident + r->estimate > budget) fail(CPM_E_NOMEM, "HBM budget exceeded");
-            allocate_replica(*r);
+            allocate_replica(*r, stream);
         } catch (...) {
             delete r;
             throw;
@@ -791,16 +909,48 @@ struct Engine {
     }
 
     // Keep the last-MCS report of every replica of g before its accounting arena moves.
+    // One report launch per replica whose last-MCS report is not cached yet, one copy
+    // back and one synchronization for the whole group.
     void cache_reports(Group &g) {
+        if (g.dirty_arena_fresh || g.slots.empty()) return;
+        std::vector<cpm::RepDyn> y((size_t)slot_cap);
+        hip_check(hipMemcpyAsync(y.data(), d_dyn, (size_t)slot_cap * sizeof(cpm::RepDyn), hipMemcpyDeviceToHost,
+                                 stream), "read progress");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        metrics[3]++;
+        std::vector<int> todo;
         for (int s : g.slots) {
-            Replica *r = reps[s];
-            if (!r || g.dirty_arena_fresh) continue;
+            const Replica *r = reps[s];
+            if (!r) continue;
+            const uint64_t m = y[(size_t)s].mcs;
+            if ((r->report_cached && r->cached_mcs == m) || m <= r->base_mcs) continue;
+            todo.push_back(s);
+        }
+        if (todo.empty()) return;
+        if (todo.size() * 8 > many_cap) {
+            dev_free(d_many);
+            if (h_many) (void)hipHostFree(h_many);
+            d_many = nullptr;
+            h_many = nullptr;
+            many_cap = todo.size() * 8 * 2;
+            hip_check(hipMalloc(&d_many, many_cap * sizeof(unsigned long long)), "hipMalloc");
+            hip_check(hipHostMalloc((void **)&h_many, many_cap * sizeof(unsigned long long), hipHostMallocDefault),
+                      "hipHostMalloc");
+        }
+        for (size_t k = 0; k < todo.size(); ++k) {
+            hip_check(cpm_internal_launch_report(d_reps, g.d_slots, g.ntiles, todo[k], d_many + 8 * k, stream),
+                      "report");
+            metrics[0]++;
+        }
+        hip_check(hipMemcpyAsync(h_many, d_many, todo.size() * 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 stream), "copy reports");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        metrics[3]++;
+        for (size_t k = 0; k < todo.size(); ++k) {
             uint64_t out[12];
-            try {
-                report(s, out);
-            } catch (const Failure &f) {
-                if (f.code != CPM_E_STATE) throw;
-            }
+            fill_report(reps[todo[k]], y[(size_t)todo[k]].mcs, h_many + 8 * k, out);
         }
     }
 
@@ -808,7 +958,6 @@ struct Engine {
         Replica *r = check_slot(slot);
         hip_check(hipStreamSynchronize(stream), "sync");
         Group &g = groups[r->group];
-        cache_reports(g);
         g.slots.erase(std::remove(g.slots.begin(), g.slots.end(), slot), g.slots.end());
         g.dirty = true;
         set_dyn(slot, 0, 0, 0, 1 | 2 | 4 | 8);
@@ -1065,7 +1214,11 @@ struct Engine {
         hip_check(hipStreamSynchronize(stream), "sync");
         metrics[1]++;
         metrics[3]++;
-        const unsigned long long *s = h_report;
+        fill_report(r, mcs, h_report, out);
+    }
+
+    // The 12-field report (sequential.jl:236-251) of replica r from its 7 summed counters.
+    void fill_report(Replica *r, uint64_t mcs, const unsigned long long *s, uint64_t out[12]) {
         const uint64_t rounds = (uint64_t)r->rounds;
         out[0] = mcs;
         out[1] = rounds;

@@ -393,8 +393,24 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
     const int K = d.prop.n;
     const bool simple = d.simple;
-    const bool fast = d.pat_fast && d.site_color == nullptr;
+    const bool fast = d.pat_fast;
     const int wave = (int)(threadIdx.x >> 6);
+    // colourings without a fast pattern: this tile's sites of the colour, from the lists
+    uint32_t lbeg = 0, lcnt = 0;
+    if (!fast) {
+        const CPM_GLOBAL uint32_t *off = gp(d.colour_off) + (size_t)(tile - d.tile_base) * (uint32_t)(d.rounds + 1);
+        lbeg = off[colour - 1];
+        lcnt = off[colour] - lbeg;
+        if (lcnt == 0) {
+            // nothing of this colour here: no attempts, no tile load
+            if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = 0;
+            if (threadIdx.x == 0) {
+                uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+                slot[0] = slot[1] = slot[2] = slot[3] = 0;
+            }
+            return;
+        }
+    }
 
     if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
     if constexpr (KF == 0) {
@@ -415,8 +431,6 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
             s_cls[threadIdx.x] = (uint32_t)d.pat_coord[rr][0] | ((uint32_t)d.pat_coord[rr][1] << 8) |
                                  ((uint32_t)d.pat_coord[rr][2] << 16);
         }
-    } else if (threadIdx.x < 64) {
-        s_cls[threadIdx.x] = d.pattern[threadIdx.x];
     }
     load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, nullptr);
     __syncthreads();
@@ -463,19 +477,16 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
             visit(i, ix, iy, iz);
         }
     } else {
-        // every tile site, coloured by the per-site array or the periodic pattern
-        const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
-        for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
-            int i = i0 + (int)threadIdx.x;
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
-            if (gx >= X || gy >= Y || gz >= Z) {
-                i = -1;
-            } else {
-                const uint32_t c = d.site_color
-                    ? gp(d.site_color)[(uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)]
-                    : s_cls[(gx % px) + px * ((gy % py) + py * (gz % pz))];
-                if (c != colour) i = -1;
+        // the tile's sites of this colour (ascending tile positions, all inside the lattice)
+        const CPM_GLOBAL uint16_t *list = gp(d.colour_list) + lbeg;
+        for (uint32_t j0 = 0; j0 < lcnt; j0 += kBlock) {
+            const uint32_t j = j0 + threadIdx.x;
+            int i = -1, ix = 0, iy = 0, iz = 0;
+            if (j < lcnt) {
+                i = list[j];
+                ix = i % TX;
+                iy = (i / TX) % TY;
+                iz = i / (TX * TY);
             }
             visit(i, ix, iy, iz);
         }
@@ -932,7 +943,6 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, int32_t dl, in
     }
 }
 
-constexpr int kCommitChunk = 1024;  // staged attempts per claim-check round
 
 // Claim check of one pass (checkerboard.jl:337-369). Workgroup b owns the segments of
 // tiles [8b, 8b + 8): an attempt wins iff its key is the maximum on both of its cells.

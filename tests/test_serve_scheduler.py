@@ -36,14 +36,17 @@ class FakeBatch:
         self.engine_budget = engine_budget
         self.log = []
 
-    def add(self, compiled, spec, target_mcs=None):
+    def reserve(self, spec):
         if self.engine_budget is not None and sum(self.bytes.values()) + spec.nbytes > self.engine_budget:
             raise EngineError(CPM_E_NOMEM, "HBM budget exceeded")
         s = self.next
         self.next += 1
-        self.mcs[s], self.target[s], self.bytes[s] = 0, target_mcs, spec.nbytes
-        self.log.append(("add", compiled, self.clock()))
+        self.mcs[s], self.target[s], self.bytes[s] = 0, 0, spec.nbytes
         return s
+
+    def load(self, slot, compiled, target_mcs=None):
+        self.target[slot] = target_mcs
+        self.log.append(("add", compiled, self.clock()))
 
     def remove(self, slot):
         del self.mcs[slot], self.target[slot], self.bytes[slot]
