@@ -114,6 +114,35 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
             "device_bytes": device_bytes}
 
 
+def cubes_leg(steps: int, graph_chunk: int):
+    """256^3 CheckerboardSweepCPM: 32,768 cubes of 8^3 tiling the lattice, 3 types, Float32
+    contact matrix U[2, 20], volume (1.5, 512) + 26-neighbour contact + boundary (0.5, 384),
+    T = 20."""
+    from corepotts import init_scientific, workloads
+    log("3D: building 256^3 workload")
+    model = workloads.cubes_3d(256)
+    n = int(model[0].tags.size)
+    integ = init_scientific(*model)
+    integ.set_graph_chunk(graph_chunk)
+    integ.step(2 * graph_chunk)
+    integ.synchronize()
+    log(f"3D: timed {steps} MCS")
+    t0 = time.perf_counter()
+    integ.step(steps)
+    integ.synchronize()
+    dt = time.perf_counter() - t0
+    rep = integ.current_mcs_report()
+    timing = integ.step_timed(4)
+    out = {"lattice": [256, 256, 256], "cells": 32768, "colour_passes_per_mcs": rep.internal_rounds,
+           "copy_attempts_per_s": rep.activated_attempts * steps / dt, "mcs_per_s": steps / dt,
+           "propose_kernel_avg_us": 1e3 * timing["propose_ms"] / max(timing["propose_launches"], 1),
+           "commit_kernel_avg_us": 1e3 * timing["commit_ms"] / max(timing["commit_launches"], 1),
+           "propose_algorithmic_GBps": 4.0 * n / (1e-3 * timing["propose_ms"] / max(timing["propose_launches"], 1)) / 1e9,
+           "last_report": rep._asdict()}
+    integ.close()
+    return out
+
+
 def serving_leg(jobs: int, chunk: int, budget_gb: float):
     """Served stream: the first `jobs` Poisson arrivals at 200 jobs/s (80 % 2D side
     log-uniform [128, 2048], 20 % 3D side log-uniform [32, 128], 100-2000 MCS), admitted under
@@ -150,6 +179,8 @@ def main():
     ap.add_argument("--no-lottery", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--cubes-steps", type=int, default=40)
     ap.add_argument("--batch-replicas", type=int, default=512)
     ap.add_argument("--batch-mcs", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true")
@@ -244,6 +275,8 @@ def main():
                             "last_report": lrep._asdict()}
         del lot
 
+    if not args.no_3d:
+        extra["cubes_3d"] = cubes_leg(args.cubes_steps, 10)
     if not args.no_batch:
         extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
     if not args.no_serve:

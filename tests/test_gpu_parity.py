@@ -112,7 +112,7 @@ def test_reference_fixture(gpu_lib, alg_type, dims):
 @pytest.mark.parametrize("alg", ["checkerboard", "lottery"])
 def test_config1_plain(gpu_lib, alg):
     compiled, proposal, components, algorithm, seed = workloads.config1_plain(algorithm=alg)
-    gpu, sim, reps = run_parity(compiled, proposal, components, algorithm, seed, [1, 10, 25])
+    gpu, sim, reps = run_parity(compiled, proposal, components, algorithm, seed, [1, 10, 25, 100])
     for rep in reps:
         _partition(rep)
 
