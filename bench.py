@@ -118,10 +118,11 @@ def cubes_leg(steps: int, graph_chunk: int):
     """256^3 CheckerboardSweepCPM: 32,768 cubes of 8^3 tiling the lattice, 3 types, Float32
     contact matrix U[2, 20], volume (1.5, 512) + 26-neighbour contact + boundary (0.5, 384),
     T = 20."""
-    from corepotts import init_scientific, workloads
+    from corepotts import init_scientific, tune_tile_shape, workloads
     log("3D: building 256^3 workload")
     model = workloads.cubes_3d(256)
     n = int(model[0].tags.size)
+    tuned = tune_tile_shape(*model, mcs=2, warmup=1)
     integ = init_scientific(*model)
     integ.set_graph_chunk(graph_chunk)
     integ.step(2 * graph_chunk)
@@ -134,6 +135,7 @@ def cubes_leg(steps: int, graph_chunk: int):
     rep = integ.current_mcs_report()
     timing = integ.step_timed(4)
     out = {"lattice": [256, 256, 256], "cells": 32768, "colour_passes_per_mcs": rep.internal_rounds,
+           "tile_shape": tuned,
            "copy_attempts_per_s": rep.activated_attempts * steps / dt, "mcs_per_s": steps / dt,
            "propose_kernel_avg_us": 1e3 * timing["propose_ms"] / max(timing["propose_launches"], 1),
            "commit_kernel_avg_us": 1e3 * timing["commit_ms"] / max(timing["commit_launches"], 1),
@@ -194,7 +196,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from corepotts import copy_probe, init_scientific, workloads
+    from corepotts import copy_probe, init_scientific, tune_tile_shape, workloads
 
     # plain device copy of known size: achievable-bandwidth figure and the calibration
     # kernel for the traffic counters of a profiled run
@@ -204,7 +206,9 @@ def main():
     log("building 4096x4096 workload")
     compiled, proposal, components, alg, seed = workloads.config2_large()
     n_sites = int(compiled.tags.size)
-    log("compiling schedule and uploading")
+    log("tuning the tile shape")
+    tuned = tune_tile_shape(compiled, proposal, components, alg, seed=seed)
+    log(f"tile shape {tuned['shape']} {tuned['tile']}; compiling schedule and uploading")
     integ = init_scientific(compiled, proposal, components, alg, seed=seed)
     integ.set_graph_chunk(args.graph_chunk)
     log(f"warmup {args.warmup} MCS")
@@ -244,6 +248,7 @@ def main():
 
     extra = {
         "mcs_per_s": args.steps / elapsed,
+        "tile_shape": tuned,
         "pcie": {"upload_ms": upload_s * 1e3, "download_ms": download_s * 1e3,
                  "pcie_inclusive_attempts_per_s":
                      attempts_per_mcs * args.steps / (elapsed + upload_s + download_s)},
@@ -262,6 +267,7 @@ def main():
     if not args.no_lottery:
         log(f"lottery {args.lottery_steps} MCS")
         compiled, proposal, components, lalg, lseed = workloads.config2_large(algorithm="lottery")
+        ltuned = tune_tile_shape(compiled, proposal, components, lalg, seed=lseed, mcs=2, warmup=1)
         lot = init_scientific(compiled, proposal, components, lalg, seed=lseed)
         lot.step(5)
         lot.synchronize()
@@ -272,6 +278,7 @@ def main():
         lrep = lot.current_mcs_report()
         extra["lottery"] = {"copy_attempts_per_s": lrep.activated_attempts * args.lottery_steps / dt,
                             "mcs_per_s": args.lottery_steps / dt, "rounds": lrep.internal_rounds,
+                            "tile_shape": ltuned,
                             "last_report": lrep._asdict()}
         del lot
 

@@ -57,7 +57,7 @@ EXPORTS = (
     "cpm_last_error", "cpm_abi_version", "cpm_create", "cpm_destroy", "cpm_upload_state",
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
     "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
-    "cpm_set_graph_chunk", "cpm_copy_probe",
+    "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_step", "cpm_batch_progress",
@@ -99,6 +99,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_set_graph_chunk.argtypes = [P, C.c_uint32]
     lib.cpm_step_timed.argtypes = [P, C.c_uint64, C.POINTER(C.c_double)]
     lib.cpm_copy_probe.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
+    lib.cpm_tile_shape_info.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
+    lib.cpm_set_tile_shape.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     lib.cpm_ensemble_seed.restype = C.c_uint64
     lib.cpm_ensemble_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
     lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]

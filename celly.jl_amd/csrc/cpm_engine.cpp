@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -159,6 +160,19 @@ bool has_term(const cpm_model_desc &d, int term) {
     for (int t = 0; t < d.n_energies; ++t)
         if (d.energy_order[t] == term) return true;
     return false;
+}
+
+// Process-wide tile-shape preference per (algorithm, dimension, halo); -1 = default shape.
+std::atomic<int> g_shape_pref[3][4][3];
+
+int default_shape(int ndim, int halo) { return ndim == 2 ? (halo == 1 ? 0 : 1) : (halo == 1 ? 2 : 3); }
+
+int tile_shape_for(int algorithm, int ndim, int halo) {
+    if (algorithm >= 1 && algorithm <= 2 && ndim >= 2 && ndim <= 3 && halo >= 1 && halo <= 2) {
+        const int p = g_shape_pref[algorithm][ndim][halo].load();
+        if (p > 0) return p - 1;  // stored as shape + 1 (0 = default)
+    }
+    return default_shape(ndim, halo);
 }
 
 void validate_relation(const cpm_relation_desc &r, int ndim, const char *name) {
@@ -483,10 +497,11 @@ void compile_replica(Replica &r) {
         }
     }
 
-    // tile shape by dimension and halo; bit 3 selects the kernels for proposal relations
-    // other than 2 * ndim directions (the fixed-count kernels keep the offsets in registers)
-    const int shape = d.ndim == 2 ? (r.halo == 1 ? 0 : 1) : (r.halo == 1 ? 2 : 3);
-    r.config = shape | (d.proposal.count == 2 * d.ndim ? 0 : 8);
+    // tile shape by dimension and halo (process-wide preference, else the default); bit 4
+    // selects the kernels for proposal relations other than 2 * ndim directions (the
+    // fixed-count kernels keep the offsets in registers)
+    const int shape = tile_shape_for(d.algorithm, d.ndim, r.halo);
+    r.config = shape | (d.proposal.count == 2 * d.ndim ? 0 : 16);
     int cdim, chalo, tx, ty, tz;
     cpm_internal_tile_config(shape, &cdim, &chalo, &tx, &ty, &tz);
 
@@ -713,7 +728,7 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     if (has_term(d, CPM_TERM_CONTACT)) halo = std::max(halo, max_abs_offset(d.contact, ndim));
     halo = std::min(std::max(halo, 1), 2);
     int cdim, chalo, t[3];
-    cpm_internal_tile_config(ndim == 2 ? (halo == 1 ? 0 : 1) : (halo == 1 ? 2 : 3), &cdim, &chalo, &t[0],
+    cpm_internal_tile_config(tile_shape_for(d.algorithm, ndim, halo), &cdim, &chalo, &t[0],
                              &t[1], &t[2]);
     uint64_t tiles = 1;
     for (int a = 0; a < 3; ++a) tiles *= (uint64_t)((dims[a] + t[a] - 1) / t[a]);
@@ -1529,6 +1544,29 @@ uint64_t cpm_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t r
     uint32_t c[4] = {(uint32_t)c0, (uint32_t)(c0 >> 32), (uint32_t)c1, (uint32_t)(c1 >> 32)};
     philox_host(c, (uint32_t)key, (uint32_t)(key >> 32));
     return (uint64_t)c[0] | ((uint64_t)c[1] << 32);
+}
+
+int32_t cpm_tile_shape_info(int32_t shape, int32_t out[5]) {
+    if (!out || shape < 0 || shape > 15) return CPM_E_INVALID;
+    int dim, halo, tx, ty, tz;
+    if (!cpm_internal_tile_config(shape, &dim, &halo, &tx, &ty, &tz)) return CPM_E_INVALID;
+    out[0] = dim;
+    out[1] = halo;
+    out[2] = tx;
+    out[3] = ty;
+    out[4] = tz;
+    return CPM_OK;
+}
+
+int32_t cpm_set_tile_shape(int32_t algorithm, int32_t ndim, int32_t halo, int32_t shape) {
+    if (algorithm < 1 || algorithm > 2 || ndim < 2 || ndim > 3 || halo < 1 || halo > 2) return CPM_E_INVALID;
+    if (shape >= 0) {
+        int dim, h, tx, ty, tz;
+        if (shape > 15 || !cpm_internal_tile_config(shape, &dim, &h, &tx, &ty, &tz) || dim != ndim || h != halo)
+            return CPM_E_INVALID;
+    }
+    g_shape_pref[algorithm][ndim][halo].store(shape >= 0 ? shape + 1 : 0);
+    return CPM_OK;
 }
 
 int32_t cpm_copy_probe(uint64_t bytes, int32_t reps, double *out_ms) {

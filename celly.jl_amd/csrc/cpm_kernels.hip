@@ -619,13 +619,17 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     for (int k = 0; k < (KF > 0 ? KF : 1); ++k) po[k] = d.prop_lds[k];
     load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, s_sid);
     __syncthreads();
-    // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1)
-    for (int i = threadIdx.x; i < G::LN; i += kBlock) {
-        uint32_t t = 0;
-        if (s_mut[i])
-            t = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
-                          s_sid[i], 0, 0).w0;
-        s_tk[i] = t;
+    // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1);
+    // two independent draws per iteration keep two Philox chains in flight
+    for (int i = threadIdx.x; i < G::LN; i += 2 * kBlock) {
+        const int i2 = i + kBlock;
+        const bool m1 = s_mut[i] != 0, m2 = i2 < G::LN && s_mut[i2] != 0;
+        const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                                      s_sid[i], 0, 0).w0;
+        const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                                      i2 < G::LN ? s_sid[i2] : 0u, 0, 0).w0;
+        s_tk[i] = m1 ? t1 : 0u;
+        if (i2 < G::LN) s_tk[i2] = m2 ? t2 : 0u;
     }
     __syncthreads();
 
@@ -643,21 +647,29 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
                 if (s_mut[pos]) {
                     const uint32_t site = s_sid[pos];
                     const uint32_t mine = s_tk[pos];
-                    bool wins = true;
-                    for (int k = 0; k < KC && wins; ++k) {
+                    // strict lexicographic ticket maximum over the conflict neighbours
+                    // (lottery.jl:153-185): word 1 decides unless it ties, which needs the
+                    // full tickets (rare)
+                    bool beaten = false, tied = false;
+#pragma unroll 8
+                    for (int k = 0; k < KC; ++k) {
                         const int q = pos + s_cloff[k];
-                        if (!s_mut[q]) continue;
-                        const uint32_t nsite = s_sid[q];
-                        if (nsite == site) continue;
+                        const bool other_site = s_mut[q] && s_sid[q] != site;
                         const uint32_t other = s_tk[q];
-                        if (other > mine) {
-                            wins = false;
-                        } else if (other == mine) {
-                            // full lexicographic ticket comparison (lottery.jl:168-176)
+                        beaten |= other_site && other > mine;
+                        tied |= other_site && other == mine;
+                    }
+                    bool wins = !beaten;
+                    if (wins && tied) {
+                        const Words b4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0,
+                                                   kSiteEntity, site, 0, 0);
+                        for (int k = 0; k < KC && wins; ++k) {
+                            const int q = pos + s_cloff[k];
+                            if (!s_mut[q]) continue;
+                            const uint32_t nsite = s_sid[q];
+                            if (nsite == site || s_tk[q] != mine) continue;
                             const Words a4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
                                                        mcs, sub, 0, kSiteEntity, nsite, 0, 0);
-                            const Words b4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
-                                                       mcs, sub, 0, kSiteEntity, site, 0, 0);
                             bool better;
                             if (a4.w1 != b4.w1) better = a4.w1 > b4.w1;
                             else if (a4.w2 != b4.w2) better = a4.w2 > b4.w2;
@@ -1189,14 +1201,15 @@ static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
 
 }  // namespace cpm
 
-// Tile configurations: id & 7 -> (dim, halo, TX, TY, TZ); bit 3 of id selects the kernels
+// Tile configurations: id & 15 -> (dim, halo, TX, TY, TZ); bit 4 of id selects the kernels
 // for proposal relations whose direction count differs from 2 * dim.
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz) {
-    if (id < 0 || id > 15) return 0;
-    id &= 7;
+    if (id < 0 || id > 31) return 0;
+    id &= 15;
     static const int table[][5] = {
         {2, 1, 128, 16, 1}, {2, 2, 128, 16, 1}, {3, 1, 32, 8, 8}, {3, 2, 32, 8, 8},
         {2, 1, 64, 32, 1},  {2, 1, 256, 8, 1},  {2, 1, 64, 16, 1}, {2, 1, 32, 32, 1},
+        {3, 1, 64, 8, 4},   {3, 1, 32, 16, 4},
     };
     const int n = (int)(sizeof(table) / sizeof(table[0]));
     if (id < 0 || id >= n) return 0;
@@ -1211,9 +1224,9 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream) {
     using namespace cpm;
-    if (config < 0 || config > 15) return hipErrorInvalidValue;
-    const int g = config >> 3;
-    switch (config & 7) {
+    if (config < 0 || config > 31) return hipErrorInvalidValue;
+    const int g = (config >> 4) & 1;
+    switch (config & 15) {
     case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, g, *a, stream);
     case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, g, *a, stream);
     case 2: return launch_propose_t<3, 1, 32, 8, 8>(algorithm, g, *a, stream);
@@ -1222,6 +1235,8 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
     case 5: return launch_propose_t<2, 1, 256, 8, 1>(algorithm, g, *a, stream);
     case 6: return launch_propose_t<2, 1, 64, 16, 1>(algorithm, g, *a, stream);
     case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, g, *a, stream);
+    case 8: return launch_propose_t<3, 1, 64, 8, 4>(algorithm, g, *a, stream);
+    case 9: return launch_propose_t<3, 1, 32, 16, 4>(algorithm, g, *a, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -163,6 +163,15 @@ int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]);
  * the measurement hook behind the per-kernel roofline.) */
 int32_t cpm_step_timed(cpm_ctx *ctx, uint64_t n, double out[8]);
 
+/* Tile shapes of the propose kernels (the autotuner's search space): out = {dim, halo,
+ * TX, TY, TZ} of shape id 0..15; CPM_E_INVALID for ids without a shape. (No counterpart in
+ * the reference: launch geometry of this engine.) */
+int32_t cpm_tile_shape_info(int32_t shape, int32_t out[5]);
+/* Process-wide preferred tile shape for replicas of (algorithm, ndim, halo) created after
+ * the call; shape -1 restores the default. The shape's dim and halo must match. Results do
+ * not depend on the shape. (No counterpart in the reference.) */
+int32_t cpm_set_tile_shape(int32_t algorithm, int32_t ndim, int32_t halo, int32_t shape);
+
 /* Device-to-device copy of `bytes` bytes (a multiple of 16) with a plain 16-byte-per-lane
  * kernel, launched once untimed and `reps` times timed; *out_ms = mean launch time. Errors
  * are reported through cpm_last_error(NULL). (No counterpart in the reference: the
