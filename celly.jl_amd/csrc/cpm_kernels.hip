@@ -99,7 +99,7 @@ __device__ inline Realized realize_position(const D &d, int gx, int gy, int gz) 
     return r;
 }
 
-template <int DIM, int H, int TX, int TY, int TZ>
+template <int DIM, int H, int TX, int TY, int TZ, int NT = kBlock>
 __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_t *s_own,
                                  uint8_t *s_mut, uint32_t *s_sid) {
     using G = Geom<DIM, H, TX, TY, TZ>;
@@ -108,11 +108,11 @@ __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_
     const CPM_GLOBAL int32_t *lat = gp(d.lattice);
     if (d.simple && (X & 3) == 0 && x0 >= G::PX && x0 + TX + G::PX <= X && inner_yz) {
         // interior tile of a periodic, obstacle-free lattice: 16-byte loads, all in flight
-        constexpr int NL = (G::NV + kBlock - 1) / kBlock;
+        constexpr int NL = (G::NV + NT - 1) / NT;
         v4i v[NL];
 #pragma unroll
         for (int j = 0; j < NL; ++j) {
-            const int jj = (int)threadIdx.x + j * kBlock;
+            const int jj = (int)threadIdx.x + j * NT;
             if (jj < G::NV) {
                 const int row = jj / G::LV, c4 = jj % G::LV;
                 const int gx = x0 - G::PX + 4 * c4, gy = y0 - H + row % G::LY, gz = z0 - G::HZ + row / G::LY;
@@ -121,7 +121,7 @@ __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_
         }
 #pragma unroll
         for (int j = 0; j < NL; ++j) {
-            const int jj = (int)threadIdx.x + j * kBlock;
+            const int jj = (int)threadIdx.x + j * NT;
             if (jj < G::NV) {
                 const int row = jj / G::LV, c4 = jj % G::LV;
                 *reinterpret_cast<v4i *>(s_own + row * G::LX + 4 * c4) = v[j];
@@ -133,17 +133,17 @@ __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_
             }
         }
         if (s_sid)  // lottery kernels read the mutable flags of every position
-            for (int i = threadIdx.x; i < G::LN; i += kBlock) s_mut[i] = 1;
+            for (int i = threadIdx.x; i < G::LN; i += NT) s_mut[i] = 1;
         return;
     }
-    constexpr int NL = (G::LN + kBlock - 1) / kBlock;
+    constexpr int NL = (G::LN + NT - 1) / NT;
     const bool interior = x0 >= G::PX && x0 + TX + G::PX <= X && inner_yz;
     const CPM_GLOBAL uint8_t *mask = d.mask ? gp(d.mask) : nullptr;
     int32_t w[NL];
     // every load of the tile is issued before the first LDS store
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-        const int i = (int)threadIdx.x + j * kBlock;
+        const int i = (int)threadIdx.x + j * NT;
         w[j] = 0;
         if (i < G::LN) {
             const int lx = i % G::LX;
@@ -172,7 +172,7 @@ __device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_
     }
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-        const int i = (int)threadIdx.x + j * kBlock;
+        const int i = (int)threadIdx.x + j * NT;
         if (i < G::LN) s_own[i] = w[j];
     }
 }
@@ -362,10 +362,11 @@ __device__ __forceinline__ int direction_offset(uint32_t dir, const int32_t *, c
 // ---------------------------------------------------------------------------
 // Checkerboard colour pass, propose + claim (checkerboard.jl:244-304)
 // ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ, int KF>
-__global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
+template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT>
+__global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
     using G = Geom<DIM, H, TX, TY, TZ>;
-    constexpr int Q = G::TN / kWaves;  // capacity of a wave-private list
+    constexpr int NW = NT / 64;        // waves per workgroup
+    constexpr int Q = G::TN / NW;      // capacity of a wave-private list
     static_assert(G::TN <= 4096, "packed 16-bit tile counters");
     __shared__ __align__(16) int32_t s_own[G::LN];
     __shared__ uint8_t s_mut[G::LN];
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     __shared__ uint8_t s_dir[G::TN];
     __shared__ int32_t s_ploff[KF > 0 ? KF : CPM_MAX_DIRS];
     __shared__ uint32_t s_cls[64];
-    __shared__ uint32_t s_len[2][kWaves];
+    __shared__ uint32_t s_len[2][NW];
     __shared__ uint32_t s_sum[2];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
@@ -414,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
 
     if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
     if constexpr (KF == 0) {
-        for (int k = threadIdx.x; k < K; k += kBlock)
+        for (int k = threadIdx.x; k < K; k += NT)
             s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
     } else {
         if (threadIdx.x < KF) s_ploff[threadIdx.x] = d.prop_lds[threadIdx.x];
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
                                  ((uint32_t)d.pat_coord[rr][2] << 16);
         }
     }
-    load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, nullptr);
+    load_tile<DIM, H, TX, TY, TZ, NT>(d, x0, y0, z0, s_own, s_mut, nullptr);
     __syncthreads();
 
     // 1. colour sites; interior sites (every proposal neighbour a mutable site of the
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
         const int shift = d.cls_shift, sx = d.cls_sx, sy = d.cls_sy;
         const int px = d.pat[0], py = d.pat[1], pz = d.pat[2];
         const int total = ccount << shift;
-        for (int j0 = 0; j0 < total; j0 += kBlock) {
+        for (int j0 = 0; j0 < total; j0 += NT) {
             const int j = j0 + (int)threadIdx.x;
             int i = -1, ix = 0, iy = 0, iz = 0;
             if (j < total) {
@@ -479,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     } else {
         // the tile's sites of this colour (ascending tile positions, all inside the lattice)
         const CPM_GLOBAL uint16_t *list = gp(d.colour_list) + lbeg;
-        for (uint32_t j0 = 0; j0 < lcnt; j0 += kBlock) {
+        for (uint32_t j0 = 0; j0 < lcnt; j0 += NT) {
             const uint32_t j = j0 + threadIdx.x;
             int i = -1, ix = 0, iy = 0, iz = 0;
             if (j < lcnt) {
@@ -495,10 +496,11 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
     __syncthreads();
 
     // 2. direction draws of the boundary sites (ProposalDirectionStream), attempt outcome
-    const uint32_t p1 = s_len[0][0], p2 = p1 + s_len[0][1], p3 = p2 + s_len[0][2];
-    const uint32_t nwork = p3 + s_len[0][3];
+    const uint32_t p1 = s_len[0][0], p2 = p1 + (NW > 1 ? s_len[0][NW > 1 ? 1 : 0] : 0u);
+    const uint32_t p3 = p2 + (NW > 2 ? s_len[0][NW > 2 ? 2 : 0] : 0u);
+    const uint32_t nwork = p3 + (NW > 3 ? s_len[0][NW > 3 ? 3 : 0] : 0u);
     uint32_t na = 0;
-    for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
+    for (uint32_t e0 = 0; e0 < nwork; e0 += NT) {
         const uint32_t e = e0 + threadIdx.x;
         bool act = false;
         int i = 0;
@@ -535,11 +537,12 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
 
     // 3. priority draws and claims of the actionable attempts; the tile segment keeps the
     //    wave-private layout of the lists (stretch w = entries [w * Q, w * Q + length w))
-    const uint32_t q1 = s_len[1][0], q2 = q1 + s_len[1][1], q3 = q2 + s_len[1][2];
-    const uint32_t nact = q3 + s_len[1][3];
+    const uint32_t q1 = s_len[1][0], q2 = q1 + (NW > 1 ? s_len[1][NW > 1 ? 1 : 0] : 0u);
+    const uint32_t q3 = q2 + (NW > 2 ? s_len[1][NW > 2 ? 2 : 0] : 0u);
+    const uint32_t nact = q3 + (NW > 3 ? s_len[1][NW > 3 ? 3 : 0] : 0u);
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
     CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
-    for (uint32_t e = threadIdx.x; e < nact; e += kBlock) {
+    for (uint32_t e = threadIdx.x; e < nact; e += NT) {
         const uint32_t at = list_slot<Q>(e, q1, q2, q3);
         const int i = s_act[at];
         const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
@@ -558,7 +561,8 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
         seg[at] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
     }
 
-    if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = s_len[1][threadIdx.x];
+    if (threadIdx.x < kWaves)
+        a.seg_count[tile * kWaves + threadIdx.x] = (int)threadIdx.x < NW ? s_len[1][threadIdx.x % NW] : 0u;
     if (threadIdx.x == 0) {
         uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
         const uint32_t lo = s_sum[0], hi = s_sum[1];
@@ -572,10 +576,11 @@ __global__ __launch_bounds__(kBlock) void cb_propose_kernel(PassArgs a) {
 // ---------------------------------------------------------------------------
 // Lottery round, propose + claim (lottery.jl:153-256)
 // ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ, int KF>
-__global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
+template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT>
+__global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     using G = Geom<DIM, H, TX, TY, TZ>;
-    constexpr int Q = G::TN / kWaves;
+    constexpr int NW = NT / 64;
+    constexpr int Q = G::TN / NW;
     static_assert(G::TN <= 4096, "packed 16-bit tile counters");
     __shared__ __align__(16) int32_t s_own[G::LN];
     __shared__ uint8_t s_mut[G::LN];
@@ -584,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     __shared__ uint16_t s_work[G::TN];
     __shared__ int32_t s_ploff[KF > 0 ? KF : CPM_MAX_DIRS];
     __shared__ int32_t s_cloff[CPM_MAX_DIRS];
-    __shared__ uint32_t s_len[kWaves];
+    __shared__ uint32_t s_len[NW];
     __shared__ uint32_t s_sum[2];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
@@ -607,22 +612,22 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
 
     if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
     if constexpr (KF == 0) {
-        for (int k = threadIdx.x; k < K; k += kBlock)
+        for (int k = threadIdx.x; k < K; k += NT)
             s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
     } else {
         if (threadIdx.x < KF) s_ploff[threadIdx.x] = d.prop_lds[threadIdx.x];
     }
-    for (int k = threadIdx.x; k < KC; k += kBlock)
+    for (int k = threadIdx.x; k < KC; k += NT)
         s_cloff[k] = d.conf.off[k][0] + G::LX * (d.conf.off[k][1] + G::LY * d.conf.off[k][2]);
     int32_t po[KF > 0 ? KF : 1];
 #pragma unroll
     for (int k = 0; k < (KF > 0 ? KF : 1); ++k) po[k] = d.prop_lds[k];
-    load_tile<DIM, H, TX, TY, TZ>(d, x0, y0, z0, s_own, s_mut, s_sid);
+    load_tile<DIM, H, TX, TY, TZ, NT>(d, x0, y0, z0, s_own, s_mut, s_sid);
     __syncthreads();
     // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1);
     // two independent draws per iteration keep two Philox chains in flight
-    for (int i = threadIdx.x; i < G::LN; i += 2 * kBlock) {
-        const int i2 = i + kBlock;
+    for (int i = threadIdx.x; i < G::LN; i += 2 * NT) {
+        const int i2 = i + NT;
         const bool m1 = s_mut[i] != 0, m2 = i2 < G::LN && s_mut[i2] != 0;
         const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
                                       s_sid[i], 0, 0).w0;
@@ -636,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     uint32_t c01 = 0, c23 = 0;  // activated | same owner << 16, boundary null | realized << 16
     uint32_t nw = 0;
     uint16_t *my_work = s_work + wave * Q;
-    for (int i0 = 0; i0 < G::TN; i0 += kBlock) {
+    for (int i0 = 0; i0 < G::TN; i0 += NT) {
         const int i = i0 + (int)threadIdx.x;
         bool need = false;
         if (i < G::TN) {
@@ -700,12 +705,13 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     if (__lane_id() == 0) s_len[wave] = nw;
     __syncthreads();
 
-    const uint32_t p1 = s_len[0], p2 = p1 + s_len[1], p3 = p2 + s_len[2];
-    const uint32_t nwork = p3 + s_len[3];
+    const uint32_t p1 = s_len[0], p2 = p1 + (NW > 1 ? s_len[NW > 1 ? 1 : 0] : 0u);
+    const uint32_t p3 = p2 + (NW > 2 ? s_len[NW > 2 ? 2 : 0] : 0u);
+    const uint32_t nwork = p3 + (NW > 3 ? s_len[NW > 3 ? 3 : 0] : 0u);
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
     CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
     uint32_t na = 0;  // actionable attempts of this wave (entries e with the same position)
-    for (uint32_t e0 = 0; e0 < nwork; e0 += kBlock) {
+    for (uint32_t e0 = 0; e0 < nwork; e0 += NT) {
         const uint32_t e = e0 + threadIdx.x;
         bool act = false;
         v4i v = v4i{0, 0, 0, 0};
@@ -741,6 +747,7 @@ __global__ __launch_bounds__(kBlock) void lot_propose_kernel(PassArgs a) {
     }
     block_add_packed(c01, c23, s_sum);
     if (__lane_id() == 0) a.seg_count[tile * kWaves + wave] = na;
+    if ((int)threadIdx.x >= NW && threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
@@ -1169,22 +1176,22 @@ __global__ __launch_bounds__(kBlock) void report_kernel(const RepDev *reps, cons
 // Launchers
 // ---------------------------------------------------------------------------
 // generic == 0: the proposal relation has 2 * DIM directions (offsets in registers)
-template <int DIM, int H, int TX, int TY, int TZ>
+template <int DIM, int H, int TX, int TY, int TZ, int NT = kBlock>
 static hipError_t launch_propose_t(int algorithm, int generic, const PassArgs &a, hipStream_t stream) {
     constexpr int KF = 2 * DIM;
     if (algorithm == CPM_ALG_LOTTERY) {
         if (generic)
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0>), dim3(a.ntiles), dim3(kBlock), 0,
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF>), dim3(a.ntiles), dim3(kBlock), 0,
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
     } else {
         if (generic)
-            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, 0>), dim3(a.ntiles), dim3(kBlock), 0,
+            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, 0, NT>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else
-            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, KF>), dim3(a.ntiles), dim3(kBlock), 0,
+            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, KF, NT>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
     }
     return hipGetLastError();
@@ -1210,6 +1217,8 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
         {2, 1, 128, 16, 1}, {2, 2, 128, 16, 1}, {3, 1, 32, 8, 8}, {3, 2, 32, 8, 8},
         {2, 1, 64, 32, 1},  {2, 1, 256, 8, 1},  {2, 1, 64, 16, 1}, {2, 1, 32, 32, 1},
         {3, 1, 64, 8, 4},   {3, 1, 32, 16, 4},
+        // one-wave workgroups
+        {2, 1, 32, 16, 1},  {2, 1, 64, 8, 1},
     };
     const int n = (int)(sizeof(table) / sizeof(table[0]));
     if (id < 0 || id >= n) return 0;
@@ -1237,6 +1246,8 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
     case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, g, *a, stream);
     case 8: return launch_propose_t<3, 1, 64, 8, 4>(algorithm, g, *a, stream);
     case 9: return launch_propose_t<3, 1, 32, 16, 4>(algorithm, g, *a, stream);
+    case 10: return launch_propose_t<2, 1, 32, 16, 1, 64>(algorithm, g, *a, stream);
+    case 11: return launch_propose_t<2, 1, 64, 8, 1, 64>(algorithm, g, *a, stream);
     default: return hipErrorInvalidValue;
     }
 }
