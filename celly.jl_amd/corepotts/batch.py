@@ -123,6 +123,10 @@ class ReplicaBatch:
         """The replica advances (in later steps) until its MCS counter reaches target_mcs."""
         self._check(self.lib.cpm_batch_set_target(self._b, int(slot), int(target_mcs)))
 
+    def set_mcs(self, slot: int, mcs: int):
+        """Set a replica's MCS counter (checkpoint restore); it idles until set_target."""
+        self._check(self.lib.cpm_batch_set_mcs(self._b, int(slot), int(mcs)))
+
     def step(self, n: int = 1):
         """Enqueue n MCS; each replica below its target performs min(n, remaining) of them."""
         if n < 0:

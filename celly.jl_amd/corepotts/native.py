@@ -60,7 +60,7 @@ EXPORTS = (
     "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
-    "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_step", "cpm_batch_progress",
+    "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
     "cpm_batch_capacity", "cpm_batch_report", "cpm_batch_sync", "cpm_batch_set_graph_chunk",
     "cpm_batch_bytes",
 )
@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_batch_upload_state.argtypes = [P, C.c_int32] + state_args
     lib.cpm_batch_download_state.argtypes = [P, C.c_int32] + state_args
     lib.cpm_batch_set_target.argtypes = [P, C.c_int32, C.c_uint64]
+    lib.cpm_batch_set_mcs.argtypes = [P, C.c_int32, C.c_uint64]
     lib.cpm_batch_step.argtypes = [P, C.c_uint64]
     lib.cpm_batch_progress.argtypes = [P, P, C.c_int32]
     lib.cpm_batch_capacity.argtypes = [P]

@@ -1686,6 +1686,16 @@ int32_t cpm_batch_set_target(cpm_batch *b, int32_t slot, uint64_t target_mcs) {
     });
 }
 
+int32_t cpm_batch_set_mcs(cpm_batch *b, int32_t slot, uint64_t mcs) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] {
+        Replica *r = b->engine.check_slot(slot);
+        b->engine.set_dyn(slot, mcs, mcs, 1, 1 | 2 | 8);
+        r->base_mcs = mcs;
+        r->report_cached = false;
+    });
+}
+
 int32_t cpm_batch_step(cpm_batch *b, uint64_t n) {
     if (!b) return CPM_E_INVALID;
     return guarded(&b->engine.err, [&] { b->engine.step(n); });

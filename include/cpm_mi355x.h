@@ -214,6 +214,9 @@ int32_t cpm_batch_download_state(cpm_batch *b, int32_t slot, uint8_t *tags, uint
                                  uint32_t *cell_types, double *target_volume,
                                  double *volume_strength, int64_t *target_boundary,
                                  double *boundary_strength);
+/* Set a replica's MCS counter (checkpoint restore; integrator.mcs, sequential.jl:216-234);
+ * its target becomes the same value, so it stays idle until the next cpm_batch_set_target. */
+int32_t cpm_batch_set_mcs(cpm_batch *b, int32_t slot, uint64_t mcs);
 /* The replica advances until its MCS counter reaches target_mcs. */
 int32_t cpm_batch_set_target(cpm_batch *b, int32_t slot, uint64_t target_mcs);
 /* Enqueue up to n MCS for every replica below its target (asynchronous). */
