@@ -69,12 +69,23 @@ def state_digest(state: dict) -> str:
     return h.hexdigest()
 
 
+def default_tile_shapes():
+    """Tile-shape preferences back to the engine defaults (a tuned preference is process-wide
+    and is tuned for one lattice)."""
+    from corepotts import set_tile_shape
+    for algorithm in (1, 2):
+        for ndim in (2, 3):
+            for halo in (1, 2):
+                set_tile_shape(algorithm, ndim, halo, -1)
+
+
 def batched_leg(replicas: int, mcs: int, graph_chunk: int):
     """Batched ensemble of `replicas` 2D replicas (sides log-uniform in [64, 1024], T, J and
     seed per replica) for `mcs` MCS in one ReplicaBatch, against the same replicas run one
     after another through the plain single-lattice path on the same GPU."""
     from corepotts import ReplicaBatch, init_scientific, workloads
     from corepotts.spec import ALG_CHECKERBOARD
+    default_tile_shapes()
     log(f"batched: building {replicas} replicas")
     t0 = time.perf_counter()
     models = [workloads.ensemble_replica(r) for r in range(replicas)]
@@ -151,6 +162,7 @@ def serving_leg(jobs: int, chunk: int, budget_gb: float):
     an HBM budget and advanced by continuous batching; requests are built before the clock
     starts, admission (compile, allocation, upload) and result download count."""
     from corepotts import JobServer, latency_summary, make_request, workloads
+    default_tile_shapes()
     log(f"serving: building {jobs} requests")
     t0 = time.perf_counter()
     stream = workloads.serving_jobs(jobs)
@@ -188,6 +200,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-chunk", type=int, default=20)
+    ap.add_argument("--tile-shape", type=int, default=-1,
+                    help="2D checkerboard tile shape of the headline run (default: autotuned)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -196,7 +210,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from corepotts import copy_probe, init_scientific, tune_tile_shape, workloads
+    from corepotts import copy_probe, init_scientific, set_tile_shape, tile_shapes, tune_tile_shape, workloads
+    from corepotts.spec import ALG_CHECKERBOARD
 
     # plain device copy of known size: achievable-bandwidth figure and the calibration
     # kernel for the traffic counters of a profiled run
@@ -206,8 +221,12 @@ def main():
     log("building 4096x4096 workload")
     compiled, proposal, components, alg, seed = workloads.config2_large()
     n_sites = int(compiled.tags.size)
-    log("tuning the tile shape")
-    tuned = tune_tile_shape(compiled, proposal, components, alg, seed=seed)
+    if args.tile_shape >= 0:
+        set_tile_shape(ALG_CHECKERBOARD, 2, 1, args.tile_shape)
+        tuned = {"shape": args.tile_shape, "tile": dict(tile_shapes(2, 1))[args.tile_shape], "pinned": True}
+    else:
+        log("tuning the tile shape")
+        tuned = tune_tile_shape(compiled, proposal, components, alg, seed=seed)
     log(f"tile shape {tuned['shape']} {tuned['tile']}; compiling schedule and uploading")
     integ = init_scientific(compiled, proposal, components, alg, seed=seed)
     integ.set_graph_chunk(args.graph_chunk)

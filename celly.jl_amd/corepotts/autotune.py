@@ -35,6 +35,12 @@ def set_tile_shape(algorithm: int, ndim: int, halo: int, shape: int):
         raise EngineError(rc, "tile shape does not match the dimension and halo")
 
 
+def set_row_kernels(enable: bool):
+    """Row propose kernels (2D periodic von Neumann checkerboards with a 2 x 2 colour pattern)
+    on or off for replicas created afterwards; on by default. Results do not depend on it."""
+    load_library().cpm_set_row_kernels(1 if enable else 0)
+
+
 def model_halo(spec) -> int:
     """Largest |offset| component over the relations the kernels stage (at least 1)."""
     h = 1
@@ -48,14 +54,17 @@ def model_halo(spec) -> int:
 def tune_tile_shape(state, proposal_relation, components, algorithm, seed: int = 0,
                     mcs: int = 4, warmup: int = 2) -> Dict:
     """Times each candidate shape on this model (event-timed MCS, after warmup MCS) and sets
-    the fastest as the preference. Returns {"shape", "tile", "ms_per_mcs": {shape: ms}}."""
+    the fastest as the preference. Returns {"shape", "tile", "ms_per_mcs": {shape: ms},
+    "row_kernel": {shape: whether the row propose kernel ran}}."""
     spec = build_spec(state, proposal_relation, components, algorithm, seed)
     ndim, halo = spec.ndim, model_halo(spec)
     timings = {}
     tiles = {}
+    rows = {}
     for shape, tile in tile_shapes(ndim, halo):
         set_tile_shape(spec.algorithm, ndim, halo, shape)
         integ = init_scientific(state, proposal_relation, components, algorithm, seed=seed)
+        rows[shape] = bool(integ.kernel_config() & 32)
         integ.step(warmup)
         t = integ.step_timed(mcs)
         integ.close()
@@ -63,4 +72,4 @@ def tune_tile_shape(state, proposal_relation, components, algorithm, seed: int =
         tiles[shape] = tile
     best = min(timings, key=timings.get)
     set_tile_shape(spec.algorithm, ndim, halo, best)
-    return {"shape": best, "tile": list(tiles[best]), "ms_per_mcs": timings}
+    return {"shape": best, "tile": list(tiles[best]), "ms_per_mcs": timings, "row_kernel": rows}

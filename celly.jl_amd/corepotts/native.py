@@ -58,6 +58,7 @@ EXPORTS = (
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
     "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
     "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
+    "cpm_set_row_kernels", "cpm_kernel_config",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
@@ -101,6 +102,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_copy_probe.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
     lib.cpm_tile_shape_info.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
     lib.cpm_set_tile_shape.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    lib.cpm_set_row_kernels.argtypes = [C.c_int32]
+    lib.cpm_kernel_config.argtypes = [P, C.POINTER(C.c_int32)]
     lib.cpm_ensemble_seed.restype = C.c_uint64
     lib.cpm_ensemble_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
     lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
@@ -338,6 +341,12 @@ class ScientificPottsIntegrator:
         order = np.zeros(max(count.value, 1), dtype=np.uint32)
         self._check(self.lib.cpm_schedule(self._ctx, C.byref(count), order.ctypes.data))
         return count.value, order[:count.value]
+
+    def kernel_config(self) -> int:
+        """Tile shape id | proposal-count kernels << 4 | row propose kernel << 5."""
+        v = C.c_int32()
+        self._check(self.lib.cpm_kernel_config(self._ctx, C.byref(v)))
+        return int(v.value)
 
     def site_colors(self) -> np.ndarray:
         out = np.zeros(int(np.prod(self.spec.dims)), dtype=np.uint8)

@@ -14,7 +14,7 @@ namespace cpm {
 
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kCommitTiles = 4;    // tile segments per claim-check workgroup / apply region
+constexpr int kCommitTiles = 4;    // tile segments per commit workgroup (claim check, evaluation)
 constexpr int kMaxRounds = 256;    // colour / round order capacity per replica
 constexpr int kSlotFields = 8;     // per-tile accounting fields
 
@@ -155,7 +155,6 @@ struct PassArgs {
     unsigned long long *claims; // two buffers of total_claims each
     Entry *seg;                 // per-tile segments of seg_cap entries
     uint32_t *seg_count;        // [tile][kWaves] stretch lengths (kIdleTile: not running)
-    uint32_t *win_count;        // winners packed per check workgroup
     uint32_t *slots;            // [pass][tile][kSlotFields]
     uint32_t ntiles;
     uint32_t seg_cap;

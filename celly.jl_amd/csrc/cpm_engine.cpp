@@ -165,6 +165,10 @@ bool has_term(const cpm_model_desc &d, int term) {
 // Process-wide tile-shape preference per (algorithm, dimension, halo); -1 = default shape.
 std::atomic<int> g_shape_pref[3][4][3];
 
+// Process-wide switch of the row kernels (2D periodic von Neumann checkerboards with a
+// 2 x 2 colour pattern); on by default.
+std::atomic<int> g_row_kernels{1};
+
 int default_shape(int ndim, int halo) { return ndim == 2 ? (halo == 1 ? 0 : 1) : (halo == 1 ? 2 : 3); }
 
 int tile_shape_for(int algorithm, int ndim, int halo) {
@@ -562,6 +566,19 @@ void compile_replica(Replica &r) {
             }
         }
     }
+    // row kernel: 2D, periodic and obstacle-free, canonical von Neumann proposal, one
+    // residue class of a 2 x 2 pattern per colour, whole tiles
+    if (g_row_kernels.load() && d.algorithm == CPM_ALG_CHECKERBOARD && simple && d.ndim == 2 && v.pat_fast &&
+        (r.config & 16) == 0 && v.pat[0] == 2 && v.pat[1] == 2 && v.pat[2] == 1 && r.rounds == 4 &&
+        r.desc.dims[0] % tx == 0 && r.desc.dims[1] % ty == 0 &&
+        (shape == 0 || shape == 1 || (shape >= 4 && shape <= 7))) {
+        static const int vn[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
+        bool ok = d.proposal.count == 4;
+        for (int k = 0; ok && k < 4; ++k)
+            ok = d.proposal.offsets[k][0] == vn[k][0] && d.proposal.offsets[k][1] == vn[k][1];
+        for (int c = 1; ok && c <= 4; ++c) ok = v.col_count[c] == 1;
+        if (ok) r.config |= 32;
+    }
     v.halo = r.halo;
     {
         const int64_t X = r.desc.dims[0], XY = (int64_t)r.desc.dims[0] * r.desc.dims[1];
@@ -739,7 +756,7 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + 2 * n /* colour lists or degree table */ +
                          tiles * (rounds + 1) * 4 /* colour ranges */ +
                          cap1 * (4 + 8 + 4 + 1 + 8 * 4) + CPM_MAX_MEDIA * 4;
-    const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + 4 + sizeof(cpm::TileInfo) +
+    const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + sizeof(cpm::TileInfo) +
                                      rounds * cpm::kSlotFields * 4) +
                             2 * cap1 * 8;
     return own + arenas + sizeof(cpm::RepDev) + sizeof(cpm::RepDyn);
@@ -758,7 +775,6 @@ struct Group {
     unsigned long long *d_claims = nullptr;
     cpm::Entry *d_seg = nullptr;
     uint32_t *d_seg_count = nullptr;
-    uint32_t *d_win_count = nullptr;
     uint32_t *d_slots = nullptr;
     size_t cap_tiles = 0, cap_claims = 0, cap_slots = 0, cap_seg = 0;
     uint64_t bytes = 0;
@@ -768,9 +784,9 @@ struct Group {
 
     void release() {
         dev_free(d_tiles); dev_free(d_claims); dev_free(d_seg); dev_free(d_seg_count);
-        dev_free(d_win_count); dev_free(d_slots);
+        dev_free(d_slots);
         d_tiles = nullptr; d_claims = nullptr; d_seg = nullptr; d_seg_count = nullptr;
-        d_win_count = nullptr; d_slots = nullptr;
+        d_slots = nullptr;
         cap_tiles = cap_claims = cap_slots = cap_seg = 0;
         bytes = 0;
     }
@@ -1024,7 +1040,6 @@ struct Engine {
             g.release();
             g.d_tiles = dev_alloc<cpm::TileInfo>(need_tiles, g.bytes);
             g.d_seg_count = dev_alloc<uint32_t>(need_tiles * cpm::kWaves, g.bytes);
-            g.d_win_count = dev_alloc<uint32_t>(need_tiles / cpm::kCommitTiles + 1, g.bytes);
             g.d_claims = dev_alloc<unsigned long long>(2 * (size_t)g.total_claims, g.bytes);
             g.d_slots = dev_alloc<uint32_t>(need_slots, g.bytes);
             g.d_seg = dev_alloc<cpm::Entry>(need_seg, g.bytes);
@@ -1060,7 +1075,6 @@ struct Engine {
         a.claims = g.d_claims;
         a.seg = g.d_seg;
         a.seg_count = g.d_seg_count;
-        a.win_count = g.d_win_count;
         a.slots = g.d_slots;
         a.ntiles = g.ntiles;
         a.seg_cap = g.seg_cap;
@@ -1512,6 +1526,12 @@ int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors) {
     return CPM_OK;
 }
 
+int32_t cpm_kernel_config(const cpm_ctx *ctx, int32_t *config) {
+    if (!ctx || !config) return CPM_E_INVALID;
+    *config = ctx->engine.reps[ctx->slot]->config;
+    return CPM_OK;
+}
+
 int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]) {
     if (!ctx || !out) return CPM_E_INVALID;
     std::memcpy(out, ctx->engine.metrics, sizeof(ctx->engine.metrics));
@@ -1566,6 +1586,11 @@ int32_t cpm_set_tile_shape(int32_t algorithm, int32_t ndim, int32_t halo, int32_
             return CPM_E_INVALID;
     }
     g_shape_pref[algorithm][ndim][halo].store(shape >= 0 ? shape + 1 : 0);
+    return CPM_OK;
+}
+
+int32_t cpm_set_row_kernels(int32_t enable) {
+    g_row_kernels.store(enable ? 1 : 0);
     return CPM_OK;
 }
 

@@ -1,13 +1,15 @@
 // HIP kernels of the MI355X CPM engine (gfx950).
 //
 // One colour pass (checkerboard) or lottery round is two launches:
-//   propose  : LDS-staged lattice tile + halo; schedule filter (colour or lottery
-//              local-maximum); interior sites whose proposal neighbours all share
-//              the recipient's owner are settled as same-owner no-ops without RNG;
-//              the rest are compacted into an LDS work list (wave ballot) and get
-//              direction / priority Philox draws; actionable attempts publish a 64-bit
-//              claim key (priority << 32 | ~recipient) to both cells with one atomic
-//              max and are staged in a per-tile segment.
+//   propose  : LDS-staged lattice tile + halo (or, for 2D periodic von Neumann
+//              checkerboards with a 2 x 2 colour pattern, rows read straight into
+//              registers); schedule filter (colour or lottery local-maximum); interior
+//              sites whose proposal neighbours all share the recipient's owner are
+//              settled as same-owner no-ops without RNG; the rest are compacted into an
+//              LDS work list (wave ballot) and get direction / priority Philox draws;
+//              actionable attempts publish a 64-bit claim key (priority << 32 |
+//              ~recipient) to both cells with one atomic max and are staged in a
+//              per-tile segment.
 //   commit   : per-tile segment; an attempt wins iff it holds the key of both of its
 //              cells (equivalent to the reference's max-priority then min-identity
 //              claims, checkerboard.jl:281-369); winners evaluate the energy change
@@ -574,6 +576,184 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Checkerboard colour pass on a 2D periodic obstacle-free lattice whose proposal relation
+// is the von Neumann shell in canonical order (-1,0), (0,-1), (0,1), (1,0) and whose
+// colouring is a 2 x 2 pattern, one residue class per colour (checkerboard.jl:244-304).
+// No LDS tile and no barrier before the end: every wave reads its rows of the tile
+// straight into registers (16-byte loads of the colour row and the rows above and below,
+// plus the one x neighbour outside the 16 bytes), settles interior sites, and compacts
+// boundary sites and then actionable attempts in wave-private LDS lists so that the
+// Philox draws run on full waves. Outputs (claims, segment stretches, counters) are those
+// of cb_propose_kernel with NT = 256.
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+struct RowGeom {
+    static constexpr int LPR = TX / 4;               // lanes per lattice row
+    static constexpr int RPI = 64 / LPR;             // rows per wave instruction
+    static constexpr int BAND = TY / kWaves;         // tile rows of one wave
+    static constexpr int CROWS = BAND / 2;           // rows of the colour in the band
+    static constexpr int ITERS = (CROWS + RPI - 1) / RPI;
+    static constexpr int CAP = 2 * 64 * ITERS;       // colour sites of one wave
+    static constexpr int Q = TX * TY / kWaves;       // segment stretch capacity
+    static_assert(TX % 4 == 0 && 64 % LPR == 0 && TY % (2 * kWaves) == 0, "row tile shape");
+    static_assert(CAP <= Q, "stretch holds every colour site of the wave");
+};
+
+// One colour row strip: colour sites at x residues RX and RX + 2 of each 16-byte group.
+template <int RX, int TX, int TY>
+__device__ __forceinline__ void rows_collect(const CPM_GLOBAL int32_t *lat, int X, int Y, int x0, int y0,
+                                             int ry, int wave, uint32_t &ncand, uint32_t &nsites,
+                                             uint32_t &nsame, uint32_t *s_site, int32_t *s_own,
+                                             int32_t (*s_nb)[RowGeom<TX, TY>::CAP]) {
+    using R = RowGeom<TX, TY>;
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int it = 0; it < R::ITERS; ++it) {
+        const int k = it * R::RPI + lane / R::LPR;
+        const bool valid = k < R::CROWS;
+        const int gx = x0 + 4 * (lane % R::LPR);
+        const int gy = y0 + wave * R::BAND + 2 * k + ry;
+        v4i mid = v4i{0, 0, 0, 0}, dn = mid, up = mid;
+        int32_t side = 0;
+        if (valid) {
+            const int gym = gy == 0 ? Y - 1 : gy - 1;
+            const int gyp = gy + 1 == Y ? 0 : gy + 1;
+            const uint32_t row = (uint32_t)gy * (uint32_t)X;
+            mid = *(const CPM_GLOBAL v4i *)(lat + row + (uint32_t)gx);
+            dn = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gym * (uint32_t)X + (uint32_t)gx);
+            up = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gyp * (uint32_t)X + (uint32_t)gx);
+            const int sx = RX == 0 ? (gx == 0 ? X - 1 : gx - 1) : (gx + 4 == X ? 0 : gx + 4);
+            side = lat[row + (uint32_t)sx];
+        }
+        const uint32_t site0 = 1u + (uint32_t)gx + (uint32_t)gy * (uint32_t)X;
+        nsites += 2u * (uint32_t)__popcll(__ballot(valid));
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int p = RX + 2 * h;
+            const int32_t own = p == 0 ? mid.x : p == 1 ? mid.y : p == 2 ? mid.z : mid.w;
+            const int32_t left = p == 0 ? side : p == 1 ? mid.x : p == 2 ? mid.y : mid.z;
+            const int32_t right = p == 0 ? mid.y : p == 1 ? mid.z : p == 2 ? mid.w : side;
+            const int32_t below = p == 0 ? dn.x : p == 1 ? dn.y : p == 2 ? dn.z : dn.w;
+            const int32_t above = p == 0 ? up.x : p == 1 ? up.y : p == 2 ? up.z : up.w;
+            const bool interior = own == left && own == below && own == above && own == right;
+            const bool boundary = valid && !interior;
+            nsame += (uint32_t)__popcll(__ballot(valid && interior));
+            const uint64_t m = __ballot(boundary);
+            if (boundary) {
+                const uint32_t e = ncand + __popcll(m & __lanemask_lt());
+                s_site[e] = site0 + (uint32_t)p;
+                s_own[e] = own;
+                s_nb[0][e] = left;
+                s_nb[1][e] = below;
+                s_nb[2][e] = above;
+                s_nb[3][e] = right;
+            }
+            ncand += (uint32_t)__popcll(m);
+        }
+    }
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
+    using R = RowGeom<TX, TY>;
+    __shared__ uint32_t s_site[kWaves][R::CAP];
+    __shared__ int32_t s_own[kWaves][R::CAP];
+    __shared__ int32_t s_nb[kWaves][4][R::CAP];
+    __shared__ uint32_t s_cnt[kWaves][4];
+
+    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
+    const TileInfo ti = a.tiles[tile];
+    const uint32_t r = ti.rep;
+    const RepDev &d = a.reps[r];
+    const RepDyn &y = a.dyn[r];
+    if (!y.active || a.pass >= (uint32_t)d.rounds) {
+        if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
+        return;
+    }
+    const uint64_t mcs = y.mcs;
+    const uint32_t colour = y.order[a.pass];
+    const int res = d.col_res[d.col_first[colour]];
+    const int rx = d.pat_coord[res][0], ry = d.pat_coord[res][1];
+    const int X = d.dims[0], Y = d.dims[1];
+    const int wave = (int)(threadIdx.x >> 6);
+    const int lane = (int)__lane_id();
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+
+    // 1. colour sites of the wave's rows; interior sites are same-owner no-ops
+    uint32_t ncand = 0, nsites = 0, nsame = 0;
+    if (rx == 0)
+        rows_collect<0, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, ncand, nsites, nsame, s_site[wave],
+                                s_own[wave], s_nb[wave]);
+    else
+        rows_collect<1, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, ncand, nsites, nsame, s_site[wave],
+                                s_own[wave], s_nb[wave]);
+
+    // 2. direction draws of the boundary sites (ProposalDirectionStream); actionable
+    //    attempts are compacted in place (entry e is read before any write at or below it)
+    uint32_t na = 0;
+    for (uint32_t e0 = 0; e0 < ncand; e0 += 64) {
+        const uint32_t e = e0 + (uint32_t)lane;
+        bool act = false;
+        uint32_t site = 0;
+        int32_t own = 0, gain = 0;
+        if (e < ncand) {
+            site = s_site[wave][e];
+            own = s_own[wave][e];
+            const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity,
+                                              site, 0, d.prop_bound);
+            gain = s_nb[wave][dir & 3u][e];
+            act = gain != own;
+        }
+        const uint64_t valid = __ballot(e < ncand);
+        const uint64_t m = __ballot(act);
+        nsame += (uint32_t)__popcll(valid & ~m);
+        if (act) {
+            const uint32_t at = na + __popcll(m & __lanemask_lt());
+            s_site[wave][at] = site;
+            s_own[wave][at] = own;
+            s_nb[wave][0][at] = gain;
+        }
+        na += (uint32_t)__popcll(m);
+    }
+
+    // 3. priority draws and claims of the actionable attempts; stretch `wave` of the segment
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
+    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
+    for (uint32_t e = (uint32_t)lane; e < na; e += 64) {
+        const uint32_t site = s_site[wave][e];
+        const int32_t losing = s_own[wave][e];
+        const int32_t gaining = s_nb[wave][0][e];
+        const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
+                                        kSiteEntity, site, 0, 0).w0;
+        const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
+        if (losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)losing], key);
+        if (gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)gaining], key);
+        seg[e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
+    }
+    if (lane == 0) {
+        a.seg_count[tile * kWaves + wave] = na;
+        s_cnt[wave][0] = nsites;
+        s_cnt[wave][1] = nsame;
+        s_cnt[wave][2] = na;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+        uint32_t t0 = 0, t1 = 0, t3 = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            t0 += s_cnt[w][0];
+            t1 += s_cnt[w][1];
+            t3 += s_cnt[w][2];
+        }
+        slot[0] = t0;   // candidates
+        slot[1] = t1;   // same-owner no-ops
+        slot[2] = 0;    // boundary no-ops: none on a periodic obstacle-free lattice
+        slot[3] = t3;   // realized (actionable) attempts
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Lottery round, propose + claim (lottery.jl:153-256)
 // ---------------------------------------------------------------------------
 template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT>
@@ -792,9 +972,16 @@ __device__ __forceinline__ int32_t neighbour_word(const D &d, int x, int y, int 
 // keeps the reference's operation order.
 constexpr int kNb = 8;
 
+// Tracker values of a winner's cells at the pass snapshot and the boundary changes of its
+// copy: the commit writes snapshot + change (no other winner touches these cells).
+struct Staged {
+    int32_t dl, dg;    // boundary-measure changes of the losing and gaining cells
+    int32_t vl, vg;    // snapshot volumes (cell owners only)
+    int64_t bl, bg;    // snapshot boundary measures (cell owners only)
+};
+
 template <class D>
-__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub,
-                                       int32_t *dl_out, int32_t *dg_out) {
+__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub, Staged *st) {
     const int32_t l = e.losing, g = e.gaining;
     const uint32_t idx0 = e.site - 1;
     const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
@@ -841,8 +1028,12 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
     for (int u = 0; u < kNb; ++u) surface_word(sw[u]);
     for (int k = kNb; k < ns; ++k)
         surface_word(k < 32 ? nb(d.surf.off[k], d.surf_lin[k]) : neighbour_word(d, x, y, z, d.surf.off[k]));
-    *dl_out = dl;
-    *dg_out = dg;
+    st->dl = dl;
+    st->dg = dg;
+    st->vl = vl;
+    st->vg = vg;
+    st->bl = bm_l;
+    st->bg = bm_g;
 
     double acc = 0.0;
     bool acc64 = false;
@@ -936,13 +1127,13 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
 // Commit of one accepted winning attempt: ownership, volumes, boundary measures,
 // retirement, medium volumes (checkerboard.jl:410-440).
 template <class D>
-__device__ inline void commit_attempt(const D &d, const Entry &e, int32_t dl, int32_t dg) {
+__device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &st) {
     const int32_t l = e.losing, g = e.gaining;
     gp(d.lattice)[e.site - 1] = g;
     if (l > 0) {
-        const int32_t nv = gp(d.vol)[l] - 1;
+        const int32_t nv = st.vl - 1;
         gp(d.vol)[l] = nv;
-        gp(d.bnd)[l] += dl;
+        gp(d.bnd)[l] = st.bl + st.dl;
         if (nv == 0) {
             gp(d.active)[l] = 0;
             gp(d.ctype)[l] = 0;
@@ -955,167 +1146,156 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, int32_t dl, in
         atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
     }
     if (g > 0) {
-        gp(d.vol)[g] += 1;
-        gp(d.bnd)[g] += dg;
+        gp(d.vol)[g] = st.vg + 1;
+        gp(d.bnd)[g] = st.bg + st.dg;
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
     }
 }
 
 
-// Claim check of one pass (checkerboard.jl:337-369). Workgroup b owns the segments of
-// tiles [8b, 8b + 8): an attempt wins iff its key is the maximum on both of its cells.
-// Winners are packed in place at the start of the workgroup's segment region as records
-// {site, tile within the region, losing, gaining} and counted in win_count[b]; every
-// packed position precedes every entry not yet read, so packing never overwrites input.
-__global__ __launch_bounds__(kBlock) void check_kernel(PassArgs a) {
-    // the workgroup's segments: kWaves stretches per tile, stretch w of a tile holding
-    // entries [w * Q, w * Q + count) with Q = seg_cap / kWaves
+// Claim check, evaluation, acceptance and commit of one pass (checkerboard.jl:337-457) in
+// one wave per region of kCommitTiles tile segments. An attempt wins iff its key is the
+// maximum on both of its cells. Entries are read in batches of kCheckBatch chunks whose
+// entry loads, then claim loads, are all in flight together; winners are compacted into
+// LDS and evaluated against the pass snapshot in full chunks, and commit at once (winners
+// touch disjoint cells and non-adjacent sites). The workgroup also clears a slice of the
+// next pass's claim buffer. The tiles of a replica are contiguous, so a region whose
+// first and last tile share a replica belongs to that replica alone (one round per chunk);
+// otherwise one round per tile. The replica index is uniform in every round, so its
+// descriptor is read through the scalar cache.
+constexpr int kCheckBatch = 4;
+template <int NWV>
+__global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
+    constexpr int NT = 64 * NWV;
     constexpr int NS = kCommitTiles * kWaves;
+    constexpr int WCAP = kCheckBatch * NT;
     static_assert((NS & (NS - 1)) == 0 && NS <= 64, "stretch search halves a power of two");
     __shared__ uint32_t s_base[NS + 1];
-    __shared__ uint32_t s_conf[kCommitTiles];
     __shared__ uint32_t s_cbase[kCommitTiles];
-    __shared__ uint32_t s_live[kCommitTiles];
-    __shared__ uint32_t s_nwin;
+    __shared__ uint32_t s_cnt[kCommitTiles][3];  // conflicts, rejections, acceptances
+    __shared__ v4i s_win[WCAP];
+    __shared__ uint32_t s_nw;
     const uint32_t nwg = gridDim.x;
     const uint32_t b = xcd_tile(blockIdx.x, nwg);
-    // clear the claim buffer of the next pass (its previous reader finished earlier)
+    const uint32_t lane = __lane_id();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
     {
         unsigned long long *next = a.claims + (size_t)(a.parity ^ 1u) * a.total_claims;
         const uint32_t per = (a.total_claims + nwg - 1) / nwg;
         const uint32_t lo = b * per;
         const uint32_t hi = min(lo + per, a.total_claims);
-        for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) next[j] = 0ull;
+        for (uint32_t j = lo + tid; j < hi; j += NT) next[j] = 0ull;
     }
-    const uint32_t Q = a.seg_cap / kWaves;
-    if (threadIdx.x < 64) {
-        // wave 0: stretch counts (idle tiles count zero) and their inclusive scan
-        const int k = threadIdx.x;
-        const uint32_t tile = b * kCommitTiles + k / kWaves;
-        uint32_t n = 0;
-        bool live = false;
-        if (k < NS && tile < a.ntiles) {
-            const uint32_t c = a.seg_count[tile * kWaves + k % kWaves];
-            live = c != kIdleTile;
-            n = live ? c : 0u;
-        }
-        if (k < kCommitTiles && b * kCommitTiles + k < a.ntiles)
-            s_cbase[k] = a.tiles[b * kCommitTiles + k].claim_base;
-#pragma unroll
-        for (int o = 1; o < NS; o <<= 1) {
-            const uint32_t t = __shfl_up(n, o);
-            if (k >= o) n += t;
-        }
-        if (k < NS) s_base[k + 1] = n;
-        if (k == 0) {
-            s_base[0] = 0;
-            s_nwin = 0;
-        }
-        const uint64_t lm = __ballot(live && k % kWaves == 0);  // stretch 0 carries the flag
-        if (k < kCommitTiles) {
-            s_live[k] = (uint32_t)((lm >> (k * kWaves)) & 1u);
-            s_conf[k] = 0;
-        }
-    }
-    __syncthreads();
-    const uint32_t total = s_base[NS];
-    const unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    CPM_GLOBAL v4i *region = (CPM_GLOBAL v4i *)(a.seg + (size_t)b * kCommitTiles * a.seg_cap);
-    for (uint32_t e0 = 0; e0 < total; e0 += kBlock) {
-        const uint32_t e = e0 + threadIdx.x;
-        bool win = false;
-        v4i rec = v4i{0, 0, 0, 0};
-        if (e < total) {
-            // stretch holding entry e: largest k with s_base[k] <= e (binary search)
-            int k = 0;
-#pragma unroll
-            for (int step = NS / 2; step > 0; step >>= 1)
-                if (e >= s_base[k + step]) k += step;
-            const int kt = k / kWaves;
-            const uint32_t i = (uint32_t)kt * a.seg_cap + (uint32_t)(k % kWaves) * Q + (e - s_base[k]);
-            const v4i v = region[i];
-            const uint32_t cb = s_cbase[kt];
-            const unsigned long long key = ((unsigned long long)(uint32_t)v.y << 32) | (uint32_t)(~(uint32_t)v.x);
-            win = true;
-            if (v.z > 0) win = claims[cb + (uint32_t)v.z] == key;
-            if (v.w > 0) win = win && claims[cb + (uint32_t)v.w] == key;
-            if (!win) atomicAdd(&s_conf[kt], 1u);
-            rec = v4i{v.x, kt, v.z, v.w};
-        }
-        // every lane has read its entry before any winner of this round is written
-        __syncthreads();
-        const uint64_t m = __ballot(win);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(&s_nwin, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (win) region[base + __popcll(m & __lanemask_lt())] = rec;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) a.win_count[b] = s_nwin;
-    if (threadIdx.x < kCommitTiles) {
-        const int k = threadIdx.x;
-        const uint32_t tile = b * kCommitTiles + k;
-        if (tile < a.ntiles && s_live[k]) {
-            uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-            slot[4] = s_conf[k];
-            slot[5] = 0;
-            slot[6] = 0;
-        }
-    }
-}
-
-// Energy change, acceptance and commit of the winners of one segment region
-// (checkerboard.jl:371-457); one wave per region, so every region's latency chain runs
-// at once. The tiles of a replica are contiguous, so a region whose first and last tile
-// share a replica belongs to that replica alone (one round below, normally); otherwise one
-// round per tile. The replica index is uniform in every round, so its descriptor is read
-// through the scalar cache (constant address space: descriptors never change while a
-// kernel runs).
-__global__ __launch_bounds__(64) void apply_kernel(PassArgs a) {
-    __shared__ uint32_t s_cnt[kCommitTiles][2];
-    const uint32_t b = blockIdx.x;
-    const uint32_t nwin = a.win_count[b];
-    if (nwin == 0) return;
-    if (threadIdx.x < kCommitTiles) s_cnt[threadIdx.x][0] = s_cnt[threadIdx.x][1] = 0;
-    __syncthreads();
     const uint32_t t0 = b * kCommitTiles;
     const uint32_t nt = min((uint32_t)kCommitTiles, a.ntiles - t0);
+    // stretch counts (idle tiles count zero) and their inclusive scan
+    uint32_t n = 0;
+    bool live = false;
+    if (tid < (uint32_t)NS && lane / kWaves < nt) {
+        const uint32_t c = a.seg_count[t0 * kWaves + lane];
+        live = c != kIdleTile;
+        n = live ? c : 0u;
+    }
+#pragma unroll
+    for (int o = 1; o < NS; o <<= 1) {
+        const uint32_t t = __shfl_up(n, o);
+        if ((int)lane >= o) n += t;
+    }
+    if (tid < (uint32_t)NS) s_base[tid + 1] = n;
+    if (tid == 0) {
+        s_base[0] = 0;
+        s_nw = 0;
+    }
+    const uint64_t live_tiles = __ballot(live && lane % kWaves == 0);  // wave 0: stretch 0 carries the flag
+    if (tid < nt) s_cbase[tid] = a.tiles[t0 + tid].claim_base;
+    if (tid < (uint32_t)kCommitTiles * 3) s_cnt[tid / 3][tid % 3] = 0;
+    __syncthreads();
+    const uint32_t total = s_base[NS];
+    const uint32_t Q = a.seg_cap / kWaves;
+    const unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
+    const CPM_GLOBAL v4i *region = (const CPM_GLOBAL v4i *)(a.seg + (size_t)t0 * a.seg_cap);
     const uint32_t r_first = a.tiles[t0].rep;
     const bool one = a.tiles[t0 + nt - 1].rep == r_first;
-    const CPM_GLOBAL v4i *region = (const CPM_GLOBAL v4i *)(a.seg + (size_t)t0 * a.seg_cap);
-    for (uint32_t e0 = 0; e0 < nwin; e0 += 64) {
-        const uint32_t e = e0 + threadIdx.x;
-        v4i v = v4i{0, -1, 0, 0};
-        if (e < nwin) v = region[e];
-        const int kt = v.y;  // -1 for lanes without a winner
-        for (uint32_t k = 0; k < nt; ++k) {
-            const uint32_t r = one ? r_first : a.tiles[t0 + k].rep;
-            if (e < nwin && (one || kt == (int)k)) {
-                const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
-                const RepDyn &y = a.dyn[r];
-                const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
-                const Entry ent{(uint32_t)v.x, 0u, v.z, v.w};
-                int32_t dl, dg;
-                const bool accept = evaluate_winner(d, ent, y.mcs, sub, &dl, &dg);
-                if (accept) commit_attempt(d, ent, dl, dg);
-                atomicAdd(&s_cnt[kt][accept ? 1 : 0], 1u);
+    for (uint32_t blk = 0; blk < total; blk += WCAP) {
+        // entries of the batch, then their claims
+        v4i v[kCheckBatch];
+        uint32_t kt[kCheckBatch];
+#pragma unroll
+        for (int j = 0; j < kCheckBatch; ++j) {
+            const uint32_t e = blk + (uint32_t)j * NT + tid;
+            v[j] = v4i{0, 0, 0, 0};
+            kt[j] = 0;
+            if (e < total) {
+                int k = 0;
+#pragma unroll
+                for (int step = NS / 2; step > 0; step >>= 1)
+                    if (e >= s_base[k + step]) k += step;
+                kt[j] = (uint32_t)k / kWaves;
+                v[j] = region[kt[j] * a.seg_cap + (uint32_t)(k % kWaves) * Q + (e - s_base[k])];
             }
-            if (one) break;
         }
+        unsigned long long cl[kCheckBatch], cg[kCheckBatch];
+#pragma unroll
+        for (int j = 0; j < kCheckBatch; ++j) {
+            const uint32_t e = blk + (uint32_t)j * NT + tid;
+            const uint32_t cb = s_cbase[kt[j]];
+            cl[j] = cg[j] = 0ull;
+            if (e < total && v[j].z > 0) cl[j] = claims[cb + (uint32_t)v[j].z];
+            if (e < total && v[j].w > 0) cg[j] = claims[cb + (uint32_t)v[j].w];
+        }
+#pragma unroll
+        for (int j = 0; j < kCheckBatch; ++j) {
+            const uint32_t e = blk + (uint32_t)j * NT + tid;
+            bool win = false;
+            if (e < total) {
+                const unsigned long long key =
+                    ((unsigned long long)(uint32_t)v[j].y << 32) | (uint32_t)(~(uint32_t)v[j].x);
+                win = (v[j].z <= 0 || cl[j] == key) && (v[j].w <= 0 || cg[j] == key);
+                if (!win) atomicAdd(&s_cnt[kt[j]][0], 1u);
+            }
+            const uint64_t m = __ballot(win);
+            if (m) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&s_nw, (uint32_t)__popcll(m));
+                base = __shfl(base, 0);
+                if (win) s_win[base + __popcll(m & __lanemask_lt())] = v4i{v[j].x, (int32_t)kt[j], v[j].z, v[j].w};
+            }
+        }
+        __syncthreads();
+        const uint32_t nw = s_nw;
+        // winners in full chunks (wave w takes chunks w, w + NWV, ...): energy change,
+        // acceptance and commit
+        for (uint32_t w0 = wave * 64; w0 < nw; w0 += NT) {
+            const uint32_t w = w0 + lane;
+            v4i rec = v4i{0, -1, 0, 0};
+            if (w < nw) rec = s_win[w];
+            const int kw = rec.y;  // -1 for lanes without a winner
+            for (uint32_t k = 0; k < nt; ++k) {
+                const uint32_t r = one ? r_first : a.tiles[t0 + k].rep;
+                if (w < nw && (one || kw == (int)k)) {
+                    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
+                    const RepDyn &y = a.dyn[r];
+                    const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
+                    const Entry ent{(uint32_t)rec.x, 0u, rec.z, rec.w};
+                    Staged st;
+                    const bool accept = evaluate_winner(d, ent, y.mcs, sub, &st);
+                    if (accept) commit_attempt(d, ent, st);
+                    atomicAdd(&s_cnt[kw][accept ? 2 : 1], 1u);
+                }
+                if (one) break;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_nw = 0;
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < kCommitTiles) {
-        const int k = threadIdx.x;
-        const uint32_t c0 = s_cnt[k][0], c1 = s_cnt[k][1];
-        if (c0 + c1) {
-            uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + t0 + k) * kSlotFields;
-            slot[5] = c0;
-            slot[6] = c1;
-        }
+    if (tid < nt && ((live_tiles >> (lane * kWaves)) & 1u)) {
+        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + t0 + lane) * kSlotFields;
+        slot[4] = s_cnt[lane][0];
+        slot[5] = s_cnt[lane][1];
+        slot[6] = s_cnt[lane][2];
     }
 }
 
@@ -1197,21 +1377,20 @@ static hipError_t launch_propose_t(int algorithm, int generic, const PassArgs &a
     return hipGetLastError();
 }
 
+constexpr int kCommitWaves = 2;  // claim check on both waves, winners evaluated in full chunks
 static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
     const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
-    hipLaunchKernelGGL(check_kernel, dim3(wgs), dim3(kBlock), 0, stream, a);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(apply_kernel, dim3(wgs), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((commit_kernel<kCommitWaves>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
     return hipGetLastError();
 }
 
 }  // namespace cpm
 
 // Tile configurations: id & 15 -> (dim, halo, TX, TY, TZ); bit 4 of id selects the kernels
-// for proposal relations whose direction count differs from 2 * dim.
+// for proposal relations whose direction count differs from 2 * dim; bit 5 the row kernel
+// of 2D periodic von Neumann checkerboards with a 2 x 2 colour pattern.
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz) {
-    if (id < 0 || id > 31) return 0;
+    if (id < 0 || id > 63) return 0;
     id &= 15;
     static const int table[][5] = {
         {2, 1, 128, 16, 1}, {2, 2, 128, 16, 1}, {3, 1, 32, 8, 8}, {3, 2, 32, 8, 8},
@@ -1233,7 +1412,21 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream) {
     using namespace cpm;
-    if (config < 0 || config > 31) return hipErrorInvalidValue;
+    if (config < 0 || config > 63) return hipErrorInvalidValue;
+    if (config & 32) {
+        // row kernels: checkerboard, 2D periodic von Neumann proposal, 2 x 2 colour pattern
+        if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16)) return hipErrorInvalidValue;
+        switch (config & 15) {
+        case 0:
+        case 1: hipLaunchKernelGGL((cb_propose_rows_kernel<128, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 4: hipLaunchKernelGGL((cb_propose_rows_kernel<64, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 5: hipLaunchKernelGGL((cb_propose_rows_kernel<256, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 6: hipLaunchKernelGGL((cb_propose_rows_kernel<64, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 7: hipLaunchKernelGGL((cb_propose_rows_kernel<32, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const int g = (config >> 4) & 1;
     switch (config & 15) {
     case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, g, *a, stream);

@@ -151,6 +151,10 @@ int32_t cpm_sync(cpm_ctx *ctx);
 int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order);
 /* Realized colour of every site (0 for immutable sites), N bytes. Checkerboard only. */
 int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors);
+/* Kernel configuration of the context's lattice: tile shape id (bits 0-3), kernels for
+ * proposal relations of other than 2 * ndim directions (bit 4), row propose kernel (bit 5).
+ * (No counterpart in the reference: kernel selection of this engine.) */
+int32_t cpm_kernel_config(const cpm_ctx *ctx, int32_t *config);
 
 /* ExecutionMetrics mirror: [launches, host_synchronizations, h2d_transfers,
  * d2h_transfers, device_bytes_allocated, graph_launches]. */
@@ -171,6 +175,11 @@ int32_t cpm_tile_shape_info(int32_t shape, int32_t out[5]);
  * the call; shape -1 restores the default. The shape's dim and halo must match. Results do
  * not depend on the shape. (No counterpart in the reference.) */
 int32_t cpm_set_tile_shape(int32_t algorithm, int32_t ndim, int32_t halo, int32_t shape);
+/* Process-wide switch (default on) of the row propose kernels, used for replicas created
+ * after the call whose model is a 2D periodic obstacle-free checkerboard with the canonical
+ * von Neumann proposal relation and a 2 x 2 colour pattern. Results do not depend on it.
+ * (No counterpart in the reference: kernel selection of this engine.) */
+int32_t cpm_set_row_kernels(int32_t enable);
 
 /* Device-to-device copy of `bytes` bytes (a multiple of 16) with a plain 16-byte-per-lane
  * kernel, launched once untimed and `reps` times timed; *out_ms = mean launch time. Errors

@@ -3,7 +3,7 @@ launch geometry): 2D checkerboard and lottery on odd and even sides, and 3D floa
 import numpy as np
 import pytest
 
-from corepotts import init_scientific, set_tile_shape, tile_shapes, workloads
+from corepotts import init_scientific, set_row_kernels, set_tile_shape, tile_shapes, workloads
 from corepotts.spec import ALG_CHECKERBOARD, ALG_LOTTERY
 from helpers import oracle_for
 
@@ -13,9 +13,11 @@ FIELDS = ("tags", "ids", "finite_volumes", "medium_volumes", "boundary_measures"
           "cell_types")
 
 
-def _parity(model, mcs):
+def _parity(model, mcs, rows=None):
     compiled, proposal, components, alg, seed = model
     gpu = init_scientific(compiled, proposal, components, alg, seed=seed)
+    if rows is not None:
+        assert bool(gpu.kernel_config() & 32) == rows
     sim, _ = oracle_for(compiled, proposal, components, alg, seed)
     gpu.step(mcs)
     sim.step(mcs)
@@ -36,6 +38,55 @@ def test_2d_shapes(gpu_lib, shape, alg):
             _parity(workloads.scaled_2d(L, algorithm=alg, seed=4242 + L), 4)
     finally:
         set_tile_shape(alg_id, 2, 1, -1)
+
+
+def _rectangle(X, Y, seed):
+    """X x Y periodic lattice of 10 x 10 cells (volume, Moore contact, boundary), medium gaps."""
+    from corepotts import (BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain, CheckerboardSweepCPM,
+                           ContactRole, LogicalPottsState, MediumTypeTable, ProposalRole,
+                           QuadraticBoundaryHamiltonian, QuadraticVolumeHamiltonian, ScientificComponentSet,
+                           SurfaceRole, UnorderedContactHamiltonian, compile_scientific_state,
+                           first_shell_relation, moore_relation, required_properties)
+    owners = np.full((X, Y), -1, dtype=np.int64)
+    types, cid = {}, 0
+    for y0 in range(0, Y - 9, 12):
+        for x0 in range(0, X - 9, 12):
+            cid += 1
+            owners[x0:x0 + 10, y0:y0 + 10] = cid
+            types[cid] = 2 + cid % 2
+    surface = first_shell_relation(SurfaceRole, 2)
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    contact = UnorderedContactHamiltonian(np.asarray(workloads.J_TWO_TYPES, np.float32), MediumTypeTable({1: 1}),
+                                          moore_relation(ContactRole, 2))
+    boundary = QuadraticBoundaryHamiltonian(BoundaryEdgeCount(), surface, number_type=np.float32)
+    schema = {**required_properties(volume), **required_properties(boundary)}
+    state = LogicalPottsState(owners, cid, cell_types=types, medium_domains=(1,), property_schema=schema)
+    state.property_values("target_volume")[:] = np.float32(100)
+    state.property_values("volume_strength")[:] = np.float32(1)
+    state.property_values("target_boundary")[:] = 40
+    state.property_values("boundary_strength")[:] = np.float32(1)
+    compiled = compile_scientific_state(state, CartesianDomain((X, Y)), BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
+    return (compiled, first_shell_relation(ProposalRole, 2), ScientificComponentSet(energies=(volume, contact, boundary)),
+            CheckerboardSweepCPM(np.float32(10.0)), seed)
+
+
+ROW_SHAPES = [s for s, t in tile_shapes(2, 1) if t[0] * t[1] >= 1024]
+
+
+@pytest.mark.parametrize("shape", ROW_SHAPES)
+@pytest.mark.parametrize("rows", [True, False], ids=["rows", "tiles"])
+def test_2d_row_kernels(gpu_lib, shape, rows):
+    """Lattices whose sides are whole tiles of every shape: the row propose kernel (periodic,
+    von Neumann proposal, Moore contact: a 2 x 2 colour pattern) against the oracle, and the
+    tile kernel on the same models."""
+    set_tile_shape(ALG_CHECKERBOARD, 2, 1, shape)
+    set_row_kernels(rows)
+    try:
+        _parity(workloads.scaled_2d(256, seed=9001), 5, rows)
+        _parity(_rectangle(512, 64, seed=9002), 4, rows)
+    finally:
+        set_tile_shape(ALG_CHECKERBOARD, 2, 1, -1)
+        set_row_kernels(True)
 
 
 @pytest.mark.parametrize("shape", [s for s, _ in tile_shapes(3, 1)])
