@@ -6,6 +6,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstddef>
 #include <stdint.h>
 
 #include "../../include/cpm_mi355x.h"
@@ -52,6 +54,26 @@ struct RelDev {
 };
 
 // Static description of one replica (lattice, model, schedule and buffers).
+// Per-cell state in one 64-byte record, so a copy attempt reads each of its two cells
+// with one cache line: trackers (volume, boundary measure), type, activity and the
+// property columns of the volume and boundary energies. The host converts to and from the
+// reference's per-cell arrays at upload and download.
+struct __attribute__((aligned(64))) CellRec {
+    int32_t vol;       // finite volume
+    uint32_t ctype;    // cell type (0 when inactive)
+    int64_t bnd;       // boundary measure
+    double tv;         // target_volume
+    double vs;         // volume_strength
+    int64_t tb;        // target_boundary
+    double bs;         // boundary_strength
+    uint8_t active;
+    uint8_t pad[15];
+};
+static_assert(sizeof(CellRec) == 64, "one cache line per cell");
+static_assert(offsetof(CellRec, bnd) == 8 && offsetof(CellRec, tv) == 16 && offsetof(CellRec, vs) == 24 &&
+                  offsetof(CellRec, tb) == 32 && offsetof(CellRec, bs) == 40 && offsetof(CellRec, active) == 48,
+              "load_cell reads the fields at these offsets");
+
 struct RepDev {
     int32_t ndim;
     int32_t dims[3];
@@ -107,12 +129,8 @@ struct RepDev {
     const uint16_t *colour_list; // colourings without pat_fast: tile positions by tile and colour
     const uint32_t *colour_off;  // [tile][rounds + 1] ranges into colour_list
     const uint8_t *site_degree;  // lottery, nullptr: uniform degree
-    int32_t *vol;                // cap + 1, index = cell id
-    int64_t *bnd;
-    uint32_t *ctype;
-    uint8_t *active;
-    double *tv, *vs, *bs;
-    int64_t *tb;
+    CellRec *cells;              // cap + 1, index = cell id (entry 0: spare read for media)
+    uint32_t *ctype;             // cap + 1 cell types (neighbour type lookups)
     int32_t *medium_vol;         // n_media
 };
 

@@ -126,13 +126,12 @@ struct Replica {
     cpm::RepDev dev{};
     // device buffers
     int32_t *d_lattice = nullptr;
-    uint8_t *d_mask = nullptr, *d_color = nullptr, *d_degree = nullptr, *d_active = nullptr;
+    uint8_t *d_mask = nullptr, *d_color = nullptr, *d_degree = nullptr;
     uint16_t *d_clist = nullptr;
     uint32_t *d_coff = nullptr;
-    int32_t *d_vol = nullptr, *d_mvol = nullptr;
-    int64_t *d_bnd = nullptr, *d_tb = nullptr;
+    int32_t *d_mvol = nullptr;
+    cpm::CellRec *d_cells = nullptr;
     uint32_t *d_ctype = nullptr;
-    double *d_tv = nullptr, *d_vs = nullptr, *d_bs = nullptr;
     uint64_t bytes = 0;
     uint64_t estimate = 0;      // estimate_replica_bytes at admission
     uint64_t base_mcs = 0;      // MCS counter when the state was last set
@@ -144,8 +143,7 @@ struct Replica {
     ~Replica() {
         dev_free(d_lattice); dev_free(d_mask); dev_free(d_color); dev_free(d_degree);
         dev_free(d_clist); dev_free(d_coff);
-        dev_free(d_active); dev_free(d_vol); dev_free(d_mvol); dev_free(d_bnd); dev_free(d_tb);
-        dev_free(d_ctype); dev_free(d_tv); dev_free(d_vs); dev_free(d_bs);
+        dev_free(d_cells); dev_free(d_ctype); dev_free(d_mvol);
     }
 };
 
@@ -571,7 +569,7 @@ void compile_replica(Replica &r) {
     if (g_row_kernels.load() && d.algorithm == CPM_ALG_CHECKERBOARD && simple && d.ndim == 2 && v.pat_fast &&
         (r.config & 16) == 0 && v.pat[0] == 2 && v.pat[1] == 2 && v.pat[2] == 1 && r.rounds == 4 &&
         r.desc.dims[0] % tx == 0 && r.desc.dims[1] % ty == 0 &&
-        (shape == 0 || shape == 1 || (shape >= 4 && shape <= 7))) {
+        (shape == 0 || shape == 1 || (shape >= 4 && shape <= 7) || shape == 12 || shape == 13)) {
         static const int vn[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
         bool ok = d.proposal.count == 4;
         for (int k = 0; ok && k < 4; ++k)
@@ -698,24 +696,12 @@ void allocate_replica(Replica &r, hipStream_t stream) {
         r.d_degree = dev_alloc<uint8_t>((size_t)r.n, r.bytes);
         hip_check(hipMemcpyAsync(r.d_degree, r.degrees.data(), (size_t)r.n, hipMemcpyHostToDevice, stream), "copy degrees");
     }
-    r.d_vol = dev_alloc<int32_t>(cap1, r.bytes);
-    r.d_bnd = dev_alloc<int64_t>(cap1, r.bytes);
+    r.d_cells = dev_alloc<cpm::CellRec>(cap1, r.bytes);
     r.d_ctype = dev_alloc<uint32_t>(cap1, r.bytes);
-    r.d_active = dev_alloc<uint8_t>(cap1, r.bytes);
-    r.d_tv = dev_alloc<double>(cap1, r.bytes);
-    r.d_vs = dev_alloc<double>(cap1, r.bytes);
-    r.d_tb = dev_alloc<int64_t>(cap1, r.bytes);
-    r.d_bs = dev_alloc<double>(cap1, r.bytes);
     r.d_mvol = dev_alloc<int32_t>(CPM_MAX_MEDIA, r.bytes);
     hip_check(hipMemsetAsync(r.d_lattice, 0, (size_t)r.n * 4, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_vol, 0, cap1 * 4, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_bnd, 0, cap1 * 8, stream), "memset");
+    hip_check(hipMemsetAsync(r.d_cells, 0, cap1 * sizeof(cpm::CellRec), stream), "memset");
     hip_check(hipMemsetAsync(r.d_ctype, 0, cap1 * 4, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_active, 0, cap1, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_tv, 0, cap1 * 8, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_vs, 0, cap1 * 8, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_tb, 0, cap1 * 8, stream), "memset");
-    hip_check(hipMemsetAsync(r.d_bs, 0, cap1 * 8, stream), "memset");
     hip_check(hipMemsetAsync(r.d_mvol, 0, CPM_MAX_MEDIA * 4, stream), "memset");
     cpm::RepDev &v = r.dev;
     v.lattice = r.d_lattice;
@@ -723,14 +709,8 @@ void allocate_replica(Replica &r, hipStream_t stream) {
     v.colour_list = r.d_clist;
     v.colour_off = r.d_coff;
     v.site_degree = r.d_degree;
-    v.vol = r.d_vol;
-    v.bnd = r.d_bnd;
+    v.cells = r.d_cells;
     v.ctype = r.d_ctype;
-    v.active = r.d_active;
-    v.tv = r.d_tv;
-    v.vs = r.d_vs;
-    v.tb = r.d_tb;
-    v.bs = r.d_bs;
     v.medium_vol = r.d_mvol;
 }
 
@@ -755,7 +735,7 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     const uint64_t cap1 = (uint64_t)d.capacity + 1;
     const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + 2 * n /* colour lists or degree table */ +
                          tiles * (rounds + 1) * 4 /* colour ranges */ +
-                         cap1 * (4 + 8 + 4 + 1 + 8 * 4) + CPM_MAX_MEDIA * 4;
+                         cap1 * (sizeof(cpm::CellRec) + 4) + CPM_MAX_MEDIA * 4;
     const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + sizeof(cpm::TileInfo) +
                                      rounds * cpm::kSlotFields * 4) +
                             2 * cap1 * 8;
@@ -1286,20 +1266,21 @@ struct Engine {
                 fail(CPM_E_INVALID, "owner tag must identify a cell or medium domain");
             }
         }
-        std::vector<int32_t> vol(cap1, 0);
-        std::vector<int64_t> bnd(cap1, 0), tbv(cap1, 0);
+        // per-cell arrays -> 64-byte cell records (entry 0 is the spare read for media)
+        std::vector<cpm::CellRec> cells(cap1);
+        std::memset(cells.data(), 0, cap1 * sizeof(cpm::CellRec));
         std::vector<uint32_t> ct(cap1, 0);
-        std::vector<uint8_t> act(cap1, 0);
-        std::vector<double> tvv(cap1, 0.0), vsv(cap1, 0.0), bsv(cap1, 0.0);
         for (size_t c = 0; c < cap; ++c) {
-            vol[c + 1] = fv[c];
-            bnd[c + 1] = bm[c];
+            cpm::CellRec &q = cells[c + 1];
+            q.vol = fv[c];
+            q.bnd = bm[c];
+            q.ctype = ctype[c];
+            q.active = active[c];
+            q.tv = tv ? tv[c] : 0.0;
+            q.vs = vs ? vs[c] : 0.0;
+            q.tb = tb ? tb[c] : 0;
+            q.bs = bs ? bs[c] : 0.0;
             ct[c + 1] = ctype[c];
-            act[c + 1] = active[c];
-            tvv[c + 1] = tv ? tv[c] : 0.0;
-            vsv[c + 1] = vs ? vs[c] : 0.0;
-            tbv[c + 1] = tb ? tb[c] : 0;
-            bsv[c + 1] = bs ? bs[c] : 0.0;
         }
         if (has_term(r->desc, CPM_TERM_CONTACT))
             for (size_t c = 0; c < cap; ++c)
@@ -1309,14 +1290,8 @@ struct Engine {
         for (int i = 0; i < r->desc.n_media; ++i) media[i] = mv ? mv[i] : 0;
         hip_check(hipStreamSynchronize(stream), "sync");
         hip_check(hipMemcpy(r->d_lattice, words.data(), n * 4, hipMemcpyHostToDevice), "upload lattice");
-        hip_check(hipMemcpy(r->d_vol, vol.data(), cap1 * 4, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_bnd, bnd.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
+        hip_check(hipMemcpy(r->d_cells, cells.data(), cap1 * sizeof(cpm::CellRec), hipMemcpyHostToDevice), "upload");
         hip_check(hipMemcpy(r->d_ctype, ct.data(), cap1 * 4, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_active, act.data(), cap1, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_tv, tvv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_vs, vsv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_tb, tbv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_bs, bsv.data(), cap1 * 8, hipMemcpyHostToDevice), "upload");
         hip_check(hipMemcpy(r->d_mvol, media.data(), CPM_MAX_MEDIA * 4, hipMemcpyHostToDevice), "upload");
         metrics[2]++;
         r->base_mcs = read_mcs(slot);
@@ -1339,20 +1314,23 @@ struct Engine {
                 if (ids) ids[i] = (uint32_t)(w > 0 ? w : -w);
             }
         }
-        auto get = [&](auto *out, const auto *dev, size_t elem) {
-            if (!out) return;
-            std::vector<uint8_t> tmp(cap1 * elem);
-            hip_check(hipMemcpy(tmp.data(), dev, cap1 * elem, hipMemcpyDeviceToHost), "download");
-            std::memcpy(out, tmp.data() + elem, cap * elem);
-        };
-        get(fv, r->d_vol, 4);
-        get(bm, r->d_bnd, 8);
-        get(active, r->d_active, 1);
-        get(ctype, r->d_ctype, 4);
-        get(tv, r->d_tv, 8);
-        get(vs, r->d_vs, 8);
-        get(tb, r->d_tb, 8);
-        get(bs, r->d_bs, 8);
+        if (fv || bm || active || ctype || tv || vs || tb || bs) {
+            // 64-byte cell records -> per-cell arrays
+            std::vector<cpm::CellRec> cells(cap1);
+            hip_check(hipMemcpy(cells.data(), r->d_cells, cap1 * sizeof(cpm::CellRec), hipMemcpyDeviceToHost),
+                      "download");
+            for (size_t c = 0; c < cap; ++c) {
+                const cpm::CellRec &q = cells[c + 1];
+                if (fv) fv[c] = q.vol;
+                if (bm) bm[c] = q.bnd;
+                if (active) active[c] = q.active;
+                if (ctype) ctype[c] = q.ctype;
+                if (tv) tv[c] = q.tv;
+                if (vs) vs[c] = q.vs;
+                if (tb) tb[c] = q.tb;
+                if (bs) bs[c] = q.bs;
+            }
+        }
         if (mv && r->desc.n_media > 0)
             hip_check(hipMemcpy(mv, r->d_mvol, (size_t)r->desc.n_media * 4, hipMemcpyDeviceToHost), "download");
     }

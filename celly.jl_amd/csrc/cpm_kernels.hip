@@ -592,80 +592,157 @@ struct RowGeom {
     static constexpr int RPI = 64 / LPR;             // rows per wave instruction
     static constexpr int BAND = TY / kWaves;         // tile rows of one wave
     static constexpr int CROWS = BAND / 2;           // rows of the colour in the band
-    static constexpr int ITERS = (CROWS + RPI - 1) / RPI;
-    static constexpr int CAP = 2 * 64 * ITERS;       // colour sites of one wave
+    static constexpr int ITERS = (CROWS + RPI - 1) / RPI;  // row strips of one wave
+    static constexpr int CAP = 2 * 64;               // colour sites of one strip
     static constexpr int Q = TX * TY / kWaves;       // segment stretch capacity
     static_assert(TX % 4 == 0 && 64 % LPR == 0 && TY % (2 * kWaves) == 0, "row tile shape");
-    static_assert(CAP <= Q, "stretch holds every colour site of the wave");
+    static_assert(CAP * ITERS <= Q, "stretch holds every colour site of the wave");
 };
 
-// One colour row strip: colour sites at x residues RX and RX + 2 of each 16-byte group.
+// One row strip of a wave in registers: 16 bytes of the colour row and of the rows below
+// and above, and the x neighbour outside the 16 bytes (x - 1 for RX == 0, x + 4 else).
+struct RowStrip {
+    v4i mid, dn, up;
+    int32_t side;
+    uint32_t site0;  // 1-based site of the first of the 16 bytes
+    bool valid;
+};
+
 template <int RX, int TX, int TY>
-__device__ __forceinline__ void rows_collect(const CPM_GLOBAL int32_t *lat, int X, int Y, int x0, int y0,
-                                             int ry, int wave, uint32_t &ncand, uint32_t &nsites,
-                                             uint32_t &nsame, uint32_t *s_site, int32_t *s_own,
-                                             int32_t (*s_nb)[RowGeom<TX, TY>::CAP]) {
+__device__ __forceinline__ RowStrip row_load(const CPM_GLOBAL int32_t *lat, int X, int Y, int x0, int y0, int ry,
+                                             int wave, int it) {
     using R = RowGeom<TX, TY>;
     const int lane = (int)__lane_id();
+    const int k = it * R::RPI + lane / R::LPR;
+    RowStrip st;
+    st.valid = k < R::CROWS;
+    const int gx = x0 + 4 * (lane % R::LPR);
+    const int gy = y0 + wave * R::BAND + 2 * k + ry;
+    st.mid = st.dn = st.up = v4i{0, 0, 0, 0};
+    st.side = 0;
+    st.site0 = 1u + (uint32_t)gx + (uint32_t)gy * (uint32_t)X;
+    if (st.valid) {
+        const int gym = gy == 0 ? Y - 1 : gy - 1;
+        const int gyp = gy + 1 == Y ? 0 : gy + 1;
+        const uint32_t row = (uint32_t)gy * (uint32_t)X;
+        st.mid = *(const CPM_GLOBAL v4i *)(lat + row + (uint32_t)gx);
+        st.dn = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gym * (uint32_t)X + (uint32_t)gx);
+        st.up = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gyp * (uint32_t)X + (uint32_t)gx);
+        const int sx = RX == 0 ? (gx == 0 ? X - 1 : gx - 1) : (gx + 4 == X ? 0 : gx + 4);
+        st.side = lat[row + (uint32_t)sx];
+    }
+    return st;
+}
+
+// Colour sites of one strip (x residues RX and RX + 2 of each 16 bytes): interior sites are
+// counted as same-owner no-ops, boundary sites appended to the wave's candidate list.
+template <int RX>
+__device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand, uint32_t &nsites,
+                                            uint32_t &nsame, uint32_t *s_site, int32_t *s_own,
+                                            int32_t (*s_nb)[2 * 64]) {
+    nsites += 2u * (uint32_t)__popcll(__ballot(st.valid));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = RX + 2 * h;
+        const v4i mid = st.mid, dn = st.dn, up = st.up;
+        const int32_t own = p == 0 ? mid.x : p == 1 ? mid.y : p == 2 ? mid.z : mid.w;
+        const int32_t left = p == 0 ? st.side : p == 1 ? mid.x : p == 2 ? mid.y : mid.z;
+        const int32_t right = p == 0 ? mid.y : p == 1 ? mid.z : p == 2 ? mid.w : st.side;
+        const int32_t below = p == 0 ? dn.x : p == 1 ? dn.y : p == 2 ? dn.z : dn.w;
+        const int32_t above = p == 0 ? up.x : p == 1 ? up.y : p == 2 ? up.z : up.w;
+        const bool interior = own == left && own == below && own == above && own == right;
+        const bool boundary = st.valid && !interior;
+        nsame += (uint32_t)__popcll(__ballot(st.valid && interior));
+        const uint64_t m = __ballot(boundary);
+        if (boundary) {
+            const uint32_t e = ncand + __popcll(m & __lanemask_lt());
+            s_site[e] = st.site0 + (uint32_t)p;
+            s_own[e] = own;
+            s_nb[0][e] = left;
+            s_nb[1][e] = below;
+            s_nb[2][e] = above;
+            s_nb[3][e] = right;
+        }
+        ncand += (uint32_t)__popcll(m);
+    }
+}
+
+// The wave's part of one row-kernel tile: every strip's loads are issued first, then each
+// strip runs its direction draws (ProposalDirectionStream) over its compacted boundary
+// sites and its priority draws and claims over its compacted actionable attempts.
+template <int RX, int TX, int TY>
+__device__ __forceinline__ void row_wave(const PassArgs &a, const RepDev &d, const TileInfo &ti, uint32_t tile,
+                                         uint64_t mcs, int ry, int wave, uint32_t *s_site, int32_t *s_own,
+                                         int32_t (*s_nb)[2 * 64], uint32_t &nsites, uint32_t &nsame,
+                                         uint32_t &nact) {
+    using R = RowGeom<TX, TY>;
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    const int X = d.dims[0], Y = d.dims[1];
+    const int lane = (int)__lane_id();
+    RowStrip st[R::ITERS];
+#pragma unroll
+    for (int it = 0; it < R::ITERS; ++it) st[it] = row_load<RX, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, it);
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
+    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
 #pragma unroll
     for (int it = 0; it < R::ITERS; ++it) {
-        const int k = it * R::RPI + lane / R::LPR;
-        const bool valid = k < R::CROWS;
-        const int gx = x0 + 4 * (lane % R::LPR);
-        const int gy = y0 + wave * R::BAND + 2 * k + ry;
-        v4i mid = v4i{0, 0, 0, 0}, dn = mid, up = mid;
-        int32_t side = 0;
-        if (valid) {
-            const int gym = gy == 0 ? Y - 1 : gy - 1;
-            const int gyp = gy + 1 == Y ? 0 : gy + 1;
-            const uint32_t row = (uint32_t)gy * (uint32_t)X;
-            mid = *(const CPM_GLOBAL v4i *)(lat + row + (uint32_t)gx);
-            dn = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gym * (uint32_t)X + (uint32_t)gx);
-            up = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gyp * (uint32_t)X + (uint32_t)gx);
-            const int sx = RX == 0 ? (gx == 0 ? X - 1 : gx - 1) : (gx + 4 == X ? 0 : gx + 4);
-            side = lat[row + (uint32_t)sx];
-        }
-        const uint32_t site0 = 1u + (uint32_t)gx + (uint32_t)gy * (uint32_t)X;
-        nsites += 2u * (uint32_t)__popcll(__ballot(valid));
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int p = RX + 2 * h;
-            const int32_t own = p == 0 ? mid.x : p == 1 ? mid.y : p == 2 ? mid.z : mid.w;
-            const int32_t left = p == 0 ? side : p == 1 ? mid.x : p == 2 ? mid.y : mid.z;
-            const int32_t right = p == 0 ? mid.y : p == 1 ? mid.z : p == 2 ? mid.w : side;
-            const int32_t below = p == 0 ? dn.x : p == 1 ? dn.y : p == 2 ? dn.z : dn.w;
-            const int32_t above = p == 0 ? up.x : p == 1 ? up.y : p == 2 ? up.z : up.w;
-            const bool interior = own == left && own == below && own == above && own == right;
-            const bool boundary = valid && !interior;
-            nsame += (uint32_t)__popcll(__ballot(valid && interior));
-            const uint64_t m = __ballot(boundary);
-            if (boundary) {
-                const uint32_t e = ncand + __popcll(m & __lanemask_lt());
-                s_site[e] = site0 + (uint32_t)p;
-                s_own[e] = own;
-                s_nb[0][e] = left;
-                s_nb[1][e] = below;
-                s_nb[2][e] = above;
-                s_nb[3][e] = right;
+        uint32_t ncand = 0;
+        row_collect<RX>(st[it], ncand, nsites, nsame, s_site, s_own, s_nb);
+        // direction draws; actionable attempts compacted in place (entry e is read before
+        // any write at or below it)
+        uint32_t na = 0;
+        for (uint32_t e0 = 0; e0 < ncand; e0 += 64) {
+            const uint32_t e = e0 + (uint32_t)lane;
+            bool act = false;
+            uint32_t site = 0;
+            int32_t own = 0, gain = 0;
+            if (e < ncand) {
+                site = s_site[e];
+                own = s_own[e];
+                // four directions: bounded_uint(4) never rejects ((2^32 - 4) mod 4 == 0)
+                const uint32_t dir = rng_words(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity,
+                                               site, 0, 0).w0 & 3u;
+                gain = s_nb[dir][e];
+                act = gain != own;
             }
-            ncand += (uint32_t)__popcll(m);
+            const uint64_t valid = __ballot(e < ncand);
+            const uint64_t m = __ballot(act);
+            nsame += (uint32_t)__popcll(valid & ~m);
+            if (act) {
+                const uint32_t at = na + __popcll(m & __lanemask_lt());
+                s_site[at] = site;
+                s_own[at] = own;
+                s_nb[0][at] = gain;
+            }
+            na += (uint32_t)__popcll(m);
         }
+        // priority draws and claims (CheckerboardPriorityStream); stretch `wave` of the segment
+        for (uint32_t e = (uint32_t)lane; e < na; e += 64) {
+            const uint32_t site = s_site[e];
+            const int32_t losing = s_own[e];
+            const int32_t gaining = s_nb[0][e];
+            const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
+                                            kSiteEntity, site, 0, 0).w0;
+            const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
+            if (losing > 0) atomicMax(&claims[(uint32_t)losing], key);
+            if (gaining > 0) atomicMax(&claims[(uint32_t)gaining], key);
+            seg[nact + e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
+        }
+        nact += na;
     }
 }
 
 template <int TX, int TY>
 __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
-    using R = RowGeom<TX, TY>;
-    __shared__ uint32_t s_site[kWaves][R::CAP];
-    __shared__ int32_t s_own[kWaves][R::CAP];
-    __shared__ int32_t s_nb[kWaves][4][R::CAP];
-    __shared__ uint32_t s_cnt[kWaves][4];
+    __shared__ uint32_t s_site[kWaves][2 * 64];
+    __shared__ int32_t s_own[kWaves][2 * 64];
+    __shared__ int32_t s_nb[kWaves][4][2 * 64];
+    __shared__ uint32_t s_cnt[kWaves][3];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
-    const uint32_t r = ti.rep;
-    const RepDev &d = a.reps[r];
-    const RepDyn &y = a.dyn[r];
+    const RepDev &d = a.reps[ti.rep];
+    const RepDyn &y = a.dyn[ti.rep];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -674,67 +751,19 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     const uint32_t colour = y.order[a.pass];
     const int res = d.col_res[d.col_first[colour]];
     const int rx = d.pat_coord[res][0], ry = d.pat_coord[res][1];
-    const int X = d.dims[0], Y = d.dims[1];
     const int wave = (int)(threadIdx.x >> 6);
-    const int lane = (int)__lane_id();
-    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
-
-    // 1. colour sites of the wave's rows; interior sites are same-owner no-ops
-    uint32_t ncand = 0, nsites = 0, nsame = 0;
+    uint32_t nsites = 0, nsame = 0, nact = 0;
     if (rx == 0)
-        rows_collect<0, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, ncand, nsites, nsame, s_site[wave],
-                                s_own[wave], s_nb[wave]);
+        row_wave<0, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], nsites, nsame,
+                            nact);
     else
-        rows_collect<1, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, ncand, nsites, nsame, s_site[wave],
-                                s_own[wave], s_nb[wave]);
-
-    // 2. direction draws of the boundary sites (ProposalDirectionStream); actionable
-    //    attempts are compacted in place (entry e is read before any write at or below it)
-    uint32_t na = 0;
-    for (uint32_t e0 = 0; e0 < ncand; e0 += 64) {
-        const uint32_t e = e0 + (uint32_t)lane;
-        bool act = false;
-        uint32_t site = 0;
-        int32_t own = 0, gain = 0;
-        if (e < ncand) {
-            site = s_site[wave][e];
-            own = s_own[wave][e];
-            const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity,
-                                              site, 0, d.prop_bound);
-            gain = s_nb[wave][dir & 3u][e];
-            act = gain != own;
-        }
-        const uint64_t valid = __ballot(e < ncand);
-        const uint64_t m = __ballot(act);
-        nsame += (uint32_t)__popcll(valid & ~m);
-        if (act) {
-            const uint32_t at = na + __popcll(m & __lanemask_lt());
-            s_site[wave][at] = site;
-            s_own[wave][at] = own;
-            s_nb[wave][0][at] = gain;
-        }
-        na += (uint32_t)__popcll(m);
-    }
-
-    // 3. priority draws and claims of the actionable attempts; stretch `wave` of the segment
-    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
-    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
-    for (uint32_t e = (uint32_t)lane; e < na; e += 64) {
-        const uint32_t site = s_site[wave][e];
-        const int32_t losing = s_own[wave][e];
-        const int32_t gaining = s_nb[wave][0][e];
-        const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
-                                        kSiteEntity, site, 0, 0).w0;
-        const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-        if (losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)losing], key);
-        if (gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)gaining], key);
-        seg[e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
-    }
-    if (lane == 0) {
-        a.seg_count[tile * kWaves + wave] = na;
+        row_wave<1, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], nsites, nsame,
+                            nact);
+    if (__lane_id() == 0) {
+        a.seg_count[tile * kWaves + wave] = nact;
         s_cnt[wave][0] = nsites;
         s_cnt[wave][1] = nsame;
-        s_cnt[wave][2] = na;
+        s_cnt[wave][2] = nact;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -972,6 +1001,30 @@ __device__ __forceinline__ int32_t neighbour_word(const D &d, int x, int y, int 
 // keeps the reference's operation order.
 constexpr int kNb = 8;
 
+// The fields of a cell record that evaluation reads, from three 16-byte loads of its line.
+struct CellHot {
+    int32_t vol;
+    uint32_t ctype;
+    int64_t bnd;
+    double tv, vs;
+    int64_t tb;
+    double bs;
+};
+
+__device__ __forceinline__ CellHot load_cell(const CPM_GLOBAL CellRec *cells, uint32_t i) {
+    const CPM_GLOBAL v4i *p = (const CPM_GLOBAL v4i *)(cells + i);
+    const v4i a = p[0], b = p[1], c = p[2];
+    CellHot h;
+    h.vol = a.x;
+    h.ctype = (uint32_t)a.y;
+    h.bnd = (int64_t)(((uint64_t)(uint32_t)a.w << 32) | (uint32_t)a.z);
+    h.tv = __hiloint2double(b.y, b.x);
+    h.vs = __hiloint2double(b.w, b.z);
+    h.tb = (int64_t)(((uint64_t)(uint32_t)c.y << 32) | (uint32_t)c.x);
+    h.bs = __hiloint2double(c.w, c.z);
+    return h;
+}
+
 // Tracker values of a winner's cells at the pass snapshot and the boundary changes of its
 // copy: the commit writes snapshot + change (no other winner touches these cells).
 struct Staged {
@@ -993,10 +1046,11 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
     const int z = (int)zq;
     // slot 0 of every per-cell array is a spare, read in place of a medium owner
     const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
-    const int32_t vl0 = gp(d.vol)[li], vg0 = gp(d.vol)[gi];
-    const double vs_l = gp(d.vs)[li], tv_l = gp(d.tv)[li], vs_g = gp(d.vs)[gi], tv_g = gp(d.tv)[gi];
-    const double bs_l = gp(d.bs)[li], bs_g = gp(d.bs)[gi];
-    const int64_t tb_l = gp(d.tb)[li], tb_g = gp(d.tb)[gi], bm_l = gp(d.bnd)[li], bm_g = gp(d.bnd)[gi];
+    const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
+    const int32_t vl0 = cl.vol, vg0 = cg.vol;
+    const double vs_l = cl.vs, tv_l = cl.tv, vs_g = cg.vs, tv_g = cg.tv;
+    const double bs_l = cl.bs, bs_g = cg.bs;
+    const int64_t tb_l = cl.tb, tb_g = cg.tb, bm_l = cl.bnd, bm_g = cg.bnd;
     const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
     // sites at least `halo` from every face of a periodic, obstacle-free lattice reach
     // their neighbours by a linear delta
@@ -1007,11 +1061,22 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
     auto nb = [&](const auto *o, int32_t lin) __attribute__((always_inline)) {
         return inner ? lat[(int32_t)idx0 + lin] : neighbour_word(d, x, y, z, o);
     };
+    // neighbour words: one unconditional load per direction (lanes off the fast path load
+    // their own site), so every load is in flight at once; lanes within `halo` of a face
+    // or on a lattice with walls or obstacles then realize their neighbours one by one
     int32_t sw[kNb], cw[kNb];
 #pragma unroll
-    for (int u = 0; u < kNb; ++u) sw[u] = u < ns ? nb(d.surf.off[u], d.surf_lin[u]) : 0;
+    for (int u = 0; u < kNb; ++u) sw[u] = u < ns ? lat[(int32_t)idx0 + (inner ? d.surf_lin[u] : 0)] : 0;
 #pragma unroll
-    for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? nb(d.cont.off[u], d.cont_lin[u]) : 0;
+    for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? lat[(int32_t)idx0 + (inner ? d.cont_lin[u] : 0)] : 0;
+    if (!inner) {
+#pragma unroll
+        for (int u = 0; u < kNb; ++u)
+            if (u < ns) sw[u] = neighbour_word(d, x, y, z, d.surf.off[u]);
+#pragma unroll
+        for (int u = 0; u < kNb; ++u)
+            if (u < nc) cw[u] = neighbour_word(d, x, y, z, d.cont.off[u]);
+    }
     uint32_t ct[kNb];
 #pragma unroll
     for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
@@ -1058,7 +1123,8 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
             // scientific_components.jl:289-308, directions in canonical order
             v64 = d.contact_f64;
             const uint32_t nt = (uint32_t)d.n_types;
-            const uint32_t lt = owner_type(d, l), gt = owner_type(d, g);
+            const uint32_t lt = l > 0 ? min(max(cl.ctype, 1u), nt) : owner_type(d, l);
+            const uint32_t gt = g > 0 ? min(max(cg.ctype, 1u), nt) : owner_type(d, g);
             double dd = 0.0;
             float df = 0.0f;
             auto contact_term = [&](int k, int32_t w, uint32_t tw) __attribute__((always_inline)) {
@@ -1130,24 +1196,26 @@ template <class D>
 __device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &st) {
     const int32_t l = e.losing, g = e.gaining;
     gp(d.lattice)[e.site - 1] = g;
+    CPM_GLOBAL CellRec *cells = gp(d.cells);
     if (l > 0) {
         const int32_t nv = st.vl - 1;
-        gp(d.vol)[l] = nv;
-        gp(d.bnd)[l] = st.bl + st.dl;
+        cells[l].vol = nv;
+        cells[l].bnd = st.bl + st.dl;
         if (nv == 0) {
-            gp(d.active)[l] = 0;
+            cells[l].active = 0;
+            cells[l].ctype = 0;
             gp(d.ctype)[l] = 0;
-            gp(d.tv)[l] = d.retire_tv;
-            gp(d.vs)[l] = d.retire_vs;
-            gp(d.tb)[l] = d.retire_tb;
-            gp(d.bs)[l] = d.retire_bs;
+            cells[l].tv = d.retire_tv;
+            cells[l].vs = d.retire_vs;
+            cells[l].tb = d.retire_tb;
+            cells[l].bs = d.retire_bs;
         }
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
     }
     if (g > 0) {
-        gp(d.vol)[g] = st.vg + 1;
-        gp(d.bnd)[g] = st.bg + st.dg;
+        cells[g].vol = st.vg + 1;
+        cells[g].bnd = st.bg + st.dg;
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
     }
@@ -1222,28 +1290,24 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
         // entries of the batch, then their claims
         v4i v[kCheckBatch];
         uint32_t kt[kCheckBatch];
+        // branch-free: lanes past the end read entry 0 and cell slot 0, so all loads of
+        // the batch are in flight together
 #pragma unroll
         for (int j = 0; j < kCheckBatch; ++j) {
-            const uint32_t e = blk + (uint32_t)j * NT + tid;
-            v[j] = v4i{0, 0, 0, 0};
-            kt[j] = 0;
-            if (e < total) {
-                int k = 0;
+            const uint32_t e = min(blk + (uint32_t)j * NT + tid, total - 1);
+            int k = 0;
 #pragma unroll
-                for (int step = NS / 2; step > 0; step >>= 1)
-                    if (e >= s_base[k + step]) k += step;
-                kt[j] = (uint32_t)k / kWaves;
-                v[j] = region[kt[j] * a.seg_cap + (uint32_t)(k % kWaves) * Q + (e - s_base[k])];
-            }
+            for (int step = NS / 2; step > 0; step >>= 1)
+                if (e >= s_base[k + step]) k += step;
+            kt[j] = (uint32_t)k / kWaves;
+            v[j] = region[kt[j] * a.seg_cap + (uint32_t)(k % kWaves) * Q + (e - s_base[k])];
         }
         unsigned long long cl[kCheckBatch], cg[kCheckBatch];
 #pragma unroll
         for (int j = 0; j < kCheckBatch; ++j) {
-            const uint32_t e = blk + (uint32_t)j * NT + tid;
             const uint32_t cb = s_cbase[kt[j]];
-            cl[j] = cg[j] = 0ull;
-            if (e < total && v[j].z > 0) cl[j] = claims[cb + (uint32_t)v[j].z];
-            if (e < total && v[j].w > 0) cg[j] = claims[cb + (uint32_t)v[j].w];
+            cl[j] = claims[cb + (uint32_t)max(v[j].z, 0)];
+            cg[j] = claims[cb + (uint32_t)max(v[j].w, 0)];
         }
 #pragma unroll
         for (int j = 0; j < kCheckBatch; ++j) {
@@ -1398,6 +1462,8 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
         {3, 1, 64, 8, 4},   {3, 1, 32, 16, 4},
         // one-wave workgroups
         {2, 1, 32, 16, 1},  {2, 1, 64, 8, 1},
+        // 4096-site tiles (two row strips per wave in the row kernel)
+        {2, 1, 256, 16, 1}, {2, 1, 128, 32, 1},
     };
     const int n = (int)(sizeof(table) / sizeof(table[0]));
     if (id < 0 || id >= n) return 0;
@@ -1423,6 +1489,8 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
         case 5: hipLaunchKernelGGL((cb_propose_rows_kernel<256, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         case 6: hipLaunchKernelGGL((cb_propose_rows_kernel<64, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         case 7: hipLaunchKernelGGL((cb_propose_rows_kernel<32, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 12: hipLaunchKernelGGL((cb_propose_rows_kernel<256, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 13: hipLaunchKernelGGL((cb_propose_rows_kernel<128, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -1441,6 +1509,8 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
     case 9: return launch_propose_t<3, 1, 32, 16, 4>(algorithm, g, *a, stream);
     case 10: return launch_propose_t<2, 1, 32, 16, 1, 64>(algorithm, g, *a, stream);
     case 11: return launch_propose_t<2, 1, 64, 8, 1, 64>(algorithm, g, *a, stream);
+    case 12: return launch_propose_t<2, 1, 256, 16, 1>(algorithm, g, *a, stream);
+    case 13: return launch_propose_t<2, 1, 128, 32, 1>(algorithm, g, *a, stream);
     default: return hipErrorInvalidValue;
     }
 }
