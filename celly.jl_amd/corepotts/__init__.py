@@ -4,7 +4,9 @@ Host-side mirror of the CorePotts API surface for CheckerboardSweepCPM and Lotte
 backed by hand-written HIP kernels for gfx950 in libcpm_mi355x.so (C-ABI in
 include/cpm_mi355x.h).
 """
-from .model import (AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
+from .model import (AlgorithmTemperatureNoise, ConstitutiveMeanInitialization, FixedMechanicalNoise,
+                    FluctuatingSurfaceTension, FluctuatingVolumePressure, PreserveMechanicalInitialization,
+                    StationaryMechanicalInitialization, AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
                     CellOwner, CheckerboardSweepCPM, ClosedBoundary, CompiledScientificState,
                     ContactRole, FixedExterior, LogicalPottsState, LotteryCPM, MediumOwner,
                     MediumTypeTable, PeriodicBoundary, ProposalRole, QuadraticBoundaryHamiltonian,

@@ -16,8 +16,8 @@ from typing import Dict, Optional
 import numpy as np
 
 from .model import CompiledScientificState
-from .native import (EngineError, ScientificMCSReport, load_library, make_desc, snapshot,
-                     state_buffers, state_pointers)
+from .native import (EngineError, ScientificMCSReport, _ptr, load_library, make_desc, mechanics_arrays,
+                     mechanics_columns, snapshot, state_buffers, state_pointers)
 from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ModelSpec, build_spec, state_arrays
 
 
@@ -83,12 +83,19 @@ class ReplicaBatch:
         self.specs[int(slot.value)] = spec
         return int(slot.value)
 
-    def load(self, slot: int, compiled: CompiledScientificState, target_mcs: Optional[int] = None):
-        """Upload the state of a reserved replica and set its target MCS."""
+    def load(self, slot: int, compiled: CompiledScientificState, target_mcs: Optional[int] = None,
+             initialize_mechanics: bool = True):
+        """Upload the state of a reserved replica and set its target MCS. Mechanical states are
+        initialized as init_scientific does unless initialize_mechanics is False."""
         spec = self.specs[slot]
         try:
             a = state_arrays(compiled, spec)
             self._check(self.lib.cpm_batch_upload_state(self._b, int(slot), *state_pointers(a)))
+            if spec.mechanics:
+                qp, qt = mechanics_arrays(compiled, spec)
+                self._check(self.lib.cpm_batch_upload_mechanics(self._b, int(slot), _ptr(qp), _ptr(qt)))
+                if initialize_mechanics:
+                    self._check(self.lib.cpm_batch_initialize_mechanics(self._b, int(slot)))
             if target_mcs is not None:
                 self.set_target(slot, target_mcs)
         except Exception:
@@ -152,6 +159,13 @@ class ReplicaBatch:
         out = state_buffers(spec)
         self._check(self.lib.cpm_batch_download_state(self._b, int(slot), *state_pointers(out)))
         out["medium_volumes"] = out["medium_volumes"][:len(spec.medium_ids)]
+        if spec.mechanics:
+            qp, qt = (None if key is None else np.zeros(spec.capacity, np.float32)
+                      for key in mechanics_columns(spec))
+            self._check(self.lib.cpm_batch_download_mechanics(self._b, int(slot), _ptr(qp), _ptr(qt)))
+            for key, arr in zip(mechanics_columns(spec), (qp, qt)):
+                if key is not None:
+                    out[key] = arr
         return out
 
     def logical_state(self, slot: int) -> CompiledScientificState:

@@ -222,7 +222,9 @@ def model_fingerprint(spec) -> bytes:
                             bool(spec.contact_f64), bool(spec.boundary_f64), np.asarray(spec.J),
                             tuple(int(o) for o in spec.table_owner), tuple(int(t) for t in spec.table_type),
                             tuple(int(m) for m in spec.medium_ids), np.float32(spec.temperature),
-                            int(spec.algorithm), tuple(float(r) for r in spec.retire))
+                            int(spec.algorithm), tuple(float(r) for r in spec.retire),
+                            tuple((int(k), int(i), int(n), np.float32(e), bool(t), np.float32(z))
+                                  for k, i, n, e, t, z in spec.mechanics))
 
 
 def continuation_profile(spec, compiled) -> ExactContinuationProfile:
@@ -413,7 +415,7 @@ def restore_into_batch(cp: CanonicalCheckpoint, batch, spec, prototype_compiled,
         raise CheckpointCompatibilityError("model", "replica model fingerprint", "checkpoint model fingerprint")
     compiled = checkpoint_compiled_state(cp, prototype_compiled)
     slot = batch.reserve(dataclasses.replace(spec, seed=int(cp.seed)))
-    batch.load(slot, compiled)
+    batch.load(slot, compiled, initialize_mechanics=False)
     batch.set_mcs(slot, cp.mcs)
     if target_mcs is not None:
         batch.set_target(slot, target_mcs)

@@ -383,6 +383,77 @@ class QuadraticBoundaryHamiltonian:
     strength: str = "boundary_strength"
 
 
+# ---------------------------------------------------------------------------
+# Mechanics (scientific_components.jl:130-240, 575-660)
+# ---------------------------------------------------------------------------
+MECH_PRESSURE, MECH_TENSION = 1, 2
+MECH_STATE_COLUMN = {MECH_PRESSURE: "volume_pressure", MECH_TENSION: "surface_tension"}
+# MechanicalInitialization (scientific_components.jl:130-134)
+ConstitutiveMeanInitialization = 1
+StationaryMechanicalInitialization = 2
+PreserveMechanicalInitialization = 3
+
+
+class AlgorithmTemperatureNoise:
+    """Mechanical noise scale = the algorithm temperature (scientific_components.jl:136-137)."""
+
+
+class FixedMechanicalNoise:
+    """Mechanical noise scale independent of the acceptance temperature (:139-150)."""
+
+    def __init__(self, value: float):
+        if not np.isfinite(value) or value < 0:
+            raise ValueError("fixed mechanical noise must be finite and non-negative")
+        self.value = float(value)
+
+
+class _Mechanics:
+    def __init__(self, eta, noise, initialization, instance_id, number_type, target, strength, state,
+                 what):
+        rate = np.dtype(number_type).type(eta)
+        if not (np.isfinite(rate) and rate > 0):
+            raise ValueError(f"{what} relaxation rate must be finite and positive")
+        if isinstance(noise, AlgorithmTemperatureNoise):
+            noise = None
+        if noise is not None and not isinstance(noise, FixedMechanicalNoise):
+            raise ValueError("unsupported mechanical-noise declaration")
+        if initialization not in (ConstitutiveMeanInitialization, StationaryMechanicalInitialization,
+                                  PreserveMechanicalInitialization):
+            raise ValueError("unknown mechanical initialization")
+        if not 0 < int(instance_id) <= 0xFFFF:
+            raise ValueError("mechanical component instance ID must be positive and fit UInt16")
+        if len({target, strength, state}) != 3:
+            raise ValueError(f"{what} target, strength, and state properties must be distinct")
+        self.eta = rate
+        self.noise = noise                   # None: AlgorithmTemperatureNoise
+        self.initialization = initialization
+        self.instance_id = int(instance_id)
+        self.number_type = np.dtype(number_type).type
+        self.target, self.strength, self.state = target, strength, state
+
+
+class FluctuatingVolumePressure(_Mechanics):
+    """Per-cell pressure with an exact frozen-volume OU transition (scientific_components.jl:152-182)."""
+
+    def __init__(self, eta=1.0, noise=None, initialization=ConstitutiveMeanInitialization, instance_id=1,
+                 number_type=np.float32, target="target_volume", strength="volume_strength",
+                 state="volume_pressure"):
+        super().__init__(eta, noise, initialization, instance_id, number_type, target, strength, state,
+                         "volume-pressure")
+
+
+class FluctuatingSurfaceTension(_Mechanics):
+    """Per-cell tension with an exact frozen-surface OU transition (scientific_components.jl:576-616)."""
+
+    def __init__(self, metric, relation, eta=1.0, noise=None, initialization=ConstitutiveMeanInitialization,
+                 instance_id=1, number_type=np.float32, target="target_boundary",
+                 strength="boundary_strength", state="surface_tension"):
+        super().__init__(eta, noise, initialization, instance_id, number_type, target, strength, state,
+                         "surface-tension")
+        self.metric = metric
+        self.relation = relation
+
+
 @dataclass
 class ScientificComponentSet:
     energies: Tuple = ()
@@ -400,6 +471,13 @@ def required_properties(component) -> Dict[str, np.dtype]:
         t = np.dtype(component.number_type)
         tt = np.dtype(np.int64) if isinstance(component.metric, BoundaryEdgeCount) else t
         return {component.target: tt, component.strength: t}
+    if isinstance(component, FluctuatingVolumePressure):
+        t = np.dtype(component.number_type)
+        return {component.target: t, component.strength: t, component.state: t}
+    if isinstance(component, FluctuatingSurfaceTension):
+        t = np.dtype(component.number_type)
+        tt = np.dtype(np.int64) if isinstance(component.metric, BoundaryEdgeCount) else t
+        return {component.target: tt, component.strength: t, component.state: t}
     return {}
 
 

@@ -14,7 +14,8 @@ from typing import NamedTuple, Optional
 
 import numpy as np
 
-from .model import CompiledScientificState, StaticCartesianRelation
+from .model import (MECH_PRESSURE, MECH_STATE_COLUMN, MECH_TENSION, CompiledScientificState,
+                    StaticCartesianRelation)
 from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ModelSpec, build_spec, state_arrays
 
 MAX_DIRS = 128
@@ -34,6 +35,11 @@ class CpmRelation(C.Structure):
                 ("weights", C.c_double * MAX_DIRS)]
 
 
+class CpmMechanicsDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("initialization", C.c_int32), ("instance_id", C.c_uint32),
+                ("eta", C.c_float), ("noise_is_temperature", C.c_int32), ("noise", C.c_float)]
+
+
 class CpmModelDesc(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("algorithm", C.c_int32), ("ndim", C.c_int32),
@@ -50,6 +56,7 @@ class CpmModelDesc(C.Structure):
         ("capacity", C.c_uint32), ("seed", C.c_uint64),
         ("retire_target_volume", C.c_double), ("retire_volume_strength", C.c_double),
         ("retire_target_boundary", C.c_int64), ("retire_boundary_strength", C.c_double),
+        ("n_mechanics", C.c_int32), ("mechanics", CpmMechanicsDesc * 2),
     ]
 
 
@@ -58,7 +65,9 @@ EXPORTS = (
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
     "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
     "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
-    "cpm_set_row_kernels", "cpm_kernel_config",
+    "cpm_set_row_kernels", "cpm_kernel_config", "cpm_upload_mechanics", "cpm_download_mechanics",
+    "cpm_initialize_mechanics", "cpm_batch_upload_mechanics", "cpm_batch_download_mechanics",
+    "cpm_batch_initialize_mechanics",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
@@ -115,6 +124,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_batch_download_state.argtypes = [P, C.c_int32] + state_args
     lib.cpm_batch_set_target.argtypes = [P, C.c_int32, C.c_uint64]
     lib.cpm_batch_set_mcs.argtypes = [P, C.c_int32, C.c_uint64]
+    lib.cpm_upload_mechanics.argtypes = [P, P, P]
+    lib.cpm_download_mechanics.argtypes = [P, P, P]
+    lib.cpm_initialize_mechanics.argtypes = [P]
+    lib.cpm_batch_upload_mechanics.argtypes = [P, C.c_int32, P, P]
+    lib.cpm_batch_download_mechanics.argtypes = [P, C.c_int32, P, P]
+    lib.cpm_batch_initialize_mechanics.argtypes = [P, C.c_int32]
     lib.cpm_batch_step.argtypes = [P, C.c_uint64]
     lib.cpm_batch_progress.argtypes = [P, P, C.c_int32]
     lib.cpm_batch_capacity.argtypes = [P]
@@ -192,6 +207,11 @@ def make_desc(spec: ModelSpec, keep: list) -> CpmModelDesc:
     d.seed = spec.seed
     d.retire_target_volume, d.retire_volume_strength = spec.retire[0], spec.retire[1]
     d.retire_target_boundary, d.retire_boundary_strength = spec.retire[2], spec.retire[3]
+    d.n_mechanics = len(spec.mechanics)
+    for k, (kind, init, instance, eta, noise_is_temperature, noise) in enumerate(spec.mechanics):
+        m = d.mechanics[k]
+        m.kind, m.initialization, m.instance_id = kind, init, instance
+        m.eta, m.noise_is_temperature, m.noise = float(eta), int(noise_is_temperature), float(noise)
     return d
 
 
@@ -223,6 +243,18 @@ def state_buffers(spec: ModelSpec) -> dict:
         boundary_strength=np.zeros(cap, np.float64))
 
 
+def mechanics_columns(spec: ModelSpec):
+    """Property keys of the model's mechanical state columns, in (pressure, tension) slot order."""
+    kinds = {m[0] for m in spec.mechanics}
+    return tuple(MECH_STATE_COLUMN[k] if k in kinds else None for k in (MECH_PRESSURE, MECH_TENSION))
+
+
+def mechanics_arrays(compiled: CompiledScientificState, spec: ModelSpec):
+    """Float32 mechanical state columns for cpm_upload_mechanics (None where absent)."""
+    return tuple(None if key is None else np.ascontiguousarray(compiled.properties[key], dtype=np.float32)
+                 for key in mechanics_columns(spec))
+
+
 def snapshot(compiled: CompiledScientificState, s: dict) -> CompiledScientificState:
     """A compiled host state holding the downloaded arrays `s`."""
     snap = compiled.copy()
@@ -232,8 +264,9 @@ def snapshot(compiled: CompiledScientificState, s: dict) -> CompiledScientificSt
     snap.boundary_measures = s["boundary_measures"]
     snap.active = s["active"]
     snap.cell_types = s["cell_types"]
-    for name in ("target_volume", "volume_strength", "target_boundary", "boundary_strength"):
-        if name in snap.properties:
+    for name in ("target_volume", "volume_strength", "target_boundary", "boundary_strength",
+                 "volume_pressure", "surface_tension"):
+        if name in snap.properties and name in s:
             snap.properties[name] = s[name].astype(snap.properties[name].dtype)
     return snap
 
@@ -293,11 +326,25 @@ class ScientificPottsIntegrator:
     def upload(self, compiled: CompiledScientificState):
         a = state_arrays(compiled, self.spec)
         self._check(self.lib.cpm_upload_state(self._ctx, *state_pointers(a)))
+        if self.spec.mechanics:
+            qp, qt = mechanics_arrays(compiled, self.spec)
+            self._check(self.lib.cpm_upload_mechanics(self._ctx, _ptr(qp), _ptr(qt)))
+
+    def initialize_mechanics(self):
+        """_initialize_mechanics! on the device state (scientific_mechanics.jl:63-103)."""
+        self._check(self.lib.cpm_initialize_mechanics(self._ctx))
 
     def download(self) -> dict:
         out = state_buffers(self.spec)
         self._check(self.lib.cpm_download_state(self._ctx, *state_pointers(out)))
         out["medium_volumes"] = out["medium_volumes"][:len(self.spec.medium_ids)]
+        if self.spec.mechanics:
+            qp, qt = (None if key is None else np.zeros(self.spec.capacity, np.float32)
+                      for key in mechanics_columns(self.spec))
+            self._check(self.lib.cpm_download_mechanics(self._ctx, _ptr(qp), _ptr(qt)))
+            for key, arr in zip(mechanics_columns(self.spec), (qp, qt)):
+                if key is not None:
+                    out[key] = arr
         return out
 
     def logical_state(self) -> CompiledScientificState:
@@ -373,11 +420,14 @@ class ScientificPottsIntegrator:
 
 
 def init_scientific(state: CompiledScientificState, proposal_relation, components, algorithm,
-                    seed: int = 0) -> ScientificPottsIntegrator:
-    """init_scientific(state, proposal_relation, components, algorithm; seed)
-    (checkerboard.jl:197-219, lottery.jl:94-115) on the MI355X engine."""
+                    seed: int = 0, initialize_mechanics: bool = True) -> ScientificPottsIntegrator:
+    """init_scientific(state, proposal_relation, components, algorithm; seed,
+    initialize_mechanics) (checkerboard.jl:197-219, lottery.jl:94-115) on the MI355X engine."""
     spec = build_spec(state, proposal_relation, components, algorithm, seed)
-    return ScientificPottsIntegrator(state, spec)
+    integ = ScientificPottsIntegrator(state, spec)
+    if spec.mechanics and initialize_mechanics:
+        integ.initialize_mechanics()
+    return integ
 
 
 def ensemble_seed(master_seed: int, trajectory: int, rerun: int = 1) -> int:

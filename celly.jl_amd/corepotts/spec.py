@@ -13,8 +13,9 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .model import (BoundaryEdgeCount, CheckerboardSweepCPM, ClosedBoundary,
-                    CompiledScientificState, FixedExterior, LotteryCPM, PeriodicBoundary,
+from .model import (MECH_PRESSURE, MECH_STATE_COLUMN, MECH_TENSION, BoundaryEdgeCount,
+                    CheckerboardSweepCPM, ClosedBoundary, CompiledScientificState, FixedExterior,
+                    FluctuatingSurfaceTension, FluctuatingVolumePressure, LotteryCPM, PeriodicBoundary,
                     QuadraticBoundaryHamiltonian, QuadraticVolumeHamiltonian, SequentialCPM,
                     StaticCartesianRelation, UnorderedContactHamiltonian,
                     validate_relation_domain)
@@ -50,7 +51,10 @@ class ModelSpec:
     capacity: int
     seed: int
     retire: Tuple[float, float, int, float] = (0.0, 0.0, 0, 0.0)
-    columns: dict = field(default_factory=dict)
+    columns: dict = field(default_factory=dict)   # volume/boundary column -> property key
+    # (kind, initialization, instance_id, eta, noise_is_temperature, noise) per mechanical
+    # component, ScientificComponentSet.mechanics order
+    mechanics: tuple = ()
 
     @property
     def n_types(self) -> int:
@@ -77,6 +81,38 @@ def algorithm_id(algorithm) -> int:
     raise TypeError("unsupported algorithm")
 
 
+def _mechanics_entry(component, compiled, algorithm, columns):
+    """(kind, initialization, instance_id, eta, noise_is_temperature, noise) of a mechanical
+    component (scientific_components.jl:152-182, 576-616), after the checks init_scientific
+    makes on it; its target and strength columns join the energies' columns."""
+    props = compiled.properties
+    if isinstance(component, FluctuatingVolumePressure):
+        kind, target_name, strength_name = MECH_PRESSURE, "target_volume", "volume_strength"
+    elif isinstance(component, FluctuatingSurfaceTension):
+        if not isinstance(component.metric, BoundaryEdgeCount):
+            raise NotImplementedError("only BoundaryEdgeCount surface tension runs on this engine")
+        if component.relation != compiled.tracker.relation:
+            raise ValueError("surface tension does not match the compiled boundary tracker descriptor")
+        kind, target_name, strength_name = MECH_TENSION, "target_boundary", "boundary_strength"
+    else:
+        raise NotImplementedError(f"mechanics {type(component).__name__} is outside this engine")
+    if np.dtype(component.number_type) != np.float32:
+        raise NotImplementedError("the engine keeps mechanical state in Float32")
+    if component.state != MECH_STATE_COLUMN[kind]:
+        raise NotImplementedError(f"the engine keeps this mechanical state in the {MECH_STATE_COLUMN[kind]} column")
+    for name, key in ((target_name, component.target), (strength_name, component.strength)):
+        if columns.setdefault(name, key) != key:
+            raise NotImplementedError(f"mechanics and energies must share the {name} column")
+        if key not in props:
+            raise ValueError(f"state lacks the property column {key}")
+    if component.state not in props:
+        raise ValueError(f"state lacks the property column {component.state}")
+    noise_is_temperature = component.noise is None
+    noise = np.float32(0.0) if noise_is_temperature else np.float32(component.noise.value)
+    return (kind, component.initialization, int(component.instance_id), np.float32(component.eta),
+            noise_is_temperature, noise)
+
+
 def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCartesianRelation,
                components, algorithm, seed: int = 0) -> ModelSpec:
     if not 0 <= int(seed) <= 0xFFFFFFFFFFFFFFFF:
@@ -87,8 +123,8 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
     name = type(algorithm).__name__
     if alg != ALG_SEQUENTIAL and components.constraints:
         raise ValueError(f"{name} does not yet qualify hard constraints; use SequentialCPM")
-    if components.drives or components.kinetic_modifiers or components.mechanics:
-        raise NotImplementedError("drives, kinetic modifiers and mechanics are outside this engine")
+    if components.drives or components.kinetic_modifiers:
+        raise NotImplementedError("drives and kinetic modifiers are outside this engine")
     if components.constraints:
         raise NotImplementedError("hard constraints are outside this engine")
     if np.dtype(type(algorithm.temperature)) != np.float32:
@@ -112,8 +148,8 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
             order.append(TERM_VOLUME)
             tv, vs = props[component.target], props[component.strength]
             volume_f64 = np.result_type(tv.dtype, vs.dtype) == np.float64
-            columns["target_volume"] = tv
-            columns["volume_strength"] = vs
+            columns["target_volume"] = component.target
+            columns["volume_strength"] = component.strength
         elif isinstance(component, UnorderedContactHamiltonian):
             order.append(TERM_CONTACT)
             contact_rel = component.relation
@@ -130,12 +166,19 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
             order.append(TERM_BOUNDARY)
             tb, bs = props[component.target], props[component.strength]
             boundary_f64 = np.result_type(tb.dtype, bs.dtype, component.relation.weight_dtype) == np.float64
-            columns["target_boundary"] = tb
-            columns["boundary_strength"] = bs
+            columns["target_boundary"] = component.target
+            columns["boundary_strength"] = component.strength
         else:
             raise NotImplementedError(f"energy {type(component).__name__} is outside this engine")
     if len(set(order)) != len(order):
         raise NotImplementedError("each energy term may appear once")
+    mechanics = []
+    for component in components.mechanics:
+        mechanics.append(_mechanics_entry(component, compiled, algorithm, columns))
+    if len({m[2] for m in mechanics}) != len(mechanics):
+        raise ValueError("mechanical components need distinct instance IDs")
+    if len({m[0] for m in mechanics}) != len(mechanics):
+        raise ValueError("mechanical components need distinct state properties")
     if J.shape[0] > MAX_TYPES:
         raise NotImplementedError("at most 16 cell types")
     if len(compiled.medium_ids) > MAX_MEDIA or len(table_owner) > MAX_MEDIA:
@@ -162,7 +205,7 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
         boundary_f64=bool(boundary_f64), J=J, table_owner=tuple(table_owner),
         table_type=tuple(table_type), medium_ids=tuple(compiled.medium_ids),
         temperature=np.float32(algorithm.temperature), capacity=compiled.capacity,
-        seed=int(seed), columns=columns)
+        seed=int(seed), columns=columns, mechanics=tuple(mechanics))
 
 
 def state_arrays(compiled: CompiledScientificState, spec: ModelSpec):
@@ -170,10 +213,10 @@ def state_arrays(compiled: CompiledScientificState, spec: ModelSpec):
     cap = compiled.capacity
 
     def col(name, dtype):
-        v = spec.columns.get(name)
-        if v is None:
+        key = spec.columns.get(name)
+        if key is None:
             return np.zeros(cap, dtype=dtype)
-        return np.ascontiguousarray(v, dtype=dtype)
+        return np.ascontiguousarray(compiled.properties[key], dtype=dtype)
 
     return dict(
         tags=np.ascontiguousarray(compiled.tags, dtype=np.uint8),
@@ -188,4 +231,7 @@ def state_arrays(compiled: CompiledScientificState, spec: ModelSpec):
         volume_strength=col("volume_strength", np.float64),
         target_boundary=col("target_boundary", np.int64),
         boundary_strength=col("boundary_strength", np.float64),
+        **{MECH_STATE_COLUMN[m[0]]: np.ascontiguousarray(compiled.properties[MECH_STATE_COLUMN[m[0]]],
+                                                         dtype=np.float32)
+           for m in spec.mechanics},
     )

@@ -29,8 +29,10 @@ enum : uint32_t {
     kStreamLotteryPriority = 7,
     kStreamCheckerboardOrder = 8,
     kStreamCheckerboardPriority = 17,
+    kStreamAuxEvolution = 9,
+    kStreamAuxInitialization = 18,
 };
-enum : uint32_t { kGlobalEntity = 0, kSiteEntity = 1 };
+enum : uint32_t { kGlobalEntity = 0, kSiteEntity = 1, kCellEntity = 2 };
 
 // Division by a runtime-uniform divisor via a precomputed reciprocal: q = umulhi(n, m)
 // with m = floor(2^32 / d) undershoots the quotient by at most one.
@@ -66,12 +68,15 @@ struct __attribute__((aligned(64))) CellRec {
     double vs;         // volume_strength
     int64_t tb;        // target_boundary
     double bs;         // boundary_strength
+    float qp;          // volume_pressure (FluctuatingVolumePressure state)
+    float qt;          // surface_tension (FluctuatingSurfaceTension state)
     uint8_t active;
-    uint8_t pad[15];
+    uint8_t pad[7];
 };
 static_assert(sizeof(CellRec) == 64, "one cache line per cell");
 static_assert(offsetof(CellRec, bnd) == 8 && offsetof(CellRec, tv) == 16 && offsetof(CellRec, vs) == 24 &&
-                  offsetof(CellRec, tb) == 32 && offsetof(CellRec, bs) == 40 && offsetof(CellRec, active) == 48,
+                  offsetof(CellRec, tb) == 32 && offsetof(CellRec, bs) == 40 && offsetof(CellRec, qp) == 48 &&
+                  offsetof(CellRec, qt) == 52 && offsetof(CellRec, active) == 56,
               "load_cell reads the fields at these offsets");
 
 struct RepDev {
@@ -129,6 +134,17 @@ struct RepDev {
     const uint16_t *colour_list; // colourings without pat_fast: tile positions by tile and colour
     const uint32_t *colour_off;  // [tile][rounds + 1] ranges into colour_list
     const uint8_t *site_degree;  // lottery, nullptr: uniform degree
+    uint32_t capacity;           // cell slots (ids 1..capacity)
+    // mechanics (FluctuatingVolumePressure / FluctuatingSurfaceTension, tuple order)
+    int32_t n_mech;
+    int32_t mech_kind[2];        // CPM_MECH_*
+    int32_t mech_init[2];        // CPM_MECH_INIT_*
+    uint32_t mech_instance[2];   // RNG operation
+    float mech_eta[2];
+    float mech_noise[2];         // noise scale (the temperature for AlgorithmTemperatureNoise)
+    // half interval of the OU steps around a pass: by colour (checkerboard,
+    // size / (2 * mutable sites)) or entry 0 (lottery, 1 / (2 * rounds)), Float32
+    float mech_half[kMaxRounds + 1];
     CellRec *cells;              // cap + 1, index = cell id (entry 0: spare read for media)
     uint32_t *ctype;             // cap + 1 cell types (neighbour type lookups)
     int32_t *medium_vol;         // n_media

@@ -18,6 +18,11 @@
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz);
 extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const cpm::PassArgs *a,
                                                hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+                                                       uint32_t capacity, uint32_t pass, uint32_t phase,
+                                                       hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_mech_init(const cpm::RepDev *reps, int slot, uint32_t capacity,
+                                                    hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_commit(const cpm::PassArgs *a, hipStream_t stream);
@@ -289,6 +294,26 @@ void compile_replica(Replica &r) {
     if (!(d.temperature >= 0.0f) || !std::isfinite(d.temperature))
         fail(CPM_E_INVALID, "temperature must be finite and non-negative");
     if (d.capacity < 1 || d.capacity > 0x7ffffffeu) fail(CPM_E_INVALID, "capacity must be 1..2^31-2");
+    // mechanics (scientific_components.jl:162-182, 592-616; component checks of init_scientific)
+    if (d.n_mechanics < 0 || d.n_mechanics > CPM_MAX_MECHANICS) fail(CPM_E_INVALID, "at most two mechanical components");
+    for (int k = 0; k < d.n_mechanics; ++k) {
+        const cpm_mechanics_desc &m = d.mechanics[k];
+        if (m.kind != CPM_MECH_PRESSURE && m.kind != CPM_MECH_TENSION) fail(CPM_E_INVALID, "unknown mechanical component");
+        if (m.initialization < CPM_MECH_INIT_CONSTITUTIVE || m.initialization > CPM_MECH_INIT_PRESERVE)
+            fail(CPM_E_INVALID, "unknown mechanical initialization");
+        if (m.instance_id < 1 || m.instance_id > 0xffffu)
+            fail(CPM_E_INVALID, "mechanical component instance ID must be positive and fit UInt16");
+        if (!(m.eta > 0.0f) || !std::isfinite(m.eta))
+            fail(CPM_E_INVALID, "mechanical relaxation rate must be finite and positive");
+        if (!m.noise_is_temperature && (!(m.noise >= 0.0f) || !std::isfinite(m.noise)))
+            fail(CPM_E_INVALID, "mechanical noise must be finite and non-negative");
+        for (int j = 0; j < k; ++j) {
+            if (d.mechanics[j].instance_id == m.instance_id)
+                fail(CPM_E_INVALID, "mechanical components need distinct instance IDs");
+            if (d.mechanics[j].kind == m.kind)
+                fail(CPM_E_INVALID, "mechanical components need distinct state properties");
+        }
+    }
 
     if (contact) {
         // every medium a contact relation can see needs a declared type (scientific_components.jl:35-40)
@@ -576,6 +601,34 @@ void compile_replica(Replica &r) {
             ok = d.proposal.offsets[k][0] == vn[k][0] && d.proposal.offsets[k][1] == vn[k][1];
         for (int c = 1; ok && c <= 4; ++c) ok = v.col_count[c] == 1;
         if (ok) r.config |= 32;
+    }
+    v.capacity = d.capacity;
+    v.n_mech = d.n_mechanics;
+    for (int k = 0; k < d.n_mechanics; ++k) {
+        const cpm_mechanics_desc &m = d.mechanics[k];
+        v.mech_kind[k] = m.kind;
+        v.mech_init[k] = m.initialization;
+        v.mech_instance[k] = m.instance_id;
+        v.mech_eta[k] = m.eta;
+        v.mech_noise[k] = m.noise_is_temperature ? d.temperature : m.noise;
+    }
+    if (d.n_mechanics > 0) {
+        // OU half intervals in Float32: colour size / (2 * mutable sites) per colour
+        // (scientific_mechanics.jl:170-182), or 1 / (2 * rounds) (lottery.jl:353)
+        if (d.algorithm == CPM_ALG_LOTTERY) {
+            v.mech_half[0] = 1.0f / (2.0f * (float)r.rounds);
+        } else {
+            if (r.rounds > cpm::kMaxRounds) fail(CPM_E_UNSUPPORTED, "too many colours for the mechanical clock");
+            std::vector<uint64_t> size((size_t)r.rounds + 1, 0);
+            uint64_t mutable_sites = 0;
+            for (int64_t i = 0; i < r.n; ++i)
+                if (r.colors[(size_t)i] > 0) {
+                    size[r.colors[(size_t)i]]++;
+                    mutable_sites++;
+                }
+            for (int c = 1; c <= r.rounds; ++c)
+                v.mech_half[c] = (float)size[c] / (2.0f * (float)mutable_sites);
+        }
     }
     v.halo = r.halo;
     {
@@ -1086,13 +1139,19 @@ struct Engine {
             if (g.ntiles == 0) continue;
             for (int p = 0; p < g.max_rounds; ++p) {
                 const cpm::PassArgs a = args(g, (uint32_t)p, (uint32_t)(g.pass_counter & 1));
+                // mechanics: half-interval OU steps before and after the pass
+                // (checkerboard.jl:522-557, lottery.jl:353-391); timed with the adjacent kernel
                 if (timing_events) {
                     mark(0);
+                    advance_mechanics(g, (uint32_t)p, 0);
                     hip_check(cpm_internal_launch_propose(g.config, algorithm, &a, stream), "propose");
                     mark(1);
                     hip_check(cpm_internal_launch_commit(&a, stream), "commit");
+                    advance_mechanics(g, (uint32_t)p, 1);
                 } else {
+                    advance_mechanics(g, (uint32_t)p, 0);
                     hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
+                    advance_mechanics(g, (uint32_t)p, 1);
                 }
                 metrics[0] += 3;
                 g.pass_counter++;
@@ -1335,6 +1394,52 @@ struct Engine {
             hip_check(hipMemcpy(mv, r->d_mvol, (size_t)r->desc.n_media * 4, hipMemcpyDeviceToHost), "download");
     }
 
+    // The Float32 mechanical state fields of the cell records (strided copies, one per column).
+    void upload_mechanics(int slot, const float *qp, const float *qt) {
+        Replica *r = check_slot(slot);
+        const size_t cap = r->desc.capacity;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        if (qp)
+            hip_check(hipMemcpy2D(&r->d_cells[1].qp, sizeof(cpm::CellRec), qp, sizeof(float), sizeof(float), cap,
+                                  hipMemcpyHostToDevice), "upload mechanics");
+        if (qt)
+            hip_check(hipMemcpy2D(&r->d_cells[1].qt, sizeof(cpm::CellRec), qt, sizeof(float), sizeof(float), cap,
+                                  hipMemcpyHostToDevice), "upload mechanics");
+        metrics[2]++;
+    }
+
+    void download_mechanics(int slot, float *qp, float *qt) {
+        Replica *r = check_slot(slot);
+        const size_t cap = r->desc.capacity;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        if (qp)
+            hip_check(hipMemcpy2D(qp, sizeof(float), &r->d_cells[1].qp, sizeof(cpm::CellRec), sizeof(float), cap,
+                                  hipMemcpyDeviceToHost), "download mechanics");
+        if (qt)
+            hip_check(hipMemcpy2D(qt, sizeof(float), &r->d_cells[1].qt, sizeof(cpm::CellRec), sizeof(float), cap,
+                                  hipMemcpyDeviceToHost), "download mechanics");
+        metrics[3]++;
+    }
+
+    void initialize_mechanics(int slot) {
+        Replica *r = check_slot(slot);
+        if (r->desc.n_mechanics == 0) return;
+        hip_check(cpm_internal_launch_mech_init(d_reps, slot, r->desc.capacity, stream), "mechanics init");
+        metrics[0]++;
+    }
+
+    // Half-interval OU steps of the replicas of a group that carry mechanics.
+    void advance_mechanics(const Group &g, uint32_t pass, uint32_t phase) {
+        for (int s : g.slots) {
+            const Replica *r = reps[s];
+            if (!r || r->desc.n_mechanics == 0) continue;
+            hip_check(cpm_internal_launch_mech_advance(d_reps, d_dyn, s, r->desc.capacity, pass, phase, stream),
+                      "mechanics");
+            metrics[0]++;
+        }
+    }
+
     void set_seed(int slot, uint64_t seed) {
         Replica *r = check_slot(slot);
         r->desc.seed = seed;
@@ -1433,6 +1538,22 @@ int32_t cpm_download_state(cpm_ctx *ctx, uint8_t *tags, uint32_t *ids, int32_t *
                                    boundary_measures, active, cell_types, target_volume,
                                    volume_strength, target_boundary, boundary_strength);
     });
+}
+
+int32_t cpm_upload_mechanics(cpm_ctx *ctx, const float *volume_pressure, const float *surface_tension) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] { ctx->engine.upload_mechanics(ctx->slot, volume_pressure, surface_tension); });
+}
+
+int32_t cpm_download_mechanics(cpm_ctx *ctx, float *volume_pressure, float *surface_tension) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err,
+                   [&] { ctx->engine.download_mechanics(ctx->slot, volume_pressure, surface_tension); });
+}
+
+int32_t cpm_initialize_mechanics(cpm_ctx *ctx) {
+    if (!ctx) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] { ctx->engine.initialize_mechanics(ctx->slot); });
 }
 
 int32_t cpm_set_seed(cpm_ctx *ctx, uint64_t seed) {
@@ -1687,6 +1808,22 @@ int32_t cpm_batch_set_target(cpm_batch *b, int32_t slot, uint64_t target_mcs) {
         b->engine.check_slot(slot);
         b->engine.set_dyn(slot, 0, target_mcs, 1, 2);
     });
+}
+
+int32_t cpm_batch_upload_mechanics(cpm_batch *b, int32_t slot, const float *volume_pressure,
+                                   const float *surface_tension) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.upload_mechanics(slot, volume_pressure, surface_tension); });
+}
+
+int32_t cpm_batch_download_mechanics(cpm_batch *b, int32_t slot, float *volume_pressure, float *surface_tension) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.download_mechanics(slot, volume_pressure, surface_tension); });
+}
+
+int32_t cpm_batch_initialize_mechanics(cpm_batch *b, int32_t slot) {
+    if (!b) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] { b->engine.initialize_mechanics(slot); });
 }
 
 int32_t cpm_batch_set_mcs(cpm_batch *b, int32_t slot, uint64_t mcs) {

@@ -1047,6 +1047,16 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
     // slot 0 of every per-cell array is a spare, read in place of a medium owner
     const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
     const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
+    // mechanical state of both cells (volume_pressure, surface_tension)
+    float qpl = 0.0f, qtl = 0.0f, qpg = 0.0f, qtg = 0.0f;
+    if (d.n_mech > 0) {
+        const v4i ml = ((const CPM_GLOBAL v4i *)(gp(d.cells) + li))[3];
+        const v4i mg = ((const CPM_GLOBAL v4i *)(gp(d.cells) + gi))[3];
+        qpl = __int_as_float(ml.x);
+        qtl = __int_as_float(ml.y);
+        qpg = __int_as_float(mg.x);
+        qtg = __int_as_float(mg.y);
+    }
     const int32_t vl0 = cl.vol, vg0 = cg.vol;
     const double vs_l = cl.vs, tv_l = cl.tv, vs_g = cg.vs, tv_g = cg.tv;
     const double bs_l = cl.bs, bs_g = cg.bs;
@@ -1176,8 +1186,22 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
         }
     }
     const float dh = (float)acc;
+    // mechanical work folded in tuple order (scientific_inner_loop.jl:238-242;
+    // scientific_components.jl:220-231, 646-657), then added to delta_h
+    float mech = 0.0f;
+    for (int k = 0; k < d.n_mech; ++k) {
+        float r = 0.0f;
+        if (d.mech_kind[k] == CPM_MECH_PRESSURE) {
+            if (l > 0) r = __fsub_rn(r, qpl);
+            if (g > 0) r = __fadd_rn(r, qpg);
+        } else {
+            if (l > 0) r = __fadd_rn(r, __fmul_rn(qtl, (float)dl));
+            if (g > 0) r = __fadd_rn(r, __fmul_rn(qtg, (float)dg));
+        }
+        mech = __fadd_rn(mech, r);
+    }
     const float T = d.temperature;
-    const float dd = __fadd_rn(dh, 0.0f);
+    const float dd = __fadd_rn(dh, mech);
     float p;
     if (T == 0.0f) {
         p = __fadd_rn(dd, 0.0f) <= 0.0f ? 1.0f : 0.0f;
@@ -1209,6 +1233,8 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &
             cells[l].vs = d.retire_vs;
             cells[l].tb = d.retire_tb;
             cells[l].bs = d.retire_bs;
+            cells[l].qp = 0.0f;
+            cells[l].qt = 0.0f;
         }
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
@@ -1360,6 +1386,111 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
         slot[4] = s_cnt[lane][0];
         slot[5] = s_cnt[lane][1];
         slot[6] = s_cnt[lane][2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mechanics (lib/CorePotts/src/components/scientific_mechanics.jl)
+// ---------------------------------------------------------------------------
+
+// Correctly rounded Float32 square root: the double-precision root rounded once (the
+// double has more than 2 * 24 + 2 significand bits, so the second rounding is exact for
+// square roots). The single-precision hardware root is accurate to 1 ulp only.
+__device__ __forceinline__ float sqrt_rn(float x) { return (float)sqrt((double)x); }
+
+// normal_box_muller(Float32) (rng/semantic.jl:323-331): lanes 1 and 2 of one Philox call;
+// log and cos are evaluated in double precision and rounded once to Float32.
+__device__ inline float normal_box_muller(uint32_t key_lo, uint32_t key_hi, uint32_t stream, uint64_t mcs,
+                                          uint32_t subround, uint32_t operation, uint32_t entity,
+                                          uint32_t invocation) {
+    const Words w = rng_words(key_lo, key_hi, stream, mcs, subround, operation, kCellEntity, entity, invocation, 0);
+    const float u1 = uniform_open01(w.w0), u2 = uniform_open01(w.w1);
+    const float radius = sqrt_rn(__fmul_rn(-2.0f, (float)log((double)u1)));
+    const float angle = __fmul_rn(__fmul_rn(2.0f, 3.14159274101257324f), u2);  // 2 Float32(pi)
+    return __fmul_rn(radius, (float)cos((double)angle));
+}
+
+// _mechanical_ou_transition (scientific_mechanics.jl:29-35), Float32 in the reference's
+// operation order; exp in double precision rounded once.
+__device__ inline float ou_transition(float q, float disp, float strength, float eta, float noise, float interval,
+                                      float normal) {
+    const float alpha = (float)exp((double)__fmul_rn(-eta, interval));
+    const float two_s = __fmul_rn(2.0f, strength);
+    const float mean = __fmul_rn(__fsub_rn(1.0f, alpha), __fmul_rn(two_s, disp));
+    const float var = __fmul_rn(__fmul_rn(two_s, noise), fmaxf(0.0f, __fsub_rn(1.0f, __fmul_rn(alpha, alpha))));
+    return __fadd_rn(__fadd_rn(__fmul_rn(alpha, q), mean), __fmul_rn(sqrt_rn(var), normal));
+}
+
+// Observable minus target, and strength, of mechanical component k for one cell
+// (_mechanical_mean_inputs, scientific_mechanics.jl:47-56): finite volume against
+// target_volume / volume_strength, or boundary measure against target_boundary /
+// boundary_strength, converted to Float32.
+__device__ inline void mech_inputs(const RepDev &d, int k, const CPM_GLOBAL CellRec *c, float *disp, float *strength) {
+    if (d.mech_kind[k] == CPM_MECH_PRESSURE) {
+        *disp = __fsub_rn((float)c->vol, (float)c->tv);
+        *strength = (float)c->vs;
+    } else {
+        *disp = __fsub_rn((float)c->bnd, (float)c->tb);
+        *strength = (float)c->bs;
+    }
+}
+
+// One half-interval OU step of every mechanical state of every cell of one replica,
+// before (phase 0) or after (phase 1) pass `pass` (_advance_checkerboard_mechanics!,
+// scientific_mechanics.jl:170-196; lottery.jl:353-391 through _advance_mechanics!).
+__global__ __launch_bounds__(kBlock) void mech_advance_kernel(const RepDev *reps, const RepDyn *dyn, int slot,
+                                                              uint32_t pass, uint32_t phase) {
+    const RepDev &d = reps[slot];
+    const RepDyn &y = dyn[slot];
+    if (!y.active || pass >= (uint32_t)d.rounds) return;
+    const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
+    if (cell > d.capacity) return;
+    const float half = d.algorithm == CPM_ALG_LOTTERY ? d.mech_half[0] : d.mech_half[y.order[pass]];
+    CPM_GLOBAL CellRec *c = gp(d.cells) + cell;
+    const bool active = c->active != 0;
+    for (int k = 0; k < d.n_mech; ++k) {
+        const bool pressure = d.mech_kind[k] == CPM_MECH_PRESSURE;
+        float q = 0.0f;
+        if (active) {
+            float disp, strength;
+            mech_inputs(d, k, c, &disp, &strength);
+            const float normal = normal_box_muller(d.key_lo, d.key_hi, kStreamAuxEvolution, y.mcs, pass,
+                                                   d.mech_instance[k], cell, phase);
+            q = ou_transition(pressure ? c->qp : c->qt, disp, strength, d.mech_eta[k], d.mech_noise[k], half,
+                              normal);
+        }
+        if (pressure) c->qp = q;
+        else c->qt = q;
+    }
+}
+
+// _initialize_mechanics! (scientific_mechanics.jl:63-103): inactive cells hold zero;
+// constitutive mean 2 strength displacement, or the stationary draw around it.
+__global__ __launch_bounds__(kBlock) void mech_init_kernel(const RepDev *reps, int slot) {
+    const RepDev &d = reps[slot];
+    const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
+    if (cell > d.capacity) return;
+    CPM_GLOBAL CellRec *c = gp(d.cells) + cell;
+    const bool active = c->active != 0;
+    for (int k = 0; k < d.n_mech; ++k) {
+        const bool pressure = d.mech_kind[k] == CPM_MECH_PRESSURE;
+        float q = pressure ? c->qp : c->qt;
+        if (!active) {
+            q = 0.0f;
+        } else if (d.mech_init[k] != CPM_MECH_INIT_PRESERVE) {
+            float disp, strength;
+            mech_inputs(d, k, c, &disp, &strength);
+            const float mean = __fmul_rn(__fmul_rn(2.0f, strength), disp);
+            q = mean;
+            if (d.mech_init[k] == CPM_MECH_INIT_STATIONARY) {
+                const float normal = normal_box_muller(d.key_lo, d.key_hi, kStreamAuxInitialization, 0, 0,
+                                                       d.mech_instance[k], cell, 0);
+                q = __fadd_rn(mean, __fmul_rn(sqrt_rn(__fmul_rn(__fmul_rn(2.0f, strength), d.mech_noise[k])),
+                                              normal));
+            }
+        }
+        if (pressure) c->qp = q;
+        else c->qt = q;
     }
 }
 
@@ -1530,6 +1661,24 @@ extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::Re
                                                 int nslots, hipStream_t stream) {
     const int blocks = (nslots + 63) / 64;
     hipLaunchKernelGGL(cpm::begin_mcs_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, nslots);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+                                                       uint32_t capacity, uint32_t pass, uint32_t phase,
+                                                       hipStream_t stream) {
+    const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(cpm::mech_advance_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot, pass,
+                       phase);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cpm_internal_launch_mech_init(const cpm::RepDev *reps, int slot, uint32_t capacity,
+                                                    hipStream_t stream) {
+    const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(cpm::mech_init_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, slot);
     return hipGetLastError();
 }
 

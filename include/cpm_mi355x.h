@@ -30,6 +30,10 @@
  *                       workspace.round_count/round_order (lottery.jl:2-17)
  *   cpm_ensemble_seed   ensemble_seed(EnsembleSeedDerivationV1(), master, trajectory, rerun)
  *                       lib/CorePotts/src/sciml/interface.jl:1009-1026
+ *   cpm_*_mechanics     mechanical state columns and _initialize_mechanics!
+ *                       lib/CorePotts/src/components/scientific_mechanics.jl:63-103; the
+ *                       half-interval OU steps around every colour pass / lottery round
+ *                       (checkerboard.jl:521-558, lottery.jl:353-391) run inside cpm_step
  *   cpm_batch_*         no single counterpart: many independent integrators (the reference's
  *                       EnsembleProblem trajectories, interface.jl:1059-1084) advanced by one
  *                       fused launch per colour pass.
@@ -64,6 +68,15 @@ enum cpm_status {
 enum cpm_face { CPM_FACE_PERIODIC = 0, CPM_FACE_CLOSED = 1, CPM_FACE_FIXED = 2 };
 enum cpm_term { CPM_TERM_VOLUME = 0, CPM_TERM_CONTACT = 1, CPM_TERM_BOUNDARY = 2 };
 enum cpm_algorithm { CPM_ALG_CHECKERBOARD = 1, CPM_ALG_LOTTERY = 2 };
+/* Mechanical components (lib/CorePotts/src/components/scientific_components.jl:130-240,
+ * 575-660) and their initialization (MechanicalInitialization, :130-134). */
+enum cpm_mechanics { CPM_MECH_PRESSURE = 1, CPM_MECH_TENSION = 2 };
+enum cpm_mech_init {
+    CPM_MECH_INIT_CONSTITUTIVE = 1,  /* ConstitutiveMeanInitialization */
+    CPM_MECH_INIT_STATIONARY = 2,    /* StationaryMechanicalInitialization */
+    CPM_MECH_INIT_PRESERVE = 3       /* PreserveMechanicalInitialization */
+};
+#define CPM_MAX_MECHANICS 2
 
 /* StaticCartesianRelation (lib/CorePotts/src/spatial/cartesian.jl:252-260), already in
  * canonical direction order; weights hold the values of the relation's eltype. */
@@ -72,6 +85,19 @@ typedef struct cpm_relation_desc {
     int32_t offsets[CPM_MAX_DIRS][3];
     double weights[CPM_MAX_DIRS];
 } cpm_relation_desc;
+
+/* FluctuatingVolumePressure (state column volume_pressure, observable finite volume,
+ * target_volume / volume_strength) or FluctuatingSurfaceTension (state column
+ * surface_tension, observable boundary measure, target_boundary / boundary_strength),
+ * Float32. scientific_components.jl:152-182, 576-616. */
+typedef struct cpm_mechanics_desc {
+    int32_t kind;                  /* CPM_MECH_* */
+    int32_t initialization;        /* CPM_MECH_INIT_* */
+    uint32_t instance_id;          /* RNG operation of the component's draws (1..65535) */
+    float eta;                     /* relaxation rate, finite and positive */
+    int32_t noise_is_temperature;  /* AlgorithmTemperatureNoise: the algorithm temperature */
+    float noise;                   /* FixedMechanicalNoise value (non-negative) otherwise */
+} cpm_mechanics_desc;
 
 /* Everything init_scientific receives besides the state arrays. */
 typedef struct cpm_model_desc {
@@ -110,6 +136,8 @@ typedef struct cpm_model_desc {
     double retire_volume_strength;
     int64_t retire_target_boundary;
     double retire_boundary_strength;
+    int32_t n_mechanics;           /* ScientificComponentSet.mechanics, tuple order */
+    cpm_mechanics_desc mechanics[CPM_MAX_MECHANICS];
 } cpm_model_desc;
 
 typedef struct cpm_ctx cpm_ctx;
@@ -134,6 +162,15 @@ int32_t cpm_download_state(cpm_ctx *ctx, uint8_t *tags, uint32_t *ids, int32_t *
                            int32_t *medium_volumes, int64_t *boundary_measures, uint8_t *active,
                            uint32_t *cell_types, double *target_volume, double *volume_strength,
                            int64_t *target_boundary, double *boundary_strength);
+
+/* Mechanical state columns (capacity Float32 each; NULL for a component the model lacks).
+ * The property columns volume_pressure and surface_tension of the compiled state. */
+int32_t cpm_upload_mechanics(cpm_ctx *ctx, const float *volume_pressure, const float *surface_tension);
+/* Device -> host copy (synchronizes). */
+int32_t cpm_download_mechanics(cpm_ctx *ctx, float *volume_pressure, float *surface_tension);
+/* _initialize_mechanics! (lib/CorePotts/src/components/scientific_mechanics.jl:63-103) on the
+ * uploaded state: init_scientific with initialize_mechanics = true. */
+int32_t cpm_initialize_mechanics(cpm_ctx *ctx);
 
 int32_t cpm_set_seed(cpm_ctx *ctx, uint64_t seed);
 int32_t cpm_set_mcs(cpm_ctx *ctx, uint64_t mcs);
@@ -223,6 +260,11 @@ int32_t cpm_batch_download_state(cpm_batch *b, int32_t slot, uint8_t *tags, uint
                                  uint32_t *cell_types, double *target_volume,
                                  double *volume_strength, int64_t *target_boundary,
                                  double *boundary_strength);
+int32_t cpm_batch_upload_mechanics(cpm_batch *b, int32_t slot, const float *volume_pressure,
+                                   const float *surface_tension);
+int32_t cpm_batch_download_mechanics(cpm_batch *b, int32_t slot, float *volume_pressure,
+                                     float *surface_tension);
+int32_t cpm_batch_initialize_mechanics(cpm_batch *b, int32_t slot);
 /* Set a replica's MCS counter (checkpoint restore; integrator.mcs, sequential.jl:216-234);
  * its target becomes the same value, so it stays idle until the next cpm_batch_set_target. */
 int32_t cpm_batch_set_mcs(cpm_batch *b, int32_t slot, uint64_t mcs);

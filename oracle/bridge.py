@@ -38,6 +38,9 @@ class OrcModel(C.Structure):
         ("capacity", C.c_int32), ("retire_target_volume", C.c_double),
         ("retire_volume_strength", C.c_double), ("retire_target_boundary", C.c_int64),
         ("retire_boundary_strength", C.c_double),
+        ("n_mech", C.c_int32), ("mech_kind", C.c_int32 * 2), ("mech_init", C.c_int32 * 2),
+        ("mech_instance", C.c_uint32 * 2), ("mech_eta", C.c_float * 2),
+        ("mech_noise_is_temperature", C.c_int32 * 2), ("mech_noise", C.c_float * 2),
     ]
 
 
@@ -45,7 +48,7 @@ class OrcState(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in (
         "tags", "ids", "finite_volumes", "medium_volumes", "boundary_measures", "active",
         "cell_types", "target_volume", "volume_strength", "target_boundary",
-        "boundary_strength")]
+        "boundary_strength", "volume_pressure", "surface_tension")]
 
 
 _lib = None
@@ -104,6 +107,14 @@ def load() -> C.CDLL:
     lib.orc_claims.argtypes = [C.c_int32, P, P, P, P, C.c_int32, P, P, P, P]
     lib.orc_sample_transitions.restype = C.c_int32
     lib.orc_sample_transitions.argtypes = [P, U64, U64, P, C.c_int32, P]
+    lib.orc_copy_evaluation.restype = C.c_int32
+    lib.orc_copy_evaluation.argtypes = [P, U32, C.c_int32, P]
+    lib.orc_initialize_mechanics.argtypes = [P]
+    lib.orc_advance_mechanics.argtypes = [P, U64, U32, U32, C.c_float]
+    lib.orc_normal_box_muller.restype = C.c_float
+    lib.orc_normal_box_muller.argtypes = [U64, U32, U64, U32, U32, U32, U32]
+    lib.orc_ou_transition.restype = C.c_float
+    lib.orc_ou_transition.argtypes = [C.c_float] * 7
     lib.orc_total_energy.restype = C.c_double
     lib.orc_total_energy.argtypes = [P]
     _lib = lib
@@ -160,12 +171,17 @@ def make_model(spec, keep: list) -> OrcModel:
     m.capacity = spec.capacity
     m.retire_target_volume, m.retire_volume_strength = spec.retire[0], spec.retire[1]
     m.retire_target_boundary, m.retire_boundary_strength = spec.retire[2], spec.retire[3]
+    m.n_mech = len(spec.mechanics)
+    for k, (kind, init, instance, eta, noise_is_temperature, noise) in enumerate(spec.mechanics):
+        m.mech_kind[k], m.mech_init[k], m.mech_instance[k] = kind, init, instance
+        m.mech_eta[k], m.mech_noise_is_temperature[k], m.mech_noise[k] = float(eta), int(noise_is_temperature), float(noise)
     return m
 
 
 STATE_FIELDS = ("tags", "ids", "finite_volumes", "medium_volumes", "boundary_measures", "active",
                 "cell_types", "target_volume", "volume_strength", "target_boundary",
                 "boundary_strength")
+MECH_FIELDS = ("volume_pressure", "surface_tension")   # mechanics kinds 1 and 2
 
 
 class OracleSim:
@@ -192,8 +208,9 @@ class OracleSim:
 
     def _state_struct(self, arrays: dict) -> OrcState:
         s = OrcState()
-        for name in STATE_FIELDS:
-            setattr(s, name, arrays[name].ctypes.data)
+        for name in STATE_FIELDS + MECH_FIELDS:
+            a = arrays.get(name)
+            setattr(s, name, None if a is None else a.ctypes.data)
         return s
 
     def load(self, arrays: dict):
@@ -214,9 +231,26 @@ class OracleSim:
                    cell_types=np.zeros(cap, np.uint32), target_volume=np.zeros(cap, np.float64),
                    volume_strength=np.zeros(cap, np.float64), target_boundary=np.zeros(cap, np.int64),
                    boundary_strength=np.zeros(cap, np.float64))
+        for name in self._mech_keys():
+            out[name] = np.zeros(cap, np.float32)
         self.lib.orc_store_state(self._sim, C.byref(self._state_struct(out)))
         out["medium_volumes"] = out["medium_volumes"][:len(self.spec.medium_ids)]
         return out
+
+    def _mech_keys(self):
+        return [MECH_FIELDS[kind - 1] for kind, *_ in self.spec.mechanics]
+
+    def copy_evaluation(self, recipient: int, direction: int):
+        """(outcome, delta_h, mechanical_work) of one proposal."""
+        out = np.zeros(2, np.float32)
+        rc = self.lib.orc_copy_evaluation(self._sim, int(recipient), int(direction), out.ctypes.data)
+        return int(rc), out[0], out[1]
+
+    def initialize_mechanics(self):
+        self.lib.orc_initialize_mechanics(self._sim)
+
+    def advance_mechanics(self, mcs: int, subround: int, phase: int, interval: float):
+        self.lib.orc_advance_mechanics(self._sim, int(mcs), int(subround), int(phase), float(interval))
 
     def step(self, n: int = 1):
         self.lib.orc_step(self._sim, int(n))
@@ -281,6 +315,15 @@ def coloring(spec) -> np.ndarray:
     if n < 0:
         raise RuntimeError("colouring exceeds 255 colours")
     return colors
+
+
+def normal_box_muller(seed, stream, mcs, subround, operation, entity, invocation) -> np.float32:
+    return np.float32(load().orc_normal_box_muller(int(seed), int(stream), int(mcs), int(subround), int(operation),
+                                                   int(entity), int(invocation)))
+
+
+def ou_transition(*args) -> np.float32:
+    return np.float32(load().orc_ou_transition(*[float(a) for a in args]))
 
 
 def rng_words(seed, stream, mcs=0, subround=0, operation=0, kind=0, entity=0, generation=0,

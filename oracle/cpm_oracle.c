@@ -104,9 +104,10 @@ uint64_t orc_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t r
 
 enum {
     S_RECIPIENT = 3, S_DIRECTION = 4, S_ACCEPTANCE = 5, S_LOTTERY_ACTIVATION = 6,
-    S_LOTTERY_PRIORITY = 7, S_CHECKERBOARD_ORDER = 8, S_CHECKERBOARD_PRIORITY = 17
+    S_LOTTERY_PRIORITY = 7, S_CHECKERBOARD_ORDER = 8, S_CHECKERBOARD_PRIORITY = 17,
+    S_AUX_EVOLUTION = 9, S_AUX_INITIALIZATION = 18
 };
-enum { K_GLOBAL = 0, K_SITE = 1 };
+enum { K_GLOBAL = 0, K_SITE = 1, K_CELL = 2 };
 
 /* ------------------------------------------------------------------------- */
 /* Domain and neighbour realization: spatial/cartesian.jl:415-525             */
@@ -368,6 +369,8 @@ orc_sim *orc_create(const orc_model *model, int32_t algorithm, uint64_t seed) {
     s->st.boundary_measures = (int64_t *)xcalloc((size_t)cap, 8);
     s->st.active = (uint8_t *)xcalloc((size_t)cap, 1);
     s->st.cell_types = (uint32_t *)xcalloc((size_t)cap, 4);
+    s->st.volume_pressure = (float *)xcalloc((size_t)cap, 4);
+    s->st.surface_tension = (float *)xcalloc((size_t)cap, 4);
     s->st.target_volume = (double *)xcalloc((size_t)cap, 8);
     s->st.volume_strength = (double *)xcalloc((size_t)cap, 8);
     s->st.target_boundary = (int64_t *)xcalloc((size_t)cap, 8);
@@ -453,6 +456,7 @@ void orc_destroy(orc_sim *s) {
     free(s->st.boundary_measures); free(s->st.active); free(s->st.cell_types);
     free(s->st.target_volume); free(s->st.volume_strength); free(s->st.target_boundary);
     free(s->st.boundary_strength);
+    free(s->st.volume_pressure); free(s->st.surface_tension);
     free(s->att_losing); free(s->att_gaining); free(s->att_recipient); free(s->att_priority);
     free(s->att_outcome); free(s->att_selected); free(s->att_disp); free(s->att_dl);
     free(s->att_dg); free(s->cell_max_priority); free(s->cell_min_identity); free(s->order);
@@ -492,6 +496,10 @@ static void copy_state(const orc_model *m, int64_t n, const orc_state *src, orc_
     memcpy(dst->volume_strength, src->volume_strength, cap * 8);
     memcpy(dst->target_boundary, src->target_boundary, cap * 8);
     memcpy(dst->boundary_strength, src->boundary_strength, cap * 8);
+    if (src->volume_pressure && dst->volume_pressure)
+        memcpy(dst->volume_pressure, src->volume_pressure, cap * 4);
+    if (src->surface_tension && dst->surface_tension)
+        memcpy(dst->surface_tension, src->surface_tension, cap * 4);
 }
 
 int32_t orc_load_state(orc_sim *s, const orc_state *src) {
@@ -708,10 +716,118 @@ static float delta_h(const orc_sim *s, uint32_t recipient, int64_t losing, int64
     return (float)acc;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Mechanics: scientific_mechanics.jl, scientific_components.jl:220-231,646-657 */
+/* ------------------------------------------------------------------------- */
+
+/* rng/semantic.jl:323-331: lanes 1 and 2 of one Philox call; log and cos in double
+ * precision, rounded once to Float32. */
+float orc_normal_box_muller(uint64_t seed, uint32_t stream, uint64_t mcs, uint32_t subround,
+                            uint32_t operation, uint32_t entity, uint32_t invocation) {
+    uint32_t w[4];
+    orc_rng_words(seed, stream, mcs, subround, operation, K_CELL, entity, 0, invocation, 0, w);
+    float u1 = orc_uniform_open01_f32(w[0]);
+    float u2 = orc_uniform_open01_f32(w[1]);
+    float radius = sqrtf(-2.0f * (float)log((double)u1));
+    float angle = (2.0f * 3.14159274101257324f) * u2; /* 2 * Float32(pi) */
+    return radius * (float)cos((double)angle);
+}
+
+/* scientific_mechanics.jl:29-35, every operation in Float32 */
+float orc_ou_transition(float q, float displacement, float strength, float eta, float noise,
+                        float interval, float standard_normal) {
+    float alpha = (float)exp((double)(-eta * interval));
+    float two_s = 2.0f * strength;
+    float mean = (1.0f - alpha) * (two_s * displacement);
+    float rest = 1.0f - alpha * alpha;
+    float variance = (two_s * noise) * (rest > 0.0f ? rest : 0.0f);
+    return (alpha * q + mean) + sqrtf(variance) * standard_normal;
+}
+
+static float mech_noise(const orc_sim *s, int k) {
+    return s->m.mech_noise_is_temperature[k] ? s->m.temperature : s->m.mech_noise[k];
+}
+
+static float *mech_column(orc_sim *s, int k) {
+    return s->m.mech_kind[k] == ORC_MECH_PRESSURE ? s->st.volume_pressure : s->st.surface_tension;
+}
+
+/* _mechanical_mean_inputs (scientific_mechanics.jl:47-56) */
+static void mech_inputs(const orc_sim *s, int k, int64_t i, float *displacement, float *strength) {
+    const orc_state *st = &s->st;
+    if (s->m.mech_kind[k] == ORC_MECH_PRESSURE) {
+        *displacement = (float)st->finite_volumes[i] - (float)st->target_volume[i];
+        *strength = (float)st->volume_strength[i];
+    } else {
+        *displacement = (float)st->boundary_measures[i] - (float)st->target_boundary[i];
+        *strength = (float)st->boundary_strength[i];
+    }
+}
+
+/* scientific_mechanics.jl:63-103 */
+void orc_initialize_mechanics(orc_sim *s) {
+    for (int k = 0; k < s->m.n_mech; ++k) {
+        float *q = mech_column(s, k);
+        for (int64_t i = 0; i < s->m.capacity; ++i) {
+            if (!s->st.active[i]) {
+                q[i] = 0.0f;
+                continue;
+            }
+            if (s->m.mech_init[k] == ORC_MECH_INIT_PRESERVE) continue;
+            float displacement, strength;
+            mech_inputs(s, k, i, &displacement, &strength);
+            float mean = (2.0f * strength) * displacement;
+            if (s->m.mech_init[k] == ORC_MECH_INIT_CONSTITUTIVE) {
+                q[i] = mean;
+            } else {
+                float normal = orc_normal_box_muller(s->seed, S_AUX_INITIALIZATION, 0, 0,
+                                                     s->m.mech_instance[k], (uint32_t)(i + 1), 0);
+                q[i] = mean + sqrtf((2.0f * strength) * mech_noise(s, k)) * normal;
+            }
+        }
+    }
+}
+
+/* scientific_mechanics.jl:135-168 */
+void orc_advance_mechanics(orc_sim *s, uint64_t mcs, uint32_t subround, uint32_t phase, float interval) {
+    for (int k = 0; k < s->m.n_mech; ++k) {
+        float *q = mech_column(s, k);
+        for (int64_t i = 0; i < s->m.capacity; ++i) {
+            if (!s->st.active[i]) {
+                q[i] = 0.0f;
+                continue;
+            }
+            float displacement, strength;
+            mech_inputs(s, k, i, &displacement, &strength);
+            float normal = orc_normal_box_muller(s->seed, S_AUX_EVOLUTION, mcs, subround,
+                                                 s->m.mech_instance[k], (uint32_t)(i + 1), phase);
+            q[i] = orc_ou_transition(q[i], displacement, strength, s->m.mech_eta[k], mech_noise(s, k),
+                                     interval, normal);
+        }
+    }
+}
+
+/* mechanical_work of every component folded in tuple order (scientific_inner_loop.jl:238-242) */
+static float mechanical_work(const orc_sim *s, int64_t losing, int64_t gaining, int64_t dl, int64_t dg) {
+    float total = 0.0f;
+    for (int k = 0; k < s->m.n_mech; ++k) {
+        float r = 0.0f;
+        if (s->m.mech_kind[k] == ORC_MECH_PRESSURE) {
+            if (losing > 0) r -= s->st.volume_pressure[losing - 1];
+            if (gaining > 0) r += s->st.volume_pressure[gaining - 1];
+        } else {
+            if (losing > 0) r += s->st.surface_tension[losing - 1] * (float)dl;
+            if (gaining > 0) r += s->st.surface_tension[gaining - 1] * (float)dg;
+        }
+        total += r;
+    }
+    return total;
+}
+
 /* protocols/scientific.jl:372-385 with zero drive, kinetic and yield terms.
  * exp is evaluated in double precision and rounded once to float. */
-static float acceptance_probability(float dh, float temperature) {
-    volatile float d = dh + 0.0f; /* delta_h + mechanical_work */
+static float acceptance_probability(float dh, float mech, float temperature) {
+    volatile float d = dh + mech; /* delta_h + mechanical_work */
     if (temperature == 0.0f) {
         float eff = d + 0.0f;
         return eff <= 0.0f ? 1.0f : 0.0f;
@@ -740,6 +856,8 @@ static void commit(orc_sim *s, uint32_t recipient, int64_t losing, int64_t gaini
             st->volume_strength[i] = s->m.retire_volume_strength;
             st->target_boundary[i] = s->m.retire_target_boundary;
             st->boundary_strength[i] = s->m.retire_boundary_strength;
+            st->volume_pressure[i] = 0.0f;
+            st->surface_tension[i] = 0.0f;
         }
     } else {
         st->medium_volumes[medium_index(&s->m, losing)] -= 1;
@@ -835,7 +953,7 @@ static void evaluate_selected(orc_sim *s, uint32_t a, uint64_t mcs, uint32_t sub
     s->att_dl[a] = dl;
     s->att_dg[a] = dg;
     float dh = delta_h(s, rec, l, g, dl, dg);
-    float p = acceptance_probability(dh, s->m.temperature);
+    float p = acceptance_probability(dh, mechanical_work(s, l, g, dl, dg), s->m.temperature);
     uint32_t w[4];
     orc_rng_words(s->seed, S_ACCEPTANCE, mcs, subround, 0, K_SITE, rec, 0, 0, 0, w);
     float u = orc_uniform_open01_f32(w[0]);
@@ -852,6 +970,9 @@ static void checkerboard_mcs(orc_sim *s) {
     for (int32_t ordinal = 0; ordinal < s->color_count; ++ordinal) {
         int32_t color = (int32_t)s->order[ordinal];
         uint32_t first = s->color_offsets[color - 1], stop = s->color_offsets[color];
+        /* half interval of the colour (scientific_mechanics.jl:175-178) */
+        float half = (float)(stop - first) / (2.0f * (float)s->n_mutable);
+        orc_advance_mechanics(s, mcs, (uint32_t)ordinal, 0, half);
         for (int64_t c = 0; c <= s->m.capacity; ++c) {
             s->cell_max_priority[c] = 0;
             s->cell_min_identity[c] = 0xffffffffu;
@@ -889,6 +1010,7 @@ static void checkerboard_mcs(orc_sim *s) {
             if (s->att_disp[k] == D_ACCEPTED)
                 commit(s, s->att_recipient[k], s->att_losing[k], s->att_gaining[k], s->att_dl[k],
                        s->att_dg[k]);
+        orc_advance_mechanics(s, mcs, (uint32_t)ordinal, 1, half);
     }
     /* checkerboard.jl:459-505 */
     for (int64_t k = 0; k < s->n_mutable; ++k) {
@@ -929,9 +1051,11 @@ static void lottery_mcs(orc_sim *s) {
     uint64_t acc[9] = {0};
     uint8_t *use = (uint8_t *)xcalloc((size_t)nm, 1);
     memset(s->activations, 0, (size_t)nm * sizeof(uint32_t));
+    float half = 1.0f / (2.0f * (float)R); /* lottery.jl:353 */
     for (int32_t ordinal = 0; ordinal < R; ++ordinal) {
         uint32_t round = s->order[ordinal];
         uint32_t sub = round - 1;
+        orc_advance_mechanics(s, mcs, (uint32_t)ordinal, 0, half);
         for (int64_t c = 0; c <= s->m.capacity; ++c) {
             s->cell_max_priority[c] = 0;
             s->cell_min_identity[c] = 0xffffffffu;
@@ -985,6 +1109,7 @@ static void lottery_mcs(orc_sim *s) {
             if (s->att_selected[i] && s->att_disp[i] == D_ACCEPTED)
                 commit(s, s->att_recipient[i], s->att_losing[i], s->att_gaining[i], s->att_dl[i],
                        s->att_dg[i]);
+        orc_advance_mechanics(s, mcs, (uint32_t)ordinal, 1, half);
     }
     free(use);
     uint64_t *r = s->report;
@@ -1001,6 +1126,7 @@ static void sequential_mcs(orc_sim *s) {
     int64_t nm = s->n_mutable;
     uint32_t K = (uint32_t)s->m.proposal.n;
     uint64_t realized = 0, same = 0, boundary = 0, immutable = 0, rejected = 0, accepted = 0;
+    orc_advance_mechanics(s, mcs, 0, 0, 0.5f); /* sequential.jl:578-579 */
     for (int64_t raw = 1; raw <= nm; ++raw) {
         uint32_t id = (uint32_t)raw;
         uint32_t mi = orc_bounded_uint(s->seed, S_RECIPIENT, mcs, 0, 0, K_GLOBAL, id, 0,
@@ -1017,7 +1143,7 @@ static void sequential_mcs(orc_sim *s) {
         int64_t dl, dg;
         boundary_change(s, rec, l, g, &dl, &dg);
         float dh = delta_h(s, rec, l, g, dl, dg);
-        float p = acceptance_probability(dh, s->m.temperature);
+        float p = acceptance_probability(dh, mechanical_work(s, l, g, dl, dg), s->m.temperature);
         uint32_t w[4];
         orc_rng_words(s->seed, S_ACCEPTANCE, mcs, 0, 0, K_GLOBAL, id, 0, 0, 0, w);
         float u = orc_uniform_open01_f32(w[0]);
@@ -1028,12 +1154,27 @@ static void sequential_mcs(orc_sim *s) {
             rejected++;
         }
     }
+    orc_advance_mechanics(s, mcs, 0, 1, 0.5f);
     uint64_t *r = s->report;
     r[0] = mcs; r[1] = 1; r[2] = (uint64_t)nm; r[3] = (uint64_t)nm; r[4] = realized;
     r[5] = same; r[6] = boundary; r[7] = immutable; r[8] = 0; r[9] = 0; r[10] = rejected;
     r[11] = accepted;
     s->has_report = 1;
     s->mcs = mcs;
+}
+
+/* evaluate_copy + acceptance_inputs for one proposal (recipient, 1-based direction):
+ * out = {delta_h, mechanical_work}; returns the attempt outcome (1 = actionable). */
+int32_t orc_copy_evaluation(const orc_sim *s, uint32_t recipient, int32_t direction, float out[2]) {
+    int64_t l, g, dl = 0, dg = 0;
+    uint32_t donor;
+    int outcome = construct_attempt(s, recipient, direction, &l, &g, &donor);
+    out[0] = out[1] = 0.0f;
+    if (outcome != OUT_ACTIONABLE) return outcome;
+    boundary_change(s, recipient, l, g, &dl, &dg);
+    out[0] = delta_h(s, recipient, l, g, dl, dg);
+    out[1] = mechanical_work(s, l, g, dl, dg);
+    return outcome;
 }
 
 int32_t orc_step(orc_sim *s, uint64_t n) {

@@ -19,6 +19,9 @@
  *   lib/CorePotts/src/algorithms/lottery.jl                lottery MCS
  *   lib/CorePotts/src/algorithms/sequential.jl:474-596     sequential MCS
  *   lib/CorePotts/src/sciml/interface.jl:1009-1026         ensemble_seed
+ *   lib/CorePotts/src/components/scientific_mechanics.jl   mechanical OU transition, init,
+ *   lib/CorePotts/src/components/scientific_components.jl  advance; mechanical work
+ *     :152-240, 576-660;  rng/semantic.jl:323-331          normal_box_muller(Float32)
  *
  * Pinning (see tests/test_oracle_*.py): Philox known answers and the frozen address
  * vector of lib/CorePotts/test/test_rng_semantics.jl:15-37; the literal conflict-claim
@@ -30,7 +33,10 @@
  * invariants, and by brute-force checks on tiny inputs.
  * Parity unpinned: Julia's Base.exp(::Float32) is not available here. The oracle
  * evaluates exp in double precision and rounds once to float (a correctly rounded
- * exp except at exact ties); Julia's result is within 1 ulp of that value.
+ * exp except at exact ties); Julia's result is within 1 ulp of that value. The same
+ * holds for the Float32 exp, log and cos of the mechanics (OU factor, Box-Muller); the
+ * mechanics are pinned by the reference's closed-form OU value, its one-MCS clock test
+ * and its kernel-moment test (lib/CorePotts/test/test_scientific_mechanics.jl).
  */
 #ifndef CPM_ORACLE_H
 #define CPM_ORACLE_H
@@ -48,6 +54,8 @@ extern "C" {
 enum { ORC_FACE_PERIODIC = 0, ORC_FACE_CLOSED = 1, ORC_FACE_FIXED = 2 };
 enum { ORC_TERM_VOLUME = 0, ORC_TERM_CONTACT = 1, ORC_TERM_BOUNDARY = 2 };
 enum { ORC_ALG_SEQUENTIAL = 0, ORC_ALG_CHECKERBOARD = 1, ORC_ALG_LOTTERY = 2 };
+enum { ORC_MECH_PRESSURE = 1, ORC_MECH_TENSION = 2 };
+enum { ORC_MECH_INIT_CONSTITUTIVE = 1, ORC_MECH_INIT_STATIONARY = 2, ORC_MECH_INIT_PRESERVE = 3 };
 
 typedef struct orc_relation {
     int32_t n;                     /* direction count, canonical order */
@@ -89,6 +97,14 @@ typedef struct orc_model {
     double retire_volume_strength;
     int64_t retire_target_boundary;
     double retire_boundary_strength;
+    /* mechanical components in ScientificComponentSet.mechanics order (Float32) */
+    int32_t n_mech;
+    int32_t mech_kind[2];           /* ORC_MECH_* */
+    int32_t mech_init[2];           /* ORC_MECH_INIT_* */
+    uint32_t mech_instance[2];      /* RNG operation of the component's draws */
+    float mech_eta[2];
+    int32_t mech_noise_is_temperature[2];
+    float mech_noise[2];
 } orc_model;
 
 typedef struct orc_state {
@@ -103,6 +119,8 @@ typedef struct orc_state {
     double *volume_strength;        /* capacity */
     int64_t *target_boundary;       /* capacity */
     double *boundary_strength;      /* capacity */
+    float *volume_pressure;         /* capacity (FluctuatingVolumePressure state) */
+    float *surface_tension;         /* capacity (FluctuatingSurfaceTension state) */
 } orc_state;
 
 /* ---- semantic RNG (rng/semantic.jl) ---- */
@@ -119,6 +137,13 @@ uint32_t orc_bounded_uint(uint64_t seed, uint32_t stream, uint64_t mcs, uint32_t
                           uint32_t operation, uint32_t entity_kind, uint32_t entity,
                           uint32_t draw, uint32_t bound);
 uint64_t orc_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t rerun);
+/* normal_box_muller(Float32) at (stream, mcs, subround, operation, CellEntity, entity,
+ * generation 0, invocation, draw 0) */
+float orc_normal_box_muller(uint64_t seed, uint32_t stream, uint64_t mcs, uint32_t subround,
+                            uint32_t operation, uint32_t entity, uint32_t invocation);
+/* _mechanical_ou_transition in Float32 */
+float orc_ou_transition(float q, float displacement, float strength, float eta, float noise,
+                        float interval, float standard_normal);
 
 /* ---- schedules ---- */
 /* Greedy realized colouring. colors[N] receives 1-based colours (0 for immutable
@@ -131,6 +156,11 @@ int32_t orc_conflict_degrees(const orc_model *m, uint8_t *degrees);
 typedef struct orc_sim orc_sim;
 orc_sim *orc_create(const orc_model *m, int32_t algorithm, uint64_t seed);
 void orc_destroy(orc_sim *s);
+/* _initialize_mechanics! and one _advance_mechanics! call (mcs, subround, phase, interval) */
+void orc_initialize_mechanics(orc_sim *s);
+void orc_advance_mechanics(orc_sim *s, uint64_t mcs, uint32_t subround, uint32_t phase, float interval);
+/* delta_h and mechanical work of one proposal (recipient site, 1-based direction) */
+int32_t orc_copy_evaluation(const orc_sim *s, uint32_t recipient, int32_t direction, float out[2]);
 orc_state *orc_state_of(orc_sim *s);     /* mutable view of the simulation state */
 void orc_set_seed(orc_sim *s, uint64_t seed);
 void orc_set_mcs(orc_sim *s, uint64_t mcs);

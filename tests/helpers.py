@@ -119,7 +119,11 @@ def scientific_fixture(dims, dtype=np.float32):
     return compiled, proposal, components
 
 
-def oracle_for(compiled, proposal, components, algorithm, seed, alg_override=None):
+def oracle_for(compiled, proposal, components, algorithm, seed, alg_override=None, initialize_mechanics=True):
+    """Oracle integrator as init_scientific builds it (mechanical states initialized)."""
     from oracle.bridge import OracleSim
     spec = build_spec(compiled, proposal, components, algorithm, seed)
-    return OracleSim(spec, state_arrays(compiled, spec), alg_override), spec
+    sim = OracleSim(spec, state_arrays(compiled, spec), alg_override)
+    if spec.mechanics and initialize_mechanics:
+        sim.initialize_mechanics()
+    return sim, spec
