@@ -6,7 +6,7 @@ include/cpm_mi355x.h).
 """
 from .model import (AlgorithmTemperatureNoise, ConstitutiveMeanInitialization, FixedMechanicalNoise,
                     FluctuatingSurfaceTension, FluctuatingVolumePressure, PreserveMechanicalInitialization,
-                    StationaryMechanicalInitialization, AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
+                    StationaryMechanicalInitialization, TiledCheckerboardCPM, AxisBoundary, BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
                     CellOwner, CheckerboardSweepCPM, ClosedBoundary, CompiledScientificState,
                     ContactRole, FixedExterior, LogicalPottsState, LotteryCPM, MediumOwner,
                     MediumTypeTable, PeriodicBoundary, ProposalRole, QuadraticBoundaryHamiltonian,

@@ -509,6 +509,37 @@ class SequentialCPM(_Algorithm):
     pass
 
 
+class TiledCheckerboardCPM:
+    """Deterministic tile-local CPM with snapshot and reconciliation subrounds
+    (tiled_checkerboard.jl:15-47): tile_size None (4 x 4 / 2 x 2 x 2) or a 2- or 3-tuple of
+    positive extents, switching_interval None or a positive UInt16, shared_memory one of
+    "auto", "required", "disabled" (launch geometry only)."""
+
+    def __init__(self, temperature=np.float32(20.0), tile_size=None, switching_interval=None,
+                 shared_memory="auto"):
+        t = temperature if isinstance(temperature, np.floating) else np.float32(temperature)
+        if not np.isfinite(t) or t < 0:
+            raise ValueError("tiled checkerboard temperature must be finite and non-negative")
+        if tile_size is not None:
+            if not isinstance(tile_size, tuple) or len(tile_size) not in (2, 3):
+                raise ValueError("tile_size must be nothing or a two-/three-dimensional tuple")
+            if not all(isinstance(v, (int, np.integer)) and v > 0 for v in tile_size):
+                raise ValueError("tile dimensions must be positive integers")
+            tile_size = tuple(int(v) for v in tile_size)
+        if switching_interval is not None:
+            if not isinstance(switching_interval, (int, np.integer)) or switching_interval <= 0:
+                raise ValueError("switching_interval must be nothing or a positive integer")
+            if switching_interval > 0xFFFF:
+                raise ValueError("switching_interval must fit UInt16")
+            switching_interval = int(switching_interval)
+        if shared_memory not in ("auto", "required", "disabled"):
+            raise ValueError("shared_memory must be :auto, :required, or :disabled")
+        self.temperature = t
+        self.tile_size = tile_size
+        self.switching_interval = switching_interval
+        self.shared_memory = shared_memory
+
+
 # ---------------------------------------------------------------------------
 # State
 # ---------------------------------------------------------------------------

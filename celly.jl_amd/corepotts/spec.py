@@ -16,12 +16,13 @@ import numpy as np
 from .model import (MECH_PRESSURE, MECH_STATE_COLUMN, MECH_TENSION, BoundaryEdgeCount,
                     CheckerboardSweepCPM, ClosedBoundary, CompiledScientificState, FixedExterior,
                     FluctuatingSurfaceTension, FluctuatingVolumePressure, LotteryCPM, PeriodicBoundary,
+                    TiledCheckerboardCPM,
                     QuadraticBoundaryHamiltonian, QuadraticVolumeHamiltonian, SequentialCPM,
                     StaticCartesianRelation, UnorderedContactHamiltonian,
                     validate_relation_domain)
 
 TERM_VOLUME, TERM_CONTACT, TERM_BOUNDARY = 0, 1, 2
-ALG_SEQUENTIAL, ALG_CHECKERBOARD, ALG_LOTTERY = 0, 1, 2
+ALG_SEQUENTIAL, ALG_CHECKERBOARD, ALG_LOTTERY, ALG_TILED = 0, 1, 2, 3
 MAX_TYPES = 16
 MAX_MEDIA = 16
 
@@ -55,6 +56,8 @@ class ModelSpec:
     # (kind, initialization, instance_id, eta, noise_is_temperature, noise) per mechanical
     # component, ScientificComponentSet.mechanics order
     mechanics: tuple = ()
+    tile_size: Tuple[int, int, int] = (0, 0, 0)   # TiledCheckerboardCPM (0: default extents)
+    switching_interval: int = 0                   # TiledCheckerboardCPM (0: none)
 
     @property
     def n_types(self) -> int:
@@ -78,6 +81,8 @@ def algorithm_id(algorithm) -> int:
         return ALG_LOTTERY
     if isinstance(algorithm, SequentialCPM):
         return ALG_SEQUENTIAL
+    if isinstance(algorithm, TiledCheckerboardCPM):
+        return ALG_TILED
     raise TypeError("unsupported algorithm")
 
 
@@ -172,6 +177,16 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
             raise NotImplementedError(f"energy {type(component).__name__} is outside this engine")
     if len(set(order)) != len(order):
         raise NotImplementedError("each energy term may appear once")
+    tile_size, switching = (0, 0, 0), 0
+    if alg == ALG_TILED:
+        # tiled_checkerboard_device.jl:132-163
+        if components.mechanics or components.constraints:
+            raise ValueError("the resident tiled path rejects constraints and mechanical components")
+        if algorithm.tile_size is not None:
+            if len(algorithm.tile_size) != compiled.domain.ndim:
+                raise ValueError("configured tile dimensionality does not match the lattice")
+            tile_size = tuple(algorithm.tile_size) + (0,) * (3 - len(algorithm.tile_size))
+        switching = algorithm.switching_interval or 0
     mechanics = []
     for component in components.mechanics:
         mechanics.append(_mechanics_entry(component, compiled, algorithm, columns))
@@ -205,7 +220,8 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
         boundary_f64=bool(boundary_f64), J=J, table_owner=tuple(table_owner),
         table_type=tuple(table_type), medium_ids=tuple(compiled.medium_ids),
         temperature=np.float32(algorithm.temperature), capacity=compiled.capacity,
-        seed=int(seed), columns=columns, mechanics=tuple(mechanics))
+        seed=int(seed), columns=columns, mechanics=tuple(mechanics), tile_size=tile_size,
+        switching_interval=switching)
 
 
 def state_arrays(compiled: CompiledScientificState, spec: ModelSpec):

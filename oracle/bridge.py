@@ -41,6 +41,7 @@ class OrcModel(C.Structure):
         ("n_mech", C.c_int32), ("mech_kind", C.c_int32 * 2), ("mech_init", C.c_int32 * 2),
         ("mech_instance", C.c_uint32 * 2), ("mech_eta", C.c_float * 2),
         ("mech_noise_is_temperature", C.c_int32 * 2), ("mech_noise", C.c_float * 2),
+        ("tile_size", C.c_int32 * 3), ("switching_interval", C.c_int32),
     ]
 
 
@@ -109,6 +110,9 @@ def load() -> C.CDLL:
     lib.orc_sample_transitions.argtypes = [P, U64, U64, P, C.c_int32, P]
     lib.orc_copy_evaluation.restype = C.c_int32
     lib.orc_copy_evaluation.argtypes = [P, U32, C.c_int32, P]
+    lib.orc_set_tiled_reconcile.argtypes = [P, C.c_int32]
+    lib.orc_tiled_layout.restype = C.c_int32
+    lib.orc_tiled_layout.argtypes = [P, P, P, P, P]
     lib.orc_initialize_mechanics.argtypes = [P]
     lib.orc_advance_mechanics.argtypes = [P, U64, U32, U32, C.c_float]
     lib.orc_normal_box_muller.restype = C.c_float
@@ -175,6 +179,9 @@ def make_model(spec, keep: list) -> OrcModel:
     for k, (kind, init, instance, eta, noise_is_temperature, noise) in enumerate(spec.mechanics):
         m.mech_kind[k], m.mech_init[k], m.mech_instance[k] = kind, init, instance
         m.mech_eta[k], m.mech_noise_is_temperature[k], m.mech_noise[k] = float(eta), int(noise_is_temperature), float(noise)
+    for a in range(3):
+        m.tile_size[a] = spec.tile_size[a]
+    m.switching_interval = spec.switching_interval
     return m
 
 
@@ -245,6 +252,21 @@ class OracleSim:
         out = np.zeros(2, np.float32)
         rc = self.lib.orc_copy_evaluation(self._sim, int(recipient), int(direction), out.ctypes.data)
         return int(rc), out[0], out[1]
+
+    def set_tiled_reconcile(self, recount: bool):
+        """Reconcile tiled subrounds by recount from ownership (True) or snapshot plus deltas."""
+        self.lib.orc_set_tiled_reconcile(self._sim, 1 if recount else 0)
+
+    def tiled_layout(self):
+        """(grid, reach, halo, tile colours) of a TiledCheckerboardCPM simulation."""
+        grid = np.zeros(3, np.int32)
+        reach = np.zeros(3, np.int32)
+        halo = C.c_int32()
+        self.lib.orc_tiled_layout(self._sim, grid.ctypes.data, reach.ctypes.data, C.byref(halo), None)
+        colors = np.zeros(int(np.prod(grid)), np.uint16)
+        count = self.lib.orc_tiled_layout(self._sim, grid.ctypes.data, reach.ctypes.data, C.byref(halo),
+                                          colors.ctypes.data)
+        return tuple(int(g) for g in grid), tuple(int(r) for r in reach), int(halo.value), colors, int(count)
 
     def initialize_mechanics(self):
         self.lib.orc_initialize_mechanics(self._sim)

@@ -105,7 +105,7 @@ uint64_t orc_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t r
 enum {
     S_RECIPIENT = 3, S_DIRECTION = 4, S_ACCEPTANCE = 5, S_LOTTERY_ACTIVATION = 6,
     S_LOTTERY_PRIORITY = 7, S_CHECKERBOARD_ORDER = 8, S_CHECKERBOARD_PRIORITY = 17,
-    S_AUX_EVOLUTION = 9, S_AUX_INITIALIZATION = 18
+    S_AUX_EVOLUTION = 9, S_AUX_INITIALIZATION = 18, S_TILED_ORDER = 19, S_TILED_PROPOSAL = 20
 };
 enum { K_GLOBAL = 0, K_SITE = 1, K_CELL = 2 };
 
@@ -236,6 +236,16 @@ struct orc_sim {
     uint32_t (*tickets)[4];
     uint32_t *activations;   /* lottery: activated opportunities of each mutable position in the
                                 last MCS (lottery.jl accounting[1:N]) */
+    /* TiledCheckerboardCPM layout (tiled_checkerboard.jl:87-196) */
+    int32_t tile_size[3], tile_grid[3], tile_reach[3], tile_halo;
+    int64_t ntiles;
+    uint16_t *tile_colors;   /* per tile, 1-based */
+    uint32_t *tile_sites;    /* mutable sites of each tile, x fastest inside a tile */
+    uint32_t *tile_offsets;  /* ntiles + 1 */
+    uint32_t *color_tiles;   /* tiles grouped by colour, ascending */
+    uint32_t *color_tile_offsets; /* color_count + 1 */
+    int32_t tiled_interval, tiled_batches, tiled_recount;
+    int64_t *acc_dvol, *acc_dbnd;
 };
 
 enum { OUT_ACTIONABLE = 1, OUT_SAME_OWNER = 2, OUT_BOUNDARY_NULL = 3, OUT_IMMUTABLE = 4 };
@@ -340,6 +350,134 @@ static void *xcalloc(size_t n, size_t sz) {
     return p;
 }
 
+static int32_t relation_radius(const orc_relation *r) {
+    int32_t h = 0;
+    for (int k = 0; k < r->n; ++k)
+        for (int a = 0; a < 3; ++a) {
+            int32_t v = r->off[k][a] < 0 ? -r->off[k][a] : r->off[k][a];
+            if (v > h) h = v;
+        }
+    return h;
+}
+
+/* tiled_checkerboard.jl:87-196 and tiled_checkerboard_device.jl:28-121: halo radius of the
+ * declared accesses, tile grid, greedy halo colouring of the tiles in linear order (two tiles
+ * conflict when their separation, periodic where the axis is, is within the reach on every
+ * axis), the mutable sites of every tile, the switching interval and batch count. Only the
+ * tiles within the reach of a tile can conflict with it, so the greedy scan visits those. */
+static int tiled_layout(orc_sim *s) {
+    const orc_model *m = &s->m;
+    int32_t halo = relation_radius(&m->proposal);
+    int32_t rs = relation_radius(&m->surface);
+    if (rs > halo) halo = rs;
+    for (int t = 0; t < m->n_energies; ++t) {
+        int32_t r = 0;
+        if (m->energy_order[t] == ORC_TERM_CONTACT) r = relation_radius(&m->contact);
+        else if (m->energy_order[t] == ORC_TERM_BOUNDARY) r = rs;
+        else continue;
+        if (r < 1) r = 1; /* dependency_radius = 1 (scientific_inner_loop.jl:321-332) */
+        if (r > halo) halo = r;
+    }
+    s->tile_halo = halo;
+    int64_t ntiles = 1;
+    int periodic[3];
+    for (int a = 0; a < 3; ++a) {
+        int32_t ts = a < m->ndim ? (m->tile_size[a] > 0 ? m->tile_size[a] : (m->ndim == 2 ? 4 : 2)) : 1;
+        s->tile_size[a] = ts;
+        s->tile_grid[a] = a < m->ndim ? (m->dims[a] + ts - 1) / ts : 1;
+        s->tile_reach[a] = a < m->ndim ? (halo + ts - 1) / ts : 0;
+        periodic[a] = a < m->ndim && m->face_kind[a][0] == ORC_FACE_PERIODIC;
+        ntiles *= s->tile_grid[a];
+    }
+    s->ntiles = ntiles;
+    const int32_t gx = s->tile_grid[0], gy = s->tile_grid[1];
+    s->tile_colors = (uint16_t *)xcalloc((size_t)ntiles, 2);
+    int32_t maxc = 0;
+    uint8_t *taken = (uint8_t *)xcalloc(65538, 1);
+    for (int64_t t = 0; t < ntiles; ++t) {
+        int32_t c[3] = {(int32_t)(t % gx), (int32_t)((t / gx) % gy), (int32_t)(t / ((int64_t)gx * gy))};
+        int32_t lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = -s->tile_reach[a]; hi[a] = s->tile_reach[a]; }
+        int32_t marked[4096];
+        int nm = 0;
+        for (int32_t dz = lo[2]; dz <= hi[2]; ++dz)
+            for (int32_t dy = lo[1]; dy <= hi[1]; ++dy)
+                for (int32_t dx = lo[0]; dx <= hi[0]; ++dx) {
+                    int32_t d[3] = {dx, dy, dz}, q[3];
+                    int ok = 1;
+                    for (int a = 0; a < 3 && ok; ++a) {
+                        int32_t v = c[a] + d[a], n = s->tile_grid[a];
+                        if (periodic[a]) { v %= n; if (v < 0) v += n; }
+                        else if (v < 0 || v >= n) ok = 0;
+                        q[a] = v;
+                    }
+                    if (!ok) continue;
+                    int64_t u = q[0] + (int64_t)gx * (q[1] + (int64_t)gy * q[2]);
+                    if (u >= t) continue;
+                    int32_t col = s->tile_colors[u];
+                    if (!taken[col] && nm < 4096) { taken[col] = 1; marked[nm++] = col; }
+                }
+        int32_t col = 1;
+        while (taken[col]) ++col;
+        for (int k = 0; k < nm; ++k) taken[marked[k]] = 0;
+        if (col > 65535) { free(taken); return 1; }
+        s->tile_colors[t] = (uint16_t)col;
+        if (col > maxc) maxc = col;
+    }
+    free(taken);
+    if (maxc > 255) return 1;
+    s->color_count = maxc;
+    s->color_tile_offsets = (uint32_t *)xcalloc((size_t)maxc + 1, 4);
+    s->color_tiles = (uint32_t *)xcalloc((size_t)ntiles, 4);
+    uint32_t pos = 0;
+    for (int32_t col = 1; col <= maxc; ++col) {
+        s->color_tile_offsets[col - 1] = pos;
+        for (int64_t t = 0; t < ntiles; ++t)
+            if (s->tile_colors[t] == col) s->color_tiles[pos++] = (uint32_t)(t + 1);
+    }
+    s->color_tile_offsets[maxc] = pos;
+    s->tile_offsets = (uint32_t *)xcalloc((size_t)ntiles + 1, 4);
+    s->tile_sites = (uint32_t *)xcalloc((size_t)s->n_mutable + 1, 4);
+    uint32_t at = 0, maxsites = 0;
+    const int32_t X = m->dims[0], Y = m->dims[1], Z = m->ndim == 3 ? m->dims[2] : 1;
+    for (int64_t t = 0; t < ntiles; ++t) {
+        s->tile_offsets[t] = at;
+        int32_t c[3] = {(int32_t)(t % gx), (int32_t)((t / gx) % gy), (int32_t)(t / ((int64_t)gx * gy))};
+        int32_t z1 = c[2] * s->tile_size[2] + s->tile_size[2]; if (z1 > Z) z1 = Z;
+        int32_t y1 = c[1] * s->tile_size[1] + s->tile_size[1]; if (y1 > Y) y1 = Y;
+        int32_t x1 = c[0] * s->tile_size[0] + s->tile_size[0]; if (x1 > X) x1 = X;
+        for (int32_t z = c[2] * s->tile_size[2]; z < z1; ++z)
+            for (int32_t y = c[1] * s->tile_size[1]; y < y1; ++y)
+                for (int32_t x = c[0] * s->tile_size[0]; x < x1; ++x) {
+                    uint32_t site = 1u + (uint32_t)x + (uint32_t)X * ((uint32_t)y + (uint32_t)Y * (uint32_t)z);
+                    if (site_mutable(m, site)) s->tile_sites[at++] = site;
+                }
+        if (at - s->tile_offsets[t] > maxsites) maxsites = at - s->tile_offsets[t];
+    }
+    s->tile_offsets[ntiles] = at;
+    if (maxsites == 0) return 1;
+    s->tiled_interval = m->switching_interval > 0 && (uint32_t)m->switching_interval < maxsites
+                            ? m->switching_interval : (int32_t)maxsites;
+    s->tiled_batches = ((int32_t)maxsites + s->tiled_interval - 1) / s->tiled_interval;
+    if (s->color_count * s->tiled_batches > 255) return 1;
+    s->acc_dvol = (int64_t *)xcalloc((size_t)m->capacity, 8);
+    s->acc_dbnd = (int64_t *)xcalloc((size_t)m->capacity, 8);
+    return 0;
+}
+
+void orc_set_tiled_reconcile(orc_sim *s, int32_t mode) { s->tiled_recount = mode; }
+
+int32_t orc_tiled_layout(const orc_sim *s, int32_t grid[3], int32_t reach[3], int32_t *halo,
+                         uint16_t *tile_colors) {
+    for (int a = 0; a < 3; ++a) {
+        grid[a] = s->tile_grid[a];
+        reach[a] = s->tile_reach[a];
+    }
+    *halo = s->tile_halo;
+    if (tile_colors) memcpy(tile_colors, s->tile_colors, (size_t)s->ntiles * 2);
+    return s->color_count;
+}
+
 orc_sim *orc_create(const orc_model *model, int32_t algorithm, uint64_t seed) {
     orc_sim *s = (orc_sim *)xcalloc(1, sizeof(orc_sim));
     s->m = *model;
@@ -438,6 +576,8 @@ orc_sim *orc_create(const orc_model *model, int32_t algorithm, uint64_t seed) {
         if (s->color_count > 255) { orc_destroy(s); return NULL; }
         s->tickets = (uint32_t (*)[4])xcalloc((size_t)nm, 16);
         s->activations = (uint32_t *)xcalloc((size_t)nm, sizeof(uint32_t));
+    } else if (algorithm == ORC_ALG_TILED) {
+        if (tiled_layout(s) != 0) { orc_destroy(s); return NULL; }
     } else {
         s->color_count = 1;
     }
@@ -463,6 +603,8 @@ void orc_destroy(orc_sim *s) {
     free(s->color_sites); free(s->color_offsets); free(s->csr_offsets); free(s->csr_nbrs);
     free(s->tickets);
     free(s->activations);
+    free(s->tile_colors); free(s->tile_sites); free(s->tile_offsets); free(s->color_tiles);
+    free(s->color_tile_offsets); free(s->acc_dvol); free(s->acc_dbnd);
     free(s);
 }
 
@@ -1120,6 +1262,140 @@ static void lottery_mcs(orc_sim *s) {
     s->mcs = mcs;
 }
 
+/* Recount of finite volumes and boundary measures from ownership (rebuild_derived_state!). */
+static void recount_trackers(orc_sim *s) {
+    int64_t cap = s->m.capacity;
+    for (int64_t c = 0; c < cap; ++c) {
+        s->st.finite_volumes[c] = 0;
+        s->st.boundary_measures[c] = 0;
+    }
+    for (int64_t i = 0; i < s->n; ++i) {
+        int64_t o = owner_at(s, (uint32_t)(i + 1));
+        if (o > 0) s->st.finite_volumes[o - 1]++;
+    }
+    for (int64_t k = 0; k < s->n_mutable; ++k) {
+        uint32_t site = s->mutable_sites[k];
+        int64_t o = owner_at(s, site);
+        if (o <= 0) continue;
+        for (int d = 0; d < s->m.surface.n; ++d) {
+            nbr_t nb = realize(&s->m, &s->m.surface, ROLE_SURFACE, site, d);
+            if (nb.kind == NB_ABSENT || nb.kind == NB_INVALID) continue;
+            int64_t no = nb.kind == NB_MUTABLE ? owner_at(s, nb.site) : nb.owner;
+            if (no != o) s->st.boundary_measures[o - 1]++;
+        }
+    }
+}
+
+/* TiledCheckerboardCPM MCS (tiled_checkerboard_device.jl:407-586,688-705,743-806; the host
+ * reference tiled_checkerboard.jl:382-463). Tiles of one colour run from one snapshot: a
+ * tile's proposals are sequential and see the snapshot trackers plus the tile's own changes;
+ * ownership is written at once (tiles of a colour lie beyond each other's halo). After the
+ * colour's batch the trackers become the snapshot plus every tile's changes (or a recount)
+ * and cells left with zero volume retire. */
+static void tiled_mcs(orc_sim *s) {
+    uint64_t mcs = s->mcs + 1;
+    int32_t C = s->color_count, B = s->tiled_batches, interval = s->tiled_interval;
+    uint32_t K = (uint32_t)s->m.proposal.n;
+    int64_t cap = s->m.capacity;
+    orc_state *st = &s->st;
+    fisher_yates(s->seed, S_TILED_ORDER, mcs, 0, C, s->order);
+    uint64_t realized = 0, same = 0, boundary = 0, immutable = 0, rejected = 0, accepted = 0;
+    uint32_t subround = 0;
+    int64_t *t_cell = (int64_t *)xcalloc((size_t)interval * 2 + 2, 8);
+    int64_t *t_dvol = (int64_t *)xcalloc((size_t)interval * 2 + 2, 8);
+    int64_t *t_dbnd = (int64_t *)xcalloc((size_t)interval * 2 + 2, 8);
+    for (int32_t ordinal = 0; ordinal < C; ++ordinal) {
+        uint32_t color = s->order[ordinal];
+        uint32_t first_tile = s->color_tile_offsets[color - 1], stop_tile = s->color_tile_offsets[color];
+        for (int32_t batch = 1; batch <= B; ++batch) {
+            subround++;
+            uint32_t batch_start = (uint32_t)(batch - 1) * (uint32_t)interval + 1;
+            memset(s->acc_dvol, 0, (size_t)cap * 8);
+            memset(s->acc_dbnd, 0, (size_t)cap * 8);
+            for (uint32_t k = first_tile; k < stop_tile; ++k) {
+                uint32_t tile = s->color_tiles[k];
+                uint32_t first = s->tile_offsets[tile - 1];
+                uint32_t count = s->tile_offsets[tile] - first;
+                if (batch_start > count) continue;
+                uint32_t stop = batch_start + (uint32_t)interval - 1;
+                if (stop > count) stop = count;
+                int touched = 0;
+                for (uint32_t lp = batch_start; lp <= stop; ++lp) {
+                    uint32_t sel = orc_bounded_uint(s->seed, S_TILED_PROPOSAL, mcs, subround, lp, K_SITE,
+                                                    tile, 0, count);
+                    uint32_t rec = s->tile_sites[first + sel];
+                    uint32_t dir = orc_bounded_uint(s->seed, S_TILED_PROPOSAL, mcs, subround, lp, K_SITE,
+                                                    tile, 1, K) + 1;
+                    int64_t l, g;
+                    uint32_t donor;
+                    int out = construct_attempt(s, rec, (int)dir, &l, &g, &donor);
+                    if (out == OUT_SAME_OWNER) { same++; continue; }
+                    if (out == OUT_BOUNDARY_NULL) { boundary++; continue; }
+                    if (out == OUT_IMMUTABLE) { immutable++; continue; }
+                    realized++;
+                    int64_t dl, dg;
+                    boundary_change(s, rec, l, g, &dl, &dg);
+                    float dh = delta_h(s, rec, l, g, dl, dg);
+                    float p = acceptance_probability(dh, 0.0f, s->m.temperature);
+                    uint32_t w[4];
+                    orc_rng_words(s->seed, S_TILED_PROPOSAL, mcs, subround, lp, K_SITE, tile, 0, 0, 2, w);
+                    if (!(orc_uniform_open01_f32(w[0]) < p)) { rejected++; continue; }
+                    accepted++;
+                    st->tags[rec - 1] = g > 0 ? 1 : 2;
+                    st->ids[rec - 1] = (uint32_t)(g > 0 ? g : -g);
+                    int64_t cells[2] = {l, g}, dv[2] = {-1, 1}, db[2] = {dl, dg};
+                    for (int e = 0; e < 2; ++e) {
+                        if (cells[e] <= 0) {
+                            st->medium_volumes[medium_index(&s->m, cells[e])] += (int32_t)dv[e];
+                            continue;
+                        }
+                        int64_t i = cells[e] - 1;
+                        /* the tile's view: snapshot trackers plus its own changes */
+                        st->finite_volumes[i] += (int32_t)dv[e];
+                        st->boundary_measures[i] += db[e];
+                        int j = 0;
+                        while (j < touched && t_cell[j] != i) ++j;
+                        if (j == touched) { t_cell[touched] = i; t_dvol[touched] = 0; t_dbnd[touched] = 0; ++touched; }
+                        t_dvol[j] += dv[e];
+                        t_dbnd[j] += db[e];
+                    }
+                }
+                /* back to the snapshot for the next tile; the changes join the colour's sum */
+                for (int j = 0; j < touched; ++j) {
+                    st->finite_volumes[t_cell[j]] -= (int32_t)t_dvol[j];
+                    st->boundary_measures[t_cell[j]] -= t_dbnd[j];
+                    s->acc_dvol[t_cell[j]] += t_dvol[j];
+                    s->acc_dbnd[t_cell[j]] += t_dbnd[j];
+                }
+            }
+            if (s->tiled_recount) {
+                recount_trackers(s);
+            } else {
+                for (int64_t c = 0; c < cap; ++c) {
+                    st->finite_volumes[c] += (int32_t)s->acc_dvol[c];
+                    st->boundary_measures[c] += s->acc_dbnd[c];
+                }
+            }
+            for (int64_t c = 0; c < cap; ++c)
+                if (st->finite_volumes[c] == 0 && st->active[c]) {
+                    st->active[c] = 0;
+                    st->cell_types[c] = 0;
+                    st->target_volume[c] = s->m.retire_target_volume;
+                    st->volume_strength[c] = s->m.retire_volume_strength;
+                    st->target_boundary[c] = s->m.retire_target_boundary;
+                    st->boundary_strength[c] = s->m.retire_boundary_strength;
+                }
+        }
+    }
+    free(t_cell); free(t_dvol); free(t_dbnd);
+    uint64_t *r = s->report;
+    r[0] = mcs; r[1] = subround; r[2] = (uint64_t)s->n_mutable; r[3] = (uint64_t)s->n_mutable;
+    r[4] = realized; r[5] = same; r[6] = boundary; r[7] = immutable; r[8] = 0; r[9] = 0;
+    r[10] = rejected; r[11] = accepted;
+    s->has_report = 1;
+    s->mcs = mcs;
+}
+
 /* sequential.jl:488-567 */
 static void sequential_mcs(orc_sim *s) {
     uint64_t mcs = s->mcs + 1;
@@ -1181,6 +1457,7 @@ int32_t orc_step(orc_sim *s, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {
         if (s->algorithm == ORC_ALG_CHECKERBOARD) checkerboard_mcs(s);
         else if (s->algorithm == ORC_ALG_LOTTERY) lottery_mcs(s);
+        else if (s->algorithm == ORC_ALG_TILED) tiled_mcs(s);
         else sequential_mcs(s);
     }
     return 0;

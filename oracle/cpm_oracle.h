@@ -18,6 +18,10 @@
  *   lib/CorePotts/src/algorithms/checkerboard.jl           colouring + checkerboard MCS
  *   lib/CorePotts/src/algorithms/lottery.jl                lottery MCS
  *   lib/CorePotts/src/algorithms/sequential.jl:474-596     sequential MCS
+ *   lib/CorePotts/src/algorithms/tiled_checkerboard.jl     tiled layout, halo colouring, colour
+ *   lib/CorePotts/src/algorithms/tiled_checkerboard_device.jl  order, tile-local proposals and
+ *                                                          reconciliation (host recount or
+ *                                                          snapshot plus additive deltas)
  *   lib/CorePotts/src/sciml/interface.jl:1009-1026         ensemble_seed
  *   lib/CorePotts/src/components/scientific_mechanics.jl   mechanical OU transition, init,
  *   lib/CorePotts/src/components/scientific_components.jl  advance; mechanical work
@@ -53,7 +57,7 @@ extern "C" {
 
 enum { ORC_FACE_PERIODIC = 0, ORC_FACE_CLOSED = 1, ORC_FACE_FIXED = 2 };
 enum { ORC_TERM_VOLUME = 0, ORC_TERM_CONTACT = 1, ORC_TERM_BOUNDARY = 2 };
-enum { ORC_ALG_SEQUENTIAL = 0, ORC_ALG_CHECKERBOARD = 1, ORC_ALG_LOTTERY = 2 };
+enum { ORC_ALG_SEQUENTIAL = 0, ORC_ALG_CHECKERBOARD = 1, ORC_ALG_LOTTERY = 2, ORC_ALG_TILED = 3 };
 enum { ORC_MECH_PRESSURE = 1, ORC_MECH_TENSION = 2 };
 enum { ORC_MECH_INIT_CONSTITUTIVE = 1, ORC_MECH_INIT_STATIONARY = 2, ORC_MECH_INIT_PRESERVE = 3 };
 
@@ -105,6 +109,10 @@ typedef struct orc_model {
     float mech_eta[2];
     int32_t mech_noise_is_temperature[2];
     float mech_noise[2];
+    /* TiledCheckerboardCPM: tile extent per axis (0: the default 4 x 4 / 2 x 2 x 2) and
+     * switching interval (0: the largest tile's mutable site count) */
+    int32_t tile_size[3];
+    int32_t switching_interval;
 } orc_model;
 
 typedef struct orc_state {
@@ -159,6 +167,14 @@ void orc_destroy(orc_sim *s);
 /* _initialize_mechanics! and one _advance_mechanics! call (mcs, subround, phase, interval) */
 void orc_initialize_mechanics(orc_sim *s);
 void orc_advance_mechanics(orc_sim *s, uint64_t mcs, uint32_t subround, uint32_t phase, float interval);
+/* TiledCheckerboardCPM: reconciliation by recount from ownership (the host reference,
+ * tiled_checkerboard.jl:371-380) when mode = 1, by snapshot plus additive per-tile deltas
+ * (the device path, tiled_checkerboard_device.jl:688-705) when mode = 0 (default). */
+void orc_set_tiled_reconcile(orc_sim *s, int32_t mode);
+/* Tile layout: grid[3], reach[3], halo; tile_colors[prod(grid)] (1-based) and the colour count. */
+int32_t orc_tiled_layout(const orc_sim *s, int32_t grid[3], int32_t reach[3], int32_t *halo,
+                         uint16_t *tile_colors);
+
 /* delta_h and mechanical work of one proposal (recipient site, 1-based direction) */
 int32_t orc_copy_evaluation(const orc_sim *s, uint32_t recipient, int32_t direction, float out[2]);
 orc_state *orc_state_of(orc_sim *s);     /* mutable view of the simulation state */
