@@ -156,6 +156,37 @@ def cubes_leg(steps: int, graph_chunk: int):
     return out
 
 
+def tiled_leg(steps: int):
+    """TiledCheckerboardCPM on the config-2 workload (4096 x 4096, 163,840 cells of 10 x 10,
+    volume + Moore contact + boundary, T = 10) with the reference's default 4 x 4 tiles: one
+    thread per tile runs the tile's 16 proposals of each colour in order."""
+    from corepotts import init_scientific, workloads
+    log("tiled: building the 4096^2 workload")
+    model = workloads.config2_large(algorithm="tiled")
+    integ = init_scientific(*model)
+    integ.step(3)
+    integ.synchronize()
+    log(f"tiled: timed {steps} MCS")
+    t0 = time.perf_counter()
+    integ.step(steps)
+    integ.synchronize()
+    dt = time.perf_counter() - t0
+    rep = integ.current_mcs_report()
+    timing = integ.step_timed(4)
+    batch_us = 1e3 * timing["propose_ms"] / max(timing["propose_launches"], 1)
+    proposals_per_launch = rep.activated_attempts / max(rep.internal_rounds, 1)
+    out = {"lattice": [4096, 4096], "cells": 163840, "tile": [4, 4],
+           "internal_rounds": rep.internal_rounds,
+           "copy_attempts_per_s": rep.activated_attempts * steps / dt, "mcs_per_s": steps / dt,
+           "batch_kernel_avg_us": batch_us,
+           "reconcile_kernel_avg_us": 1e3 * timing["commit_ms"] / max(timing["commit_launches"], 1),
+           "proposals_per_batch_launch": proposals_per_launch,
+           "proposals_per_s_in_batch_kernel": proposals_per_launch / (1e-6 * batch_us),
+           "last_report": rep._asdict()}
+    integ.close()
+    return out
+
+
 def serving_leg(jobs: int, chunk: int, budget_gb: float):
     """Served stream: the first `jobs` Poisson arrivals at 200 jobs/s (80 % 2D side
     log-uniform [128, 2048], 20 % 3D side log-uniform [32, 128], 100-2000 MCS), admitted under
@@ -198,6 +229,8 @@ def main():
     ap.add_argument("--batch-replicas", type=int, default=512)
     ap.add_argument("--batch-mcs", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tiled", action="store_true")
+    ap.add_argument("--tiled-steps", type=int, default=20)
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-chunk", type=int, default=20)
     ap.add_argument("--tile-shape", type=int, default=-1,
@@ -303,6 +336,8 @@ def main():
 
     if not args.no_3d:
         extra["cubes_3d"] = cubes_leg(args.cubes_steps, 10)
+    if not args.no_tiled:
+        extra["tiled"] = tiled_leg(args.tiled_steps)
     if not args.no_batch:
         extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
     if not args.no_serve:

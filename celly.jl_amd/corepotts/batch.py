@@ -18,7 +18,7 @@ import numpy as np
 from .model import CompiledScientificState
 from .native import (EngineError, ScientificMCSReport, _ptr, load_library, make_desc, mechanics_arrays,
                      mechanics_columns, snapshot, state_buffers, state_pointers)
-from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ModelSpec, build_spec, state_arrays
+from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ALG_TILED, ModelSpec, build_spec, state_arrays
 
 
 def replica_bytes(spec: ModelSpec) -> int:
@@ -34,12 +34,14 @@ def replica_bytes(spec: ModelSpec) -> int:
 
 
 class ReplicaBatch:
-    """Device-resident replicas of one algorithm (CheckerboardSweepCPM or LotteryCPM)."""
+    """Device-resident replicas of one algorithm (CheckerboardSweepCPM, LotteryCPM or
+    TiledCheckerboardCPM)."""
 
     def __init__(self, algorithm: int = ALG_CHECKERBOARD, hbm_budget_bytes: int = 0,
                  graph_chunk: int = 0):
-        if algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY):
-            raise NotImplementedError("the MI355X engine batches CheckerboardSweepCPM and LotteryCPM")
+        if algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY, ALG_TILED):
+            raise NotImplementedError("the MI355X engine batches CheckerboardSweepCPM, LotteryCPM and "
+                                      "TiledCheckerboardCPM")
         self.lib = load_library()
         self.algorithm = algorithm
         handle = C.c_void_p()

@@ -224,12 +224,15 @@ def model_fingerprint(spec) -> bytes:
                             tuple(int(m) for m in spec.medium_ids), np.float32(spec.temperature),
                             int(spec.algorithm), tuple(float(r) for r in spec.retire),
                             tuple((int(k), int(i), int(n), np.float32(e), bool(t), np.float32(z))
-                                  for k, i, n, e, t, z in spec.mechanics))
+                                  for k, i, n, e, t, z in spec.mechanics),
+                            *((tuple(int(t) for t in spec.tile_size), int(spec.switching_interval))
+                              if spec.algorithm == 3 else ()))
 
 
 def continuation_profile(spec, compiled) -> ExactContinuationProfile:
     scalar = tuple(sorted({_JULIA_TYPE[np.asarray(v).dtype] for v in compiled.properties.values()}))
-    name = {1: "CheckerboardSweepCPM", 2: "LotteryCPM"}.get(spec.algorithm, "SequentialCPM")
+    name = {1: "CheckerboardSweepCPM", 2: "LotteryCPM", 3: "TiledCheckerboardCPM"}.get(spec.algorithm,
+                                                                                      "SequentialCPM")
     return ExactContinuationProfile("MI355X", name, Version("1.0.0"), Version("1.0.0"), Version("1.0.0"),
                                     Version(PERSISTENCE_VERSION), "native_backend", scalar, INDEX_TYPES,
                                     ("libcpm_mi355x@abi1",), "hip-gfx950")

@@ -16,7 +16,7 @@ import numpy as np
 
 from .model import (MECH_PRESSURE, MECH_STATE_COLUMN, MECH_TENSION, CompiledScientificState,
                     StaticCartesianRelation)
-from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ModelSpec, build_spec, state_arrays
+from .spec import ALG_CHECKERBOARD, ALG_LOTTERY, ALG_TILED, ModelSpec, build_spec, state_arrays
 
 MAX_DIRS = 128
 MAX_TYPES = 16
@@ -57,6 +57,7 @@ class CpmModelDesc(C.Structure):
         ("retire_target_volume", C.c_double), ("retire_volume_strength", C.c_double),
         ("retire_target_boundary", C.c_int64), ("retire_boundary_strength", C.c_double),
         ("n_mechanics", C.c_int32), ("mechanics", CpmMechanicsDesc * 2),
+        ("tile_size", C.c_int32 * 3), ("switching_interval", C.c_int32),
     ]
 
 
@@ -67,7 +68,7 @@ EXPORTS = (
     "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
     "cpm_set_row_kernels", "cpm_kernel_config", "cpm_upload_mechanics", "cpm_download_mechanics",
     "cpm_initialize_mechanics", "cpm_batch_upload_mechanics", "cpm_batch_download_mechanics",
-    "cpm_batch_initialize_mechanics",
+    "cpm_batch_initialize_mechanics", "cpm_tiled_layout",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
@@ -118,6 +119,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
     lib.cpm_batch_destroy.argtypes = [P]
     lib.cpm_replica_bytes.argtypes = [C.POINTER(CpmModelDesc), C.POINTER(C.c_uint64)]
+    lib.cpm_tiled_layout.argtypes = [C.POINTER(CpmModelDesc), P, P, C.POINTER(C.c_int32), P,
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.cpm_batch_add.argtypes = [P, C.POINTER(CpmModelDesc), C.POINTER(C.c_int32)]
     lib.cpm_batch_remove.argtypes = [P, C.c_int32]
     lib.cpm_batch_upload_state.argtypes = [P, C.c_int32] + state_args
@@ -212,6 +215,9 @@ def make_desc(spec: ModelSpec, keep: list) -> CpmModelDesc:
         m = d.mechanics[k]
         m.kind, m.initialization, m.instance_id = kind, init, instance
         m.eta, m.noise_is_temperature, m.noise = float(eta), int(noise_is_temperature), float(noise)
+    for a in range(3):
+        d.tile_size[a] = int(spec.tile_size[a])
+    d.switching_interval = int(spec.switching_interval)
     return d
 
 
@@ -291,8 +297,9 @@ class ScientificPottsIntegrator:
     """Device-resident integrator over one lattice, backed by a cpm_ctx."""
 
     def __init__(self, compiled: CompiledScientificState, spec: ModelSpec):
-        if spec.algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY):
-            raise NotImplementedError("the MI355X engine runs CheckerboardSweepCPM and LotteryCPM")
+        if spec.algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY, ALG_TILED):
+            raise NotImplementedError("the MI355X engine runs CheckerboardSweepCPM, LotteryCPM and "
+                                      "TiledCheckerboardCPM")
         self.lib = load_library()
         self.spec = spec
         self.compiled = compiled
@@ -438,6 +445,29 @@ def ensemble_seed(master_seed: int, trajectory: int, rerun: int = 1) -> int:
     if not 1 <= rerun <= 256:
         raise ValueError("ensemble rerun identity exceeds the v1 address domain")
     return int(load_library().cpm_ensemble_seed(int(master_seed), int(trajectory), int(rerun)))
+
+
+def tiled_layout(spec: ModelSpec) -> dict:
+    """The engine's TiledCheckerboardCPM layout of a model, computed on the host
+    (tiled_layout, tiled_checkerboard.jl:166-188; the batch schedule of
+    tiled_checkerboard_device.jl:28-121): grid, reach, halo, tile colours (first axis
+    fastest), colour count, switching interval and batches per colour."""
+    lib = load_library()
+    keep: list = []
+    desc = make_desc(spec, keep)
+    grid = np.zeros(3, np.int32)
+    reach = np.zeros(3, np.int32)
+    halo, count, interval, batches = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    rc = lib.cpm_tiled_layout(C.byref(desc), grid.ctypes.data, reach.ctypes.data, C.byref(halo), None,
+                              C.byref(count), C.byref(interval), C.byref(batches))
+    if rc != 0:
+        raise EngineError(rc, lib.cpm_last_error(None).decode())
+    colors = np.zeros(int(np.prod(grid)), np.uint16)
+    rc = lib.cpm_tiled_layout(C.byref(desc), None, None, None, colors.ctypes.data, None, None, None)
+    if rc != 0:
+        raise EngineError(rc, lib.cpm_last_error(None).decode())
+    return dict(grid=tuple(int(v) for v in grid), reach=tuple(int(v) for v in reach), halo=halo.value,
+                colors=colors, color_count=count.value, interval=interval.value, batches=batches.value)
 
 
 def copy_probe(nbytes: int, reps: int = 10) -> float:

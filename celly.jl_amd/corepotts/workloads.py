@@ -18,11 +18,23 @@ import numpy as np
 from .model import (BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
                     CheckerboardSweepCPM, LogicalPottsState, LotteryCPM, MediumTypeTable,
                     ProposalRole, QuadraticBoundaryHamiltonian, QuadraticVolumeHamiltonian,
-                    ScientificComponentSet, SurfaceRole, ContactRole, UnorderedContactHamiltonian,
+                    ScientificComponentSet, SurfaceRole, ContactRole, TiledCheckerboardCPM,
+                    UnorderedContactHamiltonian,
                     compile_scientific_state, first_shell_relation, moore_relation,
                     required_properties)
 
 J_TWO_TYPES = np.array([[0, 16, 16], [16, 2, 11], [16, 11, 14]], dtype=np.float32)
+
+
+def make_algorithm(name: str, temperature):
+    """CheckerboardSweepCPM, LotteryCPM or TiledCheckerboardCPM (reference defaults) by name."""
+    if name == "checkerboard":
+        return CheckerboardSweepCPM(temperature)
+    if name == "lottery":
+        return LotteryCPM(temperature)
+    if name == "tiled":
+        return TiledCheckerboardCPM(temperature)
+    raise ValueError(f"unknown algorithm {name!r}")
 
 
 def _schema(*components):
@@ -68,7 +80,7 @@ def plain_2d(L: int, corners, side: int = 10, *, temperature=np.float32(10.0), s
         state.property_values("boundary_strength")[:ncells] = lambda_s
     tracker = BoundaryMeasureTracker(BoundaryEdgeCount(), surface)
     compiled = compile_scientific_state(state, domain, tracker)
-    alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
+    alg = make_algorithm(algorithm, temperature)
     return compiled, proposal, ScientificComponentSet(energies=tuple(energies)), alg, seed
 
 
@@ -138,7 +150,7 @@ def cubes_3d(L: int = 256, side: int = 8, *, temperature=np.float32(20.0), seed:
     state.property_values("target_boundary")[:] = 6 * side * side
     state.property_values("boundary_strength")[:] = np.float32(0.5)
     compiled = compile_scientific_state(state, domain, BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
-    alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
+    alg = make_algorithm(algorithm, temperature)
     return compiled, proposal, ScientificComponentSet(energies=energies), alg, seed
 
 
@@ -179,7 +191,7 @@ def block_cells(L: int, ndim: int, side: int, pitch: int, *, ntypes: int = 2, J=
     state.property_values("target_volume")[:ncells] = np.float32(side ** ndim)
     state.property_values("volume_strength")[:ncells] = lambda_v
     compiled = compile_scientific_state(state, domain, BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
-    alg = CheckerboardSweepCPM(temperature) if algorithm == "checkerboard" else LotteryCPM(temperature)
+    alg = make_algorithm(algorithm, temperature)
     return compiled, proposal, ScientificComponentSet(energies=energies), alg, seed
 
 

@@ -31,6 +31,8 @@ enum : uint32_t {
     kStreamCheckerboardPriority = 17,
     kStreamAuxEvolution = 9,
     kStreamAuxInitialization = 18,
+    kStreamTiledOrder = 19,
+    kStreamTiledProposal = 20,
 };
 enum : uint32_t { kGlobalEntity = 0, kSiteEntity = 1, kCellEntity = 2 };
 
@@ -148,6 +150,17 @@ struct RepDev {
     CellRec *cells;              // cap + 1, index = cell id (entry 0: spare read for media)
     uint32_t *ctype;             // cap + 1 cell types (neighbour type lookups)
     int32_t *medium_vol;         // n_media
+    // TiledCheckerboardCPM: halo-coloured tiles (1-based ids, first axis fastest), the
+    // mutable sites of each tile (x fastest inside a tile), the tiles of each colour, the
+    // per-cell tracker changes of one colour batch and the counters of the current MCS
+    int32_t tiled_interval, tiled_batches;
+    const uint32_t *tile_sites;     // mutable sites (1-based), grouped by tile
+    const uint32_t *tile_offsets;   // tiles + 1 ranges into tile_sites
+    const uint32_t *color_tiles;    // tiles (1-based) grouped by colour, ascending
+    const uint32_t *color_offsets;  // colours + 1 ranges into color_tiles
+    int32_t *tile_dvol;             // cap + 1 volume changes (summed over the batch's tiles)
+    int32_t *tile_dbnd;             // cap + 1 boundary-measure changes
+    unsigned long long *tiled_counts;  // realized, same owner, boundary null, rejected, accepted
 };
 
 // Per-replica dynamic state, updated on the device by the begin-MCS kernel.

@@ -26,6 +26,11 @@ extern "C" hipError_t cpm_internal_launch_mech_init(const cpm::RepDev *reps, int
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_commit(const cpm::PassArgs *a, hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+                                                      uint32_t ordinal, uint32_t batch, uint32_t max_tiles,
+                                                      uint32_t interval, hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+                                                          uint32_t capacity, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
                                                 int nslots, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
@@ -128,6 +133,14 @@ struct Replica {
     // of each colour (clist) and, per tile and colour, their range (coff, rounds + 1 per tile)
     std::vector<uint16_t> clist;
     std::vector<uint32_t> coff;
+    // TiledCheckerboardCPM layout: tile edge, grid and reach per axis, halo radius, halo
+    // colour of every tile (1-based), mutable sites by tile, tiles by colour, the switching
+    // interval, batches per colour and the most tiles of any colour
+    int tile_edge[3] = {1, 1, 1}, tile_grid[3] = {1, 1, 1}, tile_reach[3] = {0, 0, 0};
+    int tiled_halo = 0, tiled_interval = 0, tiled_batches = 0;
+    uint32_t tiled_max_tiles = 0;
+    std::vector<uint16_t> tile_colors;
+    std::vector<uint32_t> tile_sites, tile_offsets, color_tiles, color_offsets;
     cpm::RepDev dev{};
     // device buffers
     int32_t *d_lattice = nullptr;
@@ -137,6 +150,9 @@ struct Replica {
     int32_t *d_mvol = nullptr;
     cpm::CellRec *d_cells = nullptr;
     uint32_t *d_ctype = nullptr;
+    uint32_t *d_tile_sites = nullptr, *d_tile_offsets = nullptr, *d_color_tiles = nullptr, *d_color_offsets = nullptr;
+    int32_t *d_tile_dvol = nullptr, *d_tile_dbnd = nullptr;
+    unsigned long long *d_tiled_counts = nullptr;
     uint64_t bytes = 0;
     uint64_t estimate = 0;      // estimate_replica_bytes at admission
     uint64_t base_mcs = 0;      // MCS counter when the state was last set
@@ -149,6 +165,8 @@ struct Replica {
         dev_free(d_lattice); dev_free(d_mask); dev_free(d_color); dev_free(d_degree);
         dev_free(d_clist); dev_free(d_coff);
         dev_free(d_cells); dev_free(d_ctype); dev_free(d_mvol);
+        dev_free(d_tile_sites); dev_free(d_tile_offsets); dev_free(d_color_tiles); dev_free(d_color_offsets);
+        dev_free(d_tile_dvol); dev_free(d_tile_dbnd); dev_free(d_tiled_counts);
     }
 };
 
@@ -260,11 +278,173 @@ int conflict_neighbours(const HostGeom &g, const std::vector<int32_t> &offs, int
     return count;
 }
 
+// Group configuration of TiledCheckerboardCPM replicas (outside the tile-shape ids 0..63).
+constexpr int kTiledConfig = 64;
+
+// Largest per-axis offset of a relation (its radius).
+int relation_radius(const cpm_relation_desc &rel, int ndim) { return max_abs_offset(rel, ndim); }
+
+// TiledCheckerboardCPM layout (tiled_checkerboard.jl:87-196; tiled_checkerboard_device.jl:
+// 28-121): halo radius of the declared accesses; tile grid and reach; greedy halo colouring
+// of the tiles in linear order, first axis fastest (two tiles conflict when their tile
+// separation, periodic where the axis is, lies within the reach on every axis; only tiles
+// within the reach can conflict, so the scan visits those); the mutable sites of every tile
+// (x fastest inside a tile); the tiles of each colour; switching interval and batches.
+void compile_tiled(Replica &r, const HostGeom &g) {
+    const cpm_model_desc &d = r.desc;
+    const int nd = d.ndim;
+    int halo = std::max(relation_radius(d.proposal, nd), relation_radius(d.surface, nd));
+    for (int t = 0; t < d.n_energies; ++t) {
+        int rad;
+        if (d.energy_order[t] == CPM_TERM_CONTACT) rad = relation_radius(d.contact, nd);
+        else if (d.energy_order[t] == CPM_TERM_BOUNDARY) rad = relation_radius(d.surface, nd);
+        else continue;
+        halo = std::max(halo, std::max(rad, 1));  // dependency radius at least 1
+    }
+    r.tiled_halo = halo;
+    bool periodic[3] = {false, false, false};
+    uint64_t ntiles = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int edge = a < nd ? (d.tile_size[a] > 0 ? d.tile_size[a] : (nd == 2 ? 4 : 2)) : 1;
+        r.tile_edge[a] = edge;
+        r.tile_grid[a] = a < nd ? (g.dims[a] + edge - 1) / edge : 1;
+        r.tile_reach[a] = a < nd ? (halo + edge - 1) / edge : 0;
+        periodic[a] = a < nd && g.face_kind[a][0] == CPM_FACE_PERIODIC;
+        ntiles *= (uint64_t)r.tile_grid[a];
+    }
+    if (ntiles > 0x7fffffffull) fail(CPM_E_UNSUPPORTED, "too many tiles");
+    const int gx = r.tile_grid[0], gy = r.tile_grid[1];
+    auto tile_coords = [&](uint64_t t, int c[3]) {
+        c[0] = (int)(t % (uint64_t)gx);
+        c[1] = (int)((t / (uint64_t)gx) % (uint64_t)gy);
+        c[2] = (int)(t / ((uint64_t)gx * (uint64_t)gy));
+    };
+    // visits the tiles within `reach + extra` tiles of tile t (wrapped on periodic axes)
+    auto around = [&](uint64_t t, int extra, auto &&fn) {
+        int c[3];
+        tile_coords(t, c);
+        const int rx = r.tile_reach[0] + extra, ry = r.tile_reach[1] + extra, rz = nd == 3 ? r.tile_reach[2] + extra : 0;
+        for (int dz = -rz; dz <= rz; ++dz)
+            for (int dy = -ry; dy <= ry; ++dy)
+                for (int dx = -rx; dx <= rx; ++dx) {
+                    const int off[3] = {dx, dy, dz};
+                    int q[3];
+                    bool ok = true;
+                    for (int a = 0; a < 3 && ok; ++a) {
+                        int v = c[a] + off[a];
+                        const int n = r.tile_grid[a];
+                        if (periodic[a]) v = ((v % n) + n) % n;
+                        else if (v < 0 || v >= n) ok = false;
+                        q[a] = v;
+                    }
+                    if (ok) fn((uint64_t)q[0] + (uint64_t)gx * ((uint64_t)q[1] + (uint64_t)gy * (uint64_t)q[2]));
+                }
+    };
+    r.tile_colors.assign((size_t)ntiles, 0);
+    int maxc = 0;
+    std::vector<uint64_t> taken(8, 0);  // colours 1..511 as bits
+    for (uint64_t t = 0; t < ntiles; ++t) {
+        std::fill(taken.begin(), taken.end(), 0ull);
+        around(t, 0, [&](uint64_t u) {
+            if (u < t) {
+                const int col = r.tile_colors[(size_t)u];
+                taken[(size_t)col >> 6] |= 1ull << (col & 63);
+            }
+        });
+        int col = 1;
+        while (col < 511 && ((taken[(size_t)col >> 6] >> (col & 63)) & 1ull)) ++col;
+        if (col > 255) fail(CPM_E_UNSUPPORTED, "tile colouring exceeds 255 colours");
+        r.tile_colors[(size_t)t] = (uint16_t)col;
+        maxc = std::max(maxc, col);
+    }
+    // Tiles of one colour run concurrently and read the lattice within `halo` of their own
+    // sites, so no two may lie within `halo` of each other on every axis. The colouring
+    // guarantees this unless a periodic axis ends in a partial tile, whose seam can bring
+    // tiles closer than their tile separation; such layouts are checked site by site.
+    bool seam = false;
+    for (int a = 0; a < nd; ++a) seam = seam || (periodic[a] && g.dims[a] % r.tile_edge[a] != 0);
+    if (seam) {
+        auto separation = [&](int a, int ta, int tb) {
+            const int n = g.dims[a], e = r.tile_edge[a];
+            const int a0 = ta * e, a1 = std::min(a0 + e, n) - 1, b0 = tb * e, b1 = std::min(b0 + e, n) - 1;
+            if (ta == tb) return 0;
+            const int lo0 = ta < tb ? a0 : b0, lo1 = ta < tb ? a1 : b1, hi0 = ta < tb ? b0 : a0, hi1 = ta < tb ? b1 : a1;
+            int sep = hi0 - lo1;
+            if (periodic[a]) sep = std::min(sep, lo0 + n - hi1);
+            return sep;
+        };
+        for (uint64_t t = 0; t < ntiles; ++t) {
+            int c[3];
+            tile_coords(t, c);
+            around(t, 1, [&](uint64_t u) {
+                if (u == t || r.tile_colors[(size_t)u] != r.tile_colors[(size_t)t]) return;
+                int q[3];
+                tile_coords(u, q);
+                bool near = true;
+                for (int a = 0; a < nd; ++a) near = near && separation(a, c[a], q[a]) <= halo;
+                if (near)
+                    fail(CPM_E_UNSUPPORTED, "tiles of one colour meet within the halo across a periodic seam; "
+                                            "choose a tile size that divides the periodic axes");
+            });
+        }
+    }
+    r.rounds = maxc;
+    r.color_offsets.assign((size_t)maxc + 1, 0);
+    r.color_tiles.assign((size_t)ntiles, 0);
+    {
+        std::vector<uint32_t> count((size_t)maxc + 2, 0);
+        for (uint64_t t = 0; t < ntiles; ++t) count[r.tile_colors[(size_t)t]]++;
+        uint32_t at = 0;
+        for (int col = 1; col <= maxc; ++col) {
+            r.color_offsets[(size_t)col - 1] = at;
+            at += count[(size_t)col];
+            r.tiled_max_tiles = std::max(r.tiled_max_tiles, count[(size_t)col]);
+        }
+        r.color_offsets[(size_t)maxc] = at;
+        std::vector<uint32_t> fill(r.color_offsets.begin(), r.color_offsets.end());
+        for (uint64_t t = 0; t < ntiles; ++t) r.color_tiles[fill[r.tile_colors[(size_t)t] - 1]++] = (uint32_t)(t + 1);
+    }
+    r.tile_offsets.assign((size_t)ntiles + 1, 0);
+    r.tile_sites.clear();
+    r.tile_sites.reserve((size_t)r.nmut);
+    uint32_t maxsites = 0;
+    const int X = g.dims[0], Y = g.dims[1], Z = g.dims[2];
+    for (uint64_t t = 0; t < ntiles; ++t) {
+        r.tile_offsets[(size_t)t] = (uint32_t)r.tile_sites.size();
+        int c[3];
+        tile_coords(t, c);
+        const int x0 = c[0] * r.tile_edge[0], y0 = c[1] * r.tile_edge[1], z0 = c[2] * r.tile_edge[2];
+        const int x1 = std::min(x0 + r.tile_edge[0], X), y1 = std::min(y0 + r.tile_edge[1], Y),
+                  z1 = std::min(z0 + r.tile_edge[2], Z);
+        for (int z = z0; z < z1; ++z)
+            for (int y = y0; y < y1; ++y)
+                for (int x = x0; x < x1; ++x) {
+                    const int64_t i = x + (int64_t)X * (y + (int64_t)Y * z);
+                    if (!g.mask || g.mask[i]) r.tile_sites.push_back((uint32_t)(i + 1));
+                }
+        maxsites = std::max(maxsites, (uint32_t)r.tile_sites.size() - r.tile_offsets[(size_t)t]);
+    }
+    r.tile_offsets[(size_t)ntiles] = (uint32_t)r.tile_sites.size();
+    if (maxsites == 0) fail(CPM_E_INVALID, "a tiled domain requires a mutable site");
+    // the RNG operation field holds the 1-based proposal index in 12 bits
+    if (maxsites > 4095) fail(CPM_E_UNSUPPORTED, "tiles of more than 4095 mutable sites");
+    r.tiled_interval = d.switching_interval > 0 && (uint32_t)d.switching_interval < maxsites
+                           ? d.switching_interval : (int)maxsites;
+    if (r.tiled_interval > 64) fail(CPM_E_UNSUPPORTED, "switching intervals above 64 proposals per tile");
+    r.tiled_batches = (int)((maxsites + (uint32_t)r.tiled_interval - 1) / (uint32_t)r.tiled_interval);
+    if ((int64_t)maxc * r.tiled_batches > 255) fail(CPM_E_UNSUPPORTED, "tile colours times batches exceed 255 subrounds");
+    // realized colour of every site: the colour of its tile (0 for immutable sites)
+    r.colors.assign((size_t)r.n, 0);
+    for (uint64_t t = 0; t < ntiles; ++t)
+        for (uint32_t k = r.tile_offsets[(size_t)t]; k < r.tile_offsets[(size_t)t + 1]; ++k)
+            r.colors[r.tile_sites[k] - 1] = (uint8_t)r.tile_colors[(size_t)t];
+}
+
 void compile_replica(Replica &r) {
     const cpm_model_desc &d = r.desc;
     if (d.abi_version != CPM_ABI_VERSION) fail(CPM_E_INVALID, "model descriptor ABI version mismatch");
-    if (d.algorithm != CPM_ALG_CHECKERBOARD && d.algorithm != CPM_ALG_LOTTERY)
-        fail(CPM_E_UNSUPPORTED, "only CheckerboardSweepCPM and LotteryCPM run on this engine");
+    if (d.algorithm != CPM_ALG_CHECKERBOARD && d.algorithm != CPM_ALG_LOTTERY && d.algorithm != CPM_ALG_TILED)
+        fail(CPM_E_UNSUPPORTED, "only CheckerboardSweepCPM, LotteryCPM and TiledCheckerboardCPM run on this engine");
     if (d.ndim != 2 && d.ndim != 3) fail(CPM_E_INVALID, "Cartesian domains are two- or three-dimensional");
     r.desc.dims[2] = d.ndim == 2 ? 1 : d.dims[2];
     uint64_t n = 1;
@@ -294,6 +474,14 @@ void compile_replica(Replica &r) {
     if (!(d.temperature >= 0.0f) || !std::isfinite(d.temperature))
         fail(CPM_E_INVALID, "temperature must be finite and non-negative");
     if (d.capacity < 1 || d.capacity > 0x7ffffffeu) fail(CPM_E_INVALID, "capacity must be 1..2^31-2");
+    if (d.algorithm == CPM_ALG_TILED) {
+        // tiled_checkerboard.jl:27-60 and the tiled admission of init_scientific
+        for (int a = 0; a < 3; ++a)
+            if (d.tile_size[a] < 0 || (a < d.ndim && d.tile_size[a] > 0xffff) || (a >= d.ndim && d.tile_size[a] > 1))
+                fail(CPM_E_INVALID, "tile size needs one positive edge per lattice dimension");
+        if (d.switching_interval < 0) fail(CPM_E_INVALID, "switching interval must be positive");
+        if (d.n_mechanics > 0) fail(CPM_E_UNSUPPORTED, "TiledCheckerboardCPM does not carry mechanics");
+    }
     // mechanics (scientific_components.jl:162-182, 592-616; component checks of init_scientific)
     if (d.n_mechanics < 0 || d.n_mechanics > CPM_MAX_MECHANICS) fail(CPM_E_INVALID, "at most two mechanical components");
     for (int k = 0; k < d.n_mechanics; ++k) {
@@ -502,6 +690,8 @@ void compile_replica(Replica &r) {
                     }
         }
         r.per_site_color = !found;
+    } else if (d.algorithm == CPM_ALG_TILED) {
+        compile_tiled(r, g);
     } else {
         // lottery.jl:46-92: rounds = maximum realized conflict degree + 1
         r.degrees.assign((size_t)r.n, 0);
@@ -528,7 +718,7 @@ void compile_replica(Replica &r) {
     // selects the kernels for proposal relations other than 2 * ndim directions (the
     // fixed-count kernels keep the offsets in registers)
     const int shape = tile_shape_for(d.algorithm, d.ndim, r.halo);
-    r.config = shape | (d.proposal.count == 2 * d.ndim ? 0 : 16);
+    r.config = d.algorithm == CPM_ALG_TILED ? kTiledConfig : shape | (d.proposal.count == 2 * d.ndim ? 0 : 16);
     int cdim, chalo, tx, ty, tz;
     cpm_internal_tile_config(shape, &cdim, &chalo, &tx, &ty, &tz);
 
@@ -603,6 +793,8 @@ void compile_replica(Replica &r) {
         if (ok) r.config |= 32;
     }
     v.capacity = d.capacity;
+    v.tiled_interval = r.tiled_interval;
+    v.tiled_batches = r.tiled_batches;
     v.n_mech = d.n_mechanics;
     for (int k = 0; k < d.n_mechanics; ++k) {
         const cpm_mechanics_desc &m = d.mechanics[k];
@@ -752,6 +944,23 @@ void allocate_replica(Replica &r, hipStream_t stream) {
     r.d_cells = dev_alloc<cpm::CellRec>(cap1, r.bytes);
     r.d_ctype = dev_alloc<uint32_t>(cap1, r.bytes);
     r.d_mvol = dev_alloc<int32_t>(CPM_MAX_MEDIA, r.bytes);
+    if (r.desc.algorithm == CPM_ALG_TILED) {
+        auto upload = [&](uint32_t *&dst, const std::vector<uint32_t> &src) {
+            dst = dev_alloc<uint32_t>(std::max<size_t>(src.size(), 1), r.bytes);
+            if (!src.empty())
+                hip_check(hipMemcpyAsync(dst, src.data(), src.size() * 4, hipMemcpyHostToDevice, stream), "copy tiles");
+        };
+        upload(r.d_tile_sites, r.tile_sites);
+        upload(r.d_tile_offsets, r.tile_offsets);
+        upload(r.d_color_tiles, r.color_tiles);
+        upload(r.d_color_offsets, r.color_offsets);
+        r.d_tile_dvol = dev_alloc<int32_t>(cap1, r.bytes);
+        r.d_tile_dbnd = dev_alloc<int32_t>(cap1, r.bytes);
+        r.d_tiled_counts = dev_alloc<unsigned long long>(8, r.bytes);
+        hip_check(hipMemsetAsync(r.d_tile_dvol, 0, cap1 * 4, stream), "memset");
+        hip_check(hipMemsetAsync(r.d_tile_dbnd, 0, cap1 * 4, stream), "memset");
+        hip_check(hipMemsetAsync(r.d_tiled_counts, 0, 8 * 8, stream), "memset");
+    }
     hip_check(hipMemsetAsync(r.d_lattice, 0, (size_t)r.n * 4, stream), "memset");
     hip_check(hipMemsetAsync(r.d_cells, 0, cap1 * sizeof(cpm::CellRec), stream), "memset");
     hip_check(hipMemsetAsync(r.d_ctype, 0, cap1 * 4, stream), "memset");
@@ -765,6 +974,13 @@ void allocate_replica(Replica &r, hipStream_t stream) {
     v.cells = r.d_cells;
     v.ctype = r.d_ctype;
     v.medium_vol = r.d_mvol;
+    v.tile_sites = r.d_tile_sites;
+    v.tile_offsets = r.d_tile_offsets;
+    v.color_tiles = r.d_color_tiles;
+    v.color_offsets = r.d_color_offsets;
+    v.tile_dvol = r.d_tile_dvol;
+    v.tile_dbnd = r.d_tile_dbnd;
+    v.tiled_counts = r.d_tiled_counts;
 }
 
 // Upper bound on the device bytes a replica adds to a batch: its own buffers plus its share
@@ -788,7 +1004,8 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     const uint64_t cap1 = (uint64_t)d.capacity + 1;
     const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + 2 * n /* colour lists or degree table */ +
                          tiles * (rounds + 1) * 4 /* colour ranges */ +
-                         cap1 * (sizeof(cpm::CellRec) + 4) + CPM_MAX_MEDIA * 4;
+                         cap1 * (sizeof(cpm::CellRec) + 4) + CPM_MAX_MEDIA * 4 +
+                         (d.algorithm == CPM_ALG_TILED ? 3 * n * 4 + 16 + cap1 * 8 + 64 : 0) /* tile lists */;
     const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + sizeof(cpm::TileInfo) +
                                      rounds * cpm::kSlotFields * 4) +
                             2 * cap1 * 8;
@@ -800,6 +1017,7 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
 // ---------------------------------------------------------------------------
 struct Group {
     int config = 0;
+    bool tiled() const { return config == kTiledConfig; }
     int tx = 0, ty = 0, tz = 0;
     std::vector<int> slots;
     uint32_t ntiles = 0, total_claims = 0, seg_cap = 0;
@@ -1037,6 +1255,18 @@ struct Engine {
     }
 
     void rebuild(Group &g) {
+        if (g.tiled()) {
+            // TiledCheckerboardCPM replicas keep their own layout and counters: no arenas
+            g.ntiles = (uint32_t)g.slots.size();
+            g.max_rounds = 0;
+            std::sort(g.slots.begin(), g.slots.end());
+            for (int s : g.slots) upload_rep(s);
+            g.dirty = false;
+            g.dirty_arena_fresh = false;
+            metrics[4] = bytes();
+            drop_graph();
+            return;
+        }
         cache_reports(g);
         std::sort(g.slots.begin(), g.slots.end());
         uint32_t tiles = 0, claims = 0;
@@ -1137,6 +1367,10 @@ struct Engine {
         metrics[0]++;
         for (Group &g : groups) {
             if (g.ntiles == 0) continue;
+            if (g.tiled()) {
+                enqueue_tiled(g);
+                continue;
+            }
             for (int p = 0; p < g.max_rounds; ++p) {
                 const cpm::PassArgs a = args(g, (uint32_t)p, (uint32_t)(g.pass_counter & 1));
                 // mechanics: half-interval OU steps before and after the pass
@@ -1156,6 +1390,26 @@ struct Engine {
                 metrics[0] += 3;
                 g.pass_counter++;
             }
+        }
+    }
+
+    // TiledCheckerboardCPM MCS of every replica of g: per colour in the MCS's order and per
+    // batch of the switching interval, the tiles' proposals, then the reconciliation
+    // (tiled_checkerboard_device.jl:743-806).
+    void enqueue_tiled(const Group &g) {
+        for (int s : g.slots) {
+            const Replica *r = reps[s];
+            for (int o = 0; o < r->rounds; ++o)
+                for (int b = 0; b < r->tiled_batches; ++b) {
+                    mark(0);
+                    hip_check(cpm_internal_launch_tiled_batch(d_reps, d_dyn, s, (uint32_t)o, (uint32_t)b,
+                                                              r->tiled_max_tiles, (uint32_t)r->tiled_interval, stream),
+                              "tiled batch");
+                    mark(1);
+                    hip_check(cpm_internal_launch_tiled_reconcile(d_reps, d_dyn, s, r->desc.capacity, stream),
+                              "tiled reconcile");
+                    metrics[0] += 2;
+                }
         }
     }
 
@@ -1275,14 +1529,40 @@ struct Engine {
         if (mcs <= r->base_mcs) fail(CPM_E_STATE, "no MCS has completed since the state was set");
         const Group &g = groups[r->group];
         if (g.dirty_arena_fresh) fail(CPM_E_STATE, "replica accounting was discarded by a membership change");
-        hip_check(cpm_internal_launch_report(d_reps, g.d_slots, g.ntiles, slot, d_report, stream), "report");
-        metrics[0]++;
-        hip_check(hipMemcpyAsync(h_report, d_report, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                 stream), "copy report");
+        if (g.tiled()) {
+            // the replica's MCS counters, zeroed by the begin kernel of each MCS
+            hip_check(hipMemcpyAsync(h_report, r->d_tiled_counts, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     stream), "copy report");
+        } else {
+            hip_check(cpm_internal_launch_report(d_reps, g.d_slots, g.ntiles, slot, d_report, stream), "report");
+            metrics[0]++;
+            hip_check(hipMemcpyAsync(h_report, d_report, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     stream), "copy report");
+        }
         hip_check(hipStreamSynchronize(stream), "sync");
         metrics[1]++;
         metrics[3]++;
-        fill_report(r, mcs, h_report, out);
+        if (g.tiled()) fill_tiled_report(r, mcs, h_report, out);
+        else fill_report(r, mcs, h_report, out);
+    }
+
+    // TiledCheckerboardCPM report (tiled_checkerboard_device.jl:790-806): internal rounds
+    // colours x batches, every mutable site a candidate and activated, no conflicts.
+    void fill_tiled_report(Replica *r, uint64_t mcs, const unsigned long long *s, uint64_t out[12]) {
+        out[0] = mcs;
+        out[1] = (uint64_t)r->rounds * (uint64_t)r->tiled_batches;
+        out[2] = out[3] = (uint64_t)r->nmut;
+        out[4] = s[0];
+        out[5] = s[1];
+        out[6] = s[2];
+        out[7] = 0;
+        out[8] = 0;
+        out[9] = 0;
+        out[10] = s[3];
+        out[11] = s[4];
+        std::memcpy(r->cached_report, out, sizeof(r->cached_report));
+        r->cached_mcs = mcs;
+        r->report_cached = true;
     }
 
     // The 12-field report (sequential.jl:236-251) of replica r from its 7 summed counters.
@@ -1620,9 +1900,32 @@ int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order) {
 int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors) {
     if (!ctx || !colors) return CPM_E_INVALID;
     const Replica *r = ctx->engine.reps[ctx->slot];
-    if (r->desc.algorithm != CPM_ALG_CHECKERBOARD) return CPM_E_STATE;
+    if (r->desc.algorithm == CPM_ALG_LOTTERY) return CPM_E_STATE;
     std::memcpy(colors, r->colors.data(), (size_t)r->n);
     return CPM_OK;
+}
+
+int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t reach[3], int32_t *halo,
+                         uint16_t *tile_colors, int32_t *color_count, int32_t *interval, int32_t *batches) {
+    if (!desc) {
+        g_create_error = "null argument";
+        return CPM_E_INVALID;
+    }
+    return guarded(&g_create_error, [&] {
+        if (desc->algorithm != CPM_ALG_TILED) fail(CPM_E_INVALID, "not a TiledCheckerboardCPM model");
+        Replica r;
+        r.desc = *desc;
+        compile_replica(r);  // host work only
+        for (int a = 0; a < 3; ++a) {
+            if (grid) grid[a] = r.tile_grid[a];
+            if (reach) reach[a] = r.tile_reach[a];
+        }
+        if (halo) *halo = r.tiled_halo;
+        if (tile_colors) std::memcpy(tile_colors, r.tile_colors.data(), r.tile_colors.size() * 2);
+        if (color_count) *color_count = r.rounds;
+        if (interval) *interval = r.tiled_interval;
+        if (batches) *batches = r.tiled_batches;
+    });
 }
 
 int32_t cpm_kernel_config(const cpm_ctx *ctx, int32_t *config) {
@@ -1730,7 +2033,7 @@ int32_t cpm_copy_probe(uint64_t bytes, int32_t reps, double *out_ms) {
 // ---------------- batch ----------------
 int32_t cpm_batch_create(int32_t algorithm, uint64_t hbm_budget_bytes, cpm_batch **out) {
     if (!out) return CPM_E_INVALID;
-    if (algorithm != CPM_ALG_CHECKERBOARD && algorithm != CPM_ALG_LOTTERY) {
+    if (algorithm != CPM_ALG_CHECKERBOARD && algorithm != CPM_ALG_LOTTERY && algorithm != CPM_ALG_TILED) {
         g_create_error = "unknown algorithm";
         return CPM_E_INVALID;
     }

@@ -990,10 +990,9 @@ __device__ __forceinline__ int32_t neighbour_word(const D &d, int x, int y, int 
     return realize_position(d, x + o[0], y + o[1], z + o[2]).word;
 }
 
-// evaluate_copy + ConventionalMetropolis + acceptance draw of one winning attempt against
-// the pass snapshot (checkerboard.jl:371-408; scientific_inner_loop.jl:213-269;
-// protocols/scientific.jl:372-385). Winners touch disjoint cells and non-adjacent sites,
-// so other winners' commits never change what this evaluation reads.
+// Evaluation of a copy attempt (checkerboard.jl:371-408; scientific_inner_loop.jl:213-269;
+// protocols/scientific.jl:372-385). Checkerboard and lottery winners touch disjoint cells and
+// non-adjacent sites, so other winners' commits never change what an evaluation reads.
 //
 // Memory latency dominates this function, so every load that depends only on the entry
 // (both cells' trackers and parameters, up to kNb surface and kNb contact neighbour words)
@@ -1033,10 +1032,21 @@ struct Staged {
     int64_t bl, bg;    // snapshot boundary measures (cell owners only)
 };
 
+// Changes added to the snapshot trackers of an attempt's two cells: a tile's own earlier
+// copies in a TiledCheckerboardCPM batch (the tile-local view), zero for the other algorithms.
+struct View {
+    int32_t dvl, dvg;  // volume changes of the losing and gaining cells
+    int32_t dbl, dbg;  // boundary-measure changes
+};
+
+// evaluate_copy of one actionable attempt (recipient `site`, losing owner l, gaining owner g)
+// against the trackers plus `view`: delta_h folded over the energy terms, plus the
+// mechanical work (scientific_inner_loop.jl:213-269). Fills the tracker values and boundary
+// changes a commit writes.
 template <class D>
-__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub, Staged *st) {
-    const int32_t l = e.losing, g = e.gaining;
-    const uint32_t idx0 = e.site - 1;
+__device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t l, int32_t g, const View &view,
+                                             Staged *st) {
+    const uint32_t idx0 = site - 1;
     const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
     const uint32_t zq = fast_div(idx0, div_xy);
     const uint32_t rem = idx0 - zq * d.div_xy.d;
@@ -1057,10 +1067,10 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
         qpg = __int_as_float(mg.x);
         qtg = __int_as_float(mg.y);
     }
-    const int32_t vl0 = cl.vol, vg0 = cg.vol;
+    const int32_t vl0 = cl.vol + view.dvl, vg0 = cg.vol + view.dvg;
     const double vs_l = cl.vs, tv_l = cl.tv, vs_g = cg.vs, tv_g = cg.tv;
     const double bs_l = cl.bs, bs_g = cg.bs;
-    const int64_t tb_l = cl.tb, tb_g = cg.tb, bm_l = cl.bnd, bm_g = cg.bnd;
+    const int64_t tb_l = cl.tb, tb_g = cg.tb, bm_l = cl.bnd + view.dbl, bm_g = cg.bnd + view.dbg;
     const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
     // sites at least `halo` from every face of a periodic, obstacle-free lattice reach
     // their neighbours by a linear delta
@@ -1200,16 +1210,25 @@ __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs,
         }
         mech = __fadd_rn(mech, r);
     }
-    const float T = d.temperature;
-    const float dd = __fadd_rn(dh, mech);
-    float p;
-    if (T == 0.0f) {
-        p = __fadd_rn(dd, 0.0f) <= 0.0f ? 1.0f : 0.0f;
-    } else {
-        float lr = __fdiv_rn(-__fadd_rn(dd, 0.0f), T);
-        lr = __fadd_rn(__fadd_rn(lr, 0.0f), 0.0f);
-        p = lr >= 0.0f ? 1.0f : (float)exp((double)lr);
-    }
+    return __fadd_rn(dh, mech);
+}
+
+// ConventionalMetropolis acceptance probability of an energy change at temperature T
+// (protocols/scientific.jl:372-385); exp of the Float32 log-ratio in double precision,
+// rounded once.
+__device__ __forceinline__ float metropolis_p(float T, float dd) {
+    if (T == 0.0f) return __fadd_rn(dd, 0.0f) <= 0.0f ? 1.0f : 0.0f;
+    float lr = __fdiv_rn(-__fadd_rn(dd, 0.0f), T);
+    lr = __fadd_rn(__fadd_rn(lr, 0.0f), 0.0f);
+    return lr >= 0.0f ? 1.0f : (float)exp((double)lr);
+}
+
+// evaluate_copy + ConventionalMetropolis + acceptance draw of one winning attempt against
+// the pass snapshot (checkerboard.jl:371-408).
+template <class D>
+__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub, Staged *st) {
+    const View none{0, 0, 0, 0};
+    const float p = metropolis_p(d.temperature, copy_energy(d, e.site, e.losing, e.gaining, none, st));
     const Words w = rng_words(d.key_lo, d.key_hi, kStreamAcceptance, mcs, sub, 0, kSiteEntity, e.site, 0, 0);
     return uniform_open01(w.w0) < p;
 }
@@ -1494,6 +1513,158 @@ __global__ __launch_bounds__(kBlock) void mech_init_kernel(const RepDev *reps, i
     }
 }
 
+// ---------------------------------------------------------------------------
+// TiledCheckerboardCPM (tiled_checkerboard_device.jl:407-586, 688-705, 743-806)
+// ---------------------------------------------------------------------------
+// One thread per tile of the colour: the tile's proposals of one batch run in order, each
+// against the batch snapshot of the trackers plus the tile's own earlier changes, and
+// accepted copies write ownership at once (tiles of one colour lie beyond each other's
+// halo, so no tile reads what another writes). The tile's changes are kept per touched
+// cell in LDS, one column per thread, and summed into per-cell accumulators at the end;
+// the reconcile kernel adds them to the trackers after the batch.
+constexpr int kTiledBlock = 64;
+
+// Packed (volume change << 16) + boundary change of one touched cell; both stay far inside
+// 16 bits (at most 64 copies of at most 128 surface directions each).
+__device__ __forceinline__ int32_t tiled_boundary_part(int32_t packed) { return (int32_t)(int16_t)(packed & 0xffff); }
+__device__ __forceinline__ int32_t tiled_volume_part(int32_t packed) {
+    return (packed - tiled_boundary_part(packed)) >> 16;
+}
+
+__global__ __launch_bounds__(kTiledBlock) void tiled_batch_kernel(const RepDev *reps, const RepDyn *dyn, int slot,
+                                                                  uint32_t ordinal, uint32_t batch) {
+    extern __shared__ int32_t s_touched[];  // [2 * interval] cell ids, then as many packed changes
+    const RepDev &d = reps[slot];
+    const RepDyn &y = dyn[slot];
+    if (!y.active) return;
+    const uint32_t color = y.order[ordinal];
+    const uint32_t t0 = d.color_offsets[color - 1], t1 = d.color_offsets[color];
+    const uint32_t k = t0 + blockIdx.x * kTiledBlock + threadIdx.x;
+    const uint32_t interval = (uint32_t)d.tiled_interval;
+    uint32_t n_real = 0, n_same = 0, n_null = 0, n_rej = 0, n_acc = 0;
+    if (k < t1) {
+        const uint32_t tile = d.color_tiles[k];
+        const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
+        const uint32_t start = batch * interval + 1;
+        const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
+        const uint32_t entries = 2 * interval;
+        int32_t *s_cell = s_touched + threadIdx.x;
+        int32_t *s_chg = s_touched + entries * kTiledBlock + threadIdx.x;
+        uint32_t touched = 0;
+        auto find = [&](int32_t c) -> int {
+            for (uint32_t j = 0; j < touched; ++j)
+                if (s_cell[j * kTiledBlock] == c) return (int)j;
+            return -1;
+        };
+        auto add = [&](int32_t c, int32_t dv, int32_t db) {
+            int j = find(c);
+            if (j < 0) {
+                j = (int)touched++;
+                s_cell[j * kTiledBlock] = c;
+                s_chg[j * kTiledBlock] = 0;
+            }
+            s_chg[j * kTiledBlock] += dv * 65536 + db;
+        };
+        CPM_GLOBAL int32_t *lat = gp(d.lattice);
+        const uint32_t stop = min(start + interval - 1, count);
+        for (uint32_t lp = start; lp <= stop; ++lp) {
+            const uint32_t sel = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
+                                              tile, 0, count);
+            const uint32_t rec = d.tile_sites[first + sel];
+            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
+                                              tile, 1, (uint32_t)d.prop.n);
+            const uint32_t idx0 = rec - 1;
+            const uint32_t X = (uint32_t)d.dims[0], XY = X * (uint32_t)d.dims[1];
+            const int z = (int)(idx0 / XY), yy = (int)((idx0 % XY) / X), x = (int)(idx0 % X);
+            const Realized donor = realize_position(d, x + d.prop.off[dir][0], yy + d.prop.off[dir][1],
+                                                    z + d.prop.off[dir][2]);
+            const int32_t l = lat[idx0];
+            if (donor.site == 0 || !donor.mut) {
+                ++n_null;
+                continue;
+            }
+            const int32_t g = donor.word;
+            if (g == l) {
+                ++n_same;
+                continue;
+            }
+            ++n_real;
+            View view{0, 0, 0, 0};
+            if (l > 0) {
+                const int j = find(l);
+                if (j >= 0) {
+                    view.dvl = tiled_volume_part(s_chg[j * kTiledBlock]);
+                    view.dbl = tiled_boundary_part(s_chg[j * kTiledBlock]);
+                }
+            }
+            if (g > 0) {
+                const int j = find(g);
+                if (j >= 0) {
+                    view.dvg = tiled_volume_part(s_chg[j * kTiledBlock]);
+                    view.dbg = tiled_boundary_part(s_chg[j * kTiledBlock]);
+                }
+            }
+            Staged st;
+            const float p = metropolis_p(d.temperature, copy_energy(d, rec, l, g, view, &st));
+            const Words w = rng_words(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity, tile, 0, 2);
+            if (!(uniform_open01(w.w0) < p)) {
+                ++n_rej;
+                continue;
+            }
+            ++n_acc;
+            lat[idx0] = g;
+            if (l > 0) add(l, -1, st.dl);
+            else atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
+            if (g > 0) add(g, 1, st.dg);
+            else atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
+        }
+        for (uint32_t j = 0; j < touched; ++j) {
+            const int32_t c = s_cell[j * kTiledBlock], q = s_chg[j * kTiledBlock];
+            const int32_t dv = tiled_volume_part(q), db = tiled_boundary_part(q);
+            if (dv != 0) atomicAdd(&d.tile_dvol[c], dv);
+            if (db != 0) atomicAdd(&d.tile_dbnd[c], db);
+        }
+    }
+    uint32_t v[5] = {n_real, n_same, n_null, n_rej, n_acc};
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 5; ++i)
+            if (v[i]) atomicAdd(&d.tiled_counts[i], (unsigned long long)v[i]);
+}
+
+// After each colour batch: trackers = snapshot + the summed tile changes, accumulators
+// cleared, and active cells left with zero volume retire (tiled_checkerboard_device.jl:
+// 688-705; _retire_zero_volume_cells!).
+__global__ __launch_bounds__(kBlock) void tiled_reconcile_kernel(const RepDev *reps, const RepDyn *dyn, int slot) {
+    const RepDev &d = reps[slot];
+    if (!dyn[slot].active) return;
+    const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
+    if (cell > d.capacity) return;
+    CPM_GLOBAL CellRec *c = gp(d.cells) + cell;
+    const int32_t dv = d.tile_dvol[cell], db = d.tile_dbnd[cell];
+    int32_t vol = c->vol;
+    if ((dv | db) != 0) {
+        d.tile_dvol[cell] = 0;
+        d.tile_dbnd[cell] = 0;
+        vol += dv;
+        c->vol = vol;
+        c->bnd = c->bnd + db;
+    }
+    if (vol == 0 && c->active) {
+        c->active = 0;
+        c->ctype = 0;
+        gp(d.ctype)[cell] = 0;
+        c->tv = d.retire_tv;
+        c->vs = d.retire_vs;
+        c->tb = d.retire_tb;
+        c->bs = d.retire_bs;
+        c->qp = 0.0f;
+        c->qt = 0.0f;
+    }
+}
+
 // Starts one MCS for every replica below its target: mcs += 1 and the colour / round
 // order of that MCS (checkerboard.jl:226-242, lottery.jl:124-140).
 __global__ void begin_mcs_kernel(const RepDev *reps, RepDyn *dyn, int nslots) {
@@ -1508,10 +1679,15 @@ __global__ void begin_mcs_kernel(const RepDev *reps, RepDyn *dyn, int nslots) {
     y.mcs = m;
     y.active = 1;
     const RepDev &d = reps[r];
-    if (d.algorithm == CPM_ALG_LOTTERY)
+    if (d.algorithm == CPM_ALG_LOTTERY) {
         schedule_order(d.key_lo, d.key_hi, kStreamLotteryActivation, 1, m, d.rounds, y.order);
-    else
+    } else if (d.algorithm == CPM_ALG_TILED) {
+        // tile colour order (tiled_checkerboard_device.jl:743-760) and fresh MCS counters
+        schedule_order(d.key_lo, d.key_hi, kStreamTiledOrder, 0, m, d.rounds, y.order);
+        for (int i = 0; i < 5; ++i) d.tiled_counts[i] = 0;
+    } else {
         schedule_order(d.key_lo, d.key_hi, kStreamCheckerboardOrder, 0, m, d.rounds, y.order);
+    }
 }
 
 // Sets the host-controlled fields of one replica's dynamic state.
@@ -1661,6 +1837,25 @@ extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::Re
                                                 int nslots, hipStream_t stream) {
     const int blocks = (nslots + 63) / 64;
     hipLaunchKernelGGL(cpm::begin_mcs_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, nslots);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+                                                      uint32_t ordinal, uint32_t batch, uint32_t max_tiles,
+                                                      uint32_t interval, hipStream_t stream) {
+    const uint32_t blocks = (max_tiles + cpm::kTiledBlock - 1) / cpm::kTiledBlock;
+    if (blocks == 0) return hipSuccess;
+    const size_t lds = (size_t)4 * interval * cpm::kTiledBlock * sizeof(int32_t);
+    hipLaunchKernelGGL(cpm::tiled_batch_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn, slot,
+                       ordinal, batch);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+                                                          uint32_t capacity, hipStream_t stream) {
+    const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(cpm::tiled_reconcile_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot);
     return hipGetLastError();
 }
 

@@ -1,6 +1,6 @@
 /*
  * cpm_mi355x: C-ABI of the MI355X (gfx950) engine for CorePotts' data-parallel
- * Metropolis sweeps (CheckerboardSweepCPM and LotteryCPM), their Philox4x32-10
+ * Metropolis sweeps (CheckerboardSweepCPM, LotteryCPM and TiledCheckerboardCPM), their Philox4x32-10
  * semantic RNG, copy-attempt energy evaluation and volume/boundary tracker updates.
  *
  * Plain pointers and sizes only. Every function returns 0 on success and a
@@ -18,7 +18,8 @@
  *                       lib/CorePotts/src/sciml/interface.jl:607-631
  *   cpm_step            SciMLBase.step!(integrator, n) -> perform_scientific_mcs!
  *                       lib/CorePotts/src/algorithms/sequential.jl:599-609,
- *                       checkerboard.jl:564-575, lottery.jl:328-395 (asynchronous, no host sync)
+ *                       checkerboard.jl:564-575, lottery.jl:328-395, tiled_checkerboard_device.jl:
+ *                       743-806 (asynchronous, no host sync)
  *   cpm_report          current_mcs_report(integrator) -> ScientificMCSReport (12 x UInt64)
  *                       sequential.jl:236-251,611-623; checkerboard.jl:577-586; lottery.jl:397-415
  *   cpm_download_state  logical_state(integrator) / logical_snapshot(plan, state)
@@ -67,7 +68,7 @@ enum cpm_status {
 
 enum cpm_face { CPM_FACE_PERIODIC = 0, CPM_FACE_CLOSED = 1, CPM_FACE_FIXED = 2 };
 enum cpm_term { CPM_TERM_VOLUME = 0, CPM_TERM_CONTACT = 1, CPM_TERM_BOUNDARY = 2 };
-enum cpm_algorithm { CPM_ALG_CHECKERBOARD = 1, CPM_ALG_LOTTERY = 2 };
+enum cpm_algorithm { CPM_ALG_CHECKERBOARD = 1, CPM_ALG_LOTTERY = 2, CPM_ALG_TILED = 3 };
 /* Mechanical components (lib/CorePotts/src/components/scientific_components.jl:130-240,
  * 575-660) and their initialization (MechanicalInitialization, :130-134). */
 enum cpm_mechanics { CPM_MECH_PRESSURE = 1, CPM_MECH_TENSION = 2 };
@@ -138,6 +139,11 @@ typedef struct cpm_model_desc {
     double retire_boundary_strength;
     int32_t n_mechanics;           /* ScientificComponentSet.mechanics, tuple order */
     cpm_mechanics_desc mechanics[CPM_MAX_MECHANICS];
+    /* TiledCheckerboardCPM (tiled_checkerboard.jl:1-60): tile edge per axis (0 = the
+     * reference default, 4 in 2D and 2 in 3D) and switching interval (0 = nothing: the
+     * site count of the largest tile) */
+    int32_t tile_size[3];
+    int32_t switching_interval;
 } cpm_model_desc;
 
 typedef struct cpm_ctx cpm_ctx;
@@ -183,10 +189,19 @@ int32_t cpm_step(cpm_ctx *ctx, uint64_t n);
 int32_t cpm_report(cpm_ctx *ctx, uint64_t out[12]);
 int32_t cpm_sync(cpm_ctx *ctx);
 
-/* Colour (checkerboard) or round (lottery) count, and the order used by the last MCS.
- * order may be NULL; it needs round_count entries. Synchronizes. */
+/* Colour (checkerboard), round (lottery) or tile-colour (tiled) count, and the order used
+ * by the last MCS. order may be NULL; it needs round_count entries. Synchronizes. */
 int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order);
-/* Realized colour of every site (0 for immutable sites), N bytes. Checkerboard only. */
+/* TiledCheckerboardCPM layout of a model, computed on the host (no device needed):
+ * tiled_layout(algorithm, dims; halo_radius, periodic) (tiled_checkerboard.jl:166-188) with
+ * the halo of _tiled_required_halo (:258-296) and the device batch schedule
+ * (tiled_checkerboard_device.jl:28-121). grid and reach per axis, halo radius, the halo
+ * colour of every tile (grid[0]*grid[1]*grid[2] entries, first axis fastest; may be NULL),
+ * colour count, switching interval and batches per colour. Errors as cpm_create. */
+int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t reach[3], int32_t *halo,
+                         uint16_t *tile_colors, int32_t *color_count, int32_t *interval, int32_t *batches);
+/* Realized colour of every site (0 for immutable sites), N bytes: the site's own colour
+ * (checkerboard) or the halo colour of its tile (tiled). Not for lottery. */
 int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors);
 /* Kernel configuration of the context's lattice: tile shape id (bits 0-3), kernels for
  * proposal relations of other than 2 * ndim directions (bit 4), row propose kernel (bit 5).
