@@ -68,7 +68,7 @@ EXPORTS = (
     "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
     "cpm_set_row_kernels", "cpm_kernel_config", "cpm_upload_mechanics", "cpm_download_mechanics",
     "cpm_initialize_mechanics", "cpm_batch_upload_mechanics", "cpm_batch_download_mechanics",
-    "cpm_batch_initialize_mechanics", "cpm_tiled_layout",
+    "cpm_batch_initialize_mechanics", "cpm_tiled_layout", "cpm_model_colors",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_batch_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(P)]
     lib.cpm_batch_destroy.argtypes = [P]
     lib.cpm_replica_bytes.argtypes = [C.POINTER(CpmModelDesc), C.POINTER(C.c_uint64)]
+    lib.cpm_model_colors.argtypes = [C.POINTER(CpmModelDesc), P, C.POINTER(C.c_int32)]
     lib.cpm_tiled_layout.argtypes = [C.POINTER(CpmModelDesc), P, P, C.POINTER(C.c_int32), P,
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.cpm_batch_add.argtypes = [P, C.POINTER(CpmModelDesc), C.POINTER(C.c_int32)]
@@ -445,6 +446,20 @@ def ensemble_seed(master_seed: int, trajectory: int, rerun: int = 1) -> int:
     if not 1 <= rerun <= 256:
         raise ValueError("ensemble rerun identity exceeds the v1 address domain")
     return int(load_library().cpm_ensemble_seed(int(master_seed), int(trajectory), int(rerun)))
+
+
+def model_colors(spec: ModelSpec):
+    """(colours per site, colour count) of a checkerboard or tiled model as the engine compiles
+    it, on the host (greedy realized colouring, checkerboard.jl:57-90)."""
+    lib = load_library()
+    keep: list = []
+    desc = make_desc(spec, keep)
+    colors = np.zeros(int(np.prod(spec.dims)), np.uint8)
+    count = C.c_int32()
+    rc = lib.cpm_model_colors(C.byref(desc), colors.ctypes.data, C.byref(count))
+    if rc != 0:
+        raise EngineError(rc, lib.cpm_last_error(None).decode())
+    return colors, count.value
 
 
 def tiled_layout(spec: ModelSpec) -> dict:

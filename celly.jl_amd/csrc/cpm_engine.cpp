@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cpm_mi355x.h"
@@ -595,24 +596,34 @@ void compile_replica(Replica &r) {
     const int hh = r.halo;
     bool wide = !g.mask;
     for (int a = 0; a < d.ndim; ++a) wide = wide && g.dims[a] >= 2 * hh + 1;
-    // visits every site in ascending order with its coordinates; fn(i, x, y, z, interior)
-    auto for_sites = [&](auto &&fn) {
-        int64_t i = 0;
-        for (int z = 0; z < g.dims[2]; ++z)
-            for (int y = 0; y < g.dims[1]; ++y) {
+    // Layers along the slowest axis: rows in 2D, planes in 3D.
+    const int layer_axis = d.ndim - 1;
+    const int n_layers = g.dims[layer_axis];
+    const int64_t layer_size = r.n / n_layers;
+    // visits the sites of layers [l0, l1) in ascending order with their coordinates;
+    // fn(i, x, y, z, interior)
+    auto for_layers = [&](int l0, int l1, auto &&fn) {
+        const int z0 = layer_axis == 2 ? l0 : 0, z1 = layer_axis == 2 ? l1 : g.dims[2];
+        const int y0 = layer_axis == 1 ? l0 : 0, y1 = layer_axis == 1 ? l1 : g.dims[1];
+        for (int z = z0; z < z1; ++z)
+            for (int y = y0; y < y1; ++y) {
                 const bool iny = wide && y >= hh && y < g.dims[1] - hh &&
                                  (d.ndim == 2 || (z >= hh && z < g.dims[2] - hh));
+                int64_t i = (int64_t)g.dims[0] * (y + (int64_t)g.dims[1] * z);
                 for (int x = 0; x < g.dims[0]; ++x, ++i)
                     fn(i, x, y, z, iny && x >= hh && x < g.dims[0] - hh);
             }
     };
+    auto for_sites = [&](auto &&fn) { for_layers(0, n_layers, fn); };
     if (d.algorithm == CPM_ALG_CHECKERBOARD) {
         // checkerboard.jl:57-90: greedy smallest free colour, ascending site order
         r.colors.assign((size_t)r.n, 0);
         int maxcolor = 0;
         std::vector<char> forbidden;
         uint8_t *colors = r.colors.data();
-        for_sites([&](int64_t i, int, int, int, bool interior) {
+        // the smallest colour no conflict neighbour holds; neighbours of higher index are
+        // still uncoloured (0) when a site is visited in ascending order
+        auto colour_site = [&](int64_t i, int, int, int, bool interior) {
             if (g.mask && !g.mask[i]) return;
             int col;
             if (kconf + 1 <= 63) {
@@ -638,17 +649,69 @@ void compile_replica(Replica &r) {
             if (col > 255) fail(CPM_E_UNSUPPORTED, "realized colouring exceeds 255 colours");
             colors[i] = (uint8_t)col;
             maxcolor = std::max(maxcolor, col);
-        });
+        };
+        // Every layer at least `hh` from both ends of the layer axis is coloured by one
+        // deterministic function of the `hh` layers before it and of its own obstacles (its
+        // conflict neighbours of lower index lie in those layers or earlier in its own layer).
+        // When the state (the last `hh` layers) repeats exactly, each following obstacle-free
+        // layer equals the layer one period before it if that one is obstacle-free too; such
+        // layers are copied, the others coloured site by site. The last `hh` layers see layer
+        // 0 onward across a periodic face and are always coloured site by site. The result
+        // equals the site-by-site greedy colouring. source[m] is the layer m repeats (or -1).
+        std::vector<int> source((size_t)n_layers, -1);
+        std::vector<char> layer_free((size_t)n_layers, 1);
+        if (g.mask)
+            for (int l = 0; l < n_layers; ++l)
+                for (int64_t k = 0; k < layer_size && layer_free[(size_t)l]; ++k)
+                    layer_free[(size_t)l] = g.mask[(int64_t)l * layer_size + k] != 0;
+        {
+            std::unordered_map<uint64_t, int> seen;
+            const size_t state_bytes = (size_t)hh * (size_t)layer_size;
+            for (int l = 0; l < n_layers; ++l) {
+                if (source[(size_t)l] < 0) for_layers(l, l + 1, colour_site);
+                if (l + 1 < hh || l + 1 >= n_layers - hh) continue;
+                const uint8_t *state = colors + (int64_t)(l + 1 - hh) * layer_size;
+                uint64_t hsh = 0x9e3779b97f4a7c15ull;
+                for (size_t k = 0; k < state_bytes; k += 8) {
+                    uint64_t w = 0;
+                    std::memcpy(&w, state + k, std::min<size_t>(8, state_bytes - k));
+                    hsh = mix64(hsh ^ w);
+                }
+                auto it = seen.find(hsh);
+                if (it == seen.end()) {
+                    seen.emplace(hsh, l);
+                    continue;
+                }
+                if (std::memcmp(colors + (int64_t)(it->second + 1 - hh) * layer_size, state, state_bytes) != 0) continue;
+                const int period = l - it->second;
+                int m = l + 1;
+                for (; m < n_layers - hh && layer_free[(size_t)m] && layer_free[(size_t)(m - period)]; ++m) {
+                    std::memcpy(colors + (int64_t)m * layer_size, colors + (int64_t)(m - period) * layer_size,
+                                (size_t)layer_size);
+                    source[(size_t)m] = m - period;
+                }
+                l = m - 1;  // continues with the first layer not copied
+            }
+        }
         r.rounds = maxcolor;
         // periodic pattern detection: colour == f(x mod px, y mod py, z mod pz). Each
         // candidate is first probed on the sites within 2 of a face (where wrap seams break
-        // periodicity), then verified on every site.
+        // periodicity), then verified on every site. A repeated layer matches when the layer
+        // it repeats does and the pattern's period along the layer axis divides the
+        // repetition period, so the verification skips such layers.
         auto matches = [&](int px, int py, int pz, std::vector<uint8_t> &table, bool probe) {
+            const int pl = layer_axis == 1 ? py : pz;
+            auto repeated = [&](int l) {
+                const int src = source[(size_t)l];
+                return !probe && src >= 0 && (l - src) % pl == 0;
+            };
             int rz = 0;
             for (int z = 0; z < g.dims[2]; ++z, rz = (rz + 1 == pz ? 0 : rz + 1)) {
                 const bool ez = d.ndim == 3 && (z < 2 || z >= g.dims[2] - 2);
+                if (layer_axis == 2 && repeated(z)) continue;
                 int ry = 0;
                 for (int y = 0; y < g.dims[1]; ++y, ry = (ry + 1 == py ? 0 : ry + 1)) {
+                    if (layer_axis == 1 && repeated(y)) continue;
                     const bool ey = ez || y < 2 || y >= g.dims[1] - 2;
                     const int64_t row = (int64_t)g.dims[0] * (y + (int64_t)g.dims[1] * z);
                     const int base = px * (ry + py * rz);
@@ -694,24 +757,31 @@ void compile_replica(Replica &r) {
         compile_tiled(r, g);
     } else {
         // lottery.jl:46-92: rounds = maximum realized conflict degree + 1
-        r.degrees.assign((size_t)r.n, 0);
         int maxdeg = 0;
-        for_sites([&](int64_t i, int, int, int, bool interior) {
-            if (g.mask && !g.mask[i]) return;
-            const int c = interior ? kconf : conflict_neighbours(g, r.conf_off, i, nb.data());
-            r.degrees[i] = (uint8_t)std::min(c, 255);
-            maxdeg = std::max(maxdeg, c);
-        });
-        if (maxdeg + 1 > 255) fail(CPM_E_UNSUPPORTED, "lottery conflict degree exceeds the subround domain");
-        r.rounds = maxdeg + 1;
-        r.uniform_degree = true;
-        for (int64_t i = 0; i < r.n; ++i) {
-            if (g.mask && !g.mask[i]) continue;
-            if (r.degrees[i] != maxdeg) {
-                r.uniform_degree = false;
-                break;
+        if (all_periodic && wide) {
+            // distinct offsets within the halo reach distinct sites from every site
+            maxdeg = kconf;
+            r.uniform_degree = true;
+            r.degrees.clear();
+        } else {
+            r.degrees.assign((size_t)r.n, 0);
+            for_sites([&](int64_t i, int, int, int, bool interior) {
+                if (g.mask && !g.mask[i]) return;
+                const int c = interior ? kconf : conflict_neighbours(g, r.conf_off, i, nb.data());
+                r.degrees[i] = (uint8_t)std::min(c, 255);
+                maxdeg = std::max(maxdeg, c);
+            });
+            r.uniform_degree = true;
+            for (int64_t i = 0; i < r.n; ++i) {
+                if (g.mask && !g.mask[i]) continue;
+                if (r.degrees[i] != maxdeg) {
+                    r.uniform_degree = false;
+                    break;
+                }
             }
         }
+        if (maxdeg + 1 > 255) fail(CPM_E_UNSUPPORTED, "lottery conflict degree exceeds the subround domain");
+        r.rounds = maxdeg + 1;
     }
 
     // tile shape by dimension and halo (process-wide preference, else the default); bit 4
@@ -1903,6 +1973,21 @@ int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors) {
     if (r->desc.algorithm == CPM_ALG_LOTTERY) return CPM_E_STATE;
     std::memcpy(colors, r->colors.data(), (size_t)r->n);
     return CPM_OK;
+}
+
+int32_t cpm_model_colors(const cpm_model_desc *desc, uint8_t *colors, int32_t *count) {
+    if (!desc) {
+        g_create_error = "null argument";
+        return CPM_E_INVALID;
+    }
+    return guarded(&g_create_error, [&] {
+        if (desc->algorithm == CPM_ALG_LOTTERY) fail(CPM_E_INVALID, "LotteryCPM has no realized colouring");
+        Replica r;
+        r.desc = *desc;
+        compile_replica(r);  // host work only
+        if (colors) std::memcpy(colors, r.colors.data(), (size_t)r.n);
+        if (count) *count = r.rounds;
+    });
 }
 
 int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t reach[3], int32_t *halo,

@@ -192,6 +192,11 @@ int32_t cpm_sync(cpm_ctx *ctx);
 /* Colour (checkerboard), round (lottery) or tile-colour (tiled) count, and the order used
  * by the last MCS. order may be NULL; it needs round_count entries. Synchronizes. */
 int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order);
+/* Realized colouring of a model, computed on the host as cpm_create computes it (no device
+ * needed): the greedy colour of every site in ascending site order (checkerboard.jl:57-90) or
+ * the halo colour of its tile (tiled), 0 for immutable sites; N bytes, and the colour count.
+ * Not for lottery. Errors as cpm_create. */
+int32_t cpm_model_colors(const cpm_model_desc *desc, uint8_t *colors, int32_t *count);
 /* TiledCheckerboardCPM layout of a model, computed on the host (no device needed):
  * tiled_layout(algorithm, dims; halo_radius, periodic) (tiled_checkerboard.jl:166-188) with
  * the halo of _tiled_required_halo (:258-296) and the device batch schedule
