@@ -41,6 +41,12 @@ def set_row_kernels(enable: bool):
     load_library().cpm_set_row_kernels(1 if enable else 0)
 
 
+def set_tiled_staging(enable: bool):
+    """The staged TiledCheckerboardCPM kernel (tile box and halo in LDS) on or off for models
+    created afterwards; on by default. Results do not depend on it."""
+    load_library().cpm_set_tiled_staging(1 if enable else 0)
+
+
 def model_halo(spec) -> int:
     """Largest |offset| component over the relations the kernels stage (at least 1)."""
     h = 1

@@ -66,7 +66,7 @@ EXPORTS = (
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
     "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
     "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
-    "cpm_set_row_kernels", "cpm_kernel_config", "cpm_upload_mechanics", "cpm_download_mechanics",
+    "cpm_set_row_kernels", "cpm_set_tiled_staging", "cpm_kernel_config", "cpm_upload_mechanics", "cpm_download_mechanics",
     "cpm_initialize_mechanics", "cpm_batch_upload_mechanics", "cpm_batch_download_mechanics",
     "cpm_batch_initialize_mechanics", "cpm_tiled_layout", "cpm_model_colors",
     "cpm_ensemble_seed", "cpm_batch_create", "cpm_batch_destroy", "cpm_batch_last_error",
@@ -113,6 +113,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_tile_shape_info.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
     lib.cpm_set_tile_shape.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     lib.cpm_set_row_kernels.argtypes = [C.c_int32]
+    lib.cpm_set_tiled_staging.argtypes = [C.c_int32]
     lib.cpm_kernel_config.argtypes = [P, C.POINTER(C.c_int32)]
     lib.cpm_ensemble_seed.restype = C.c_uint64
     lib.cpm_ensemble_seed.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
@@ -121,7 +122,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_replica_bytes.argtypes = [C.POINTER(CpmModelDesc), C.POINTER(C.c_uint64)]
     lib.cpm_model_colors.argtypes = [C.POINTER(CpmModelDesc), P, C.POINTER(C.c_int32)]
     lib.cpm_tiled_layout.argtypes = [C.POINTER(CpmModelDesc), P, P, C.POINTER(C.c_int32), P,
-                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]
     lib.cpm_batch_add.argtypes = [P, C.POINTER(CpmModelDesc), C.POINTER(C.c_int32)]
     lib.cpm_batch_remove.argtypes = [P, C.c_int32]
     lib.cpm_batch_upload_state.argtypes = [P, C.c_int32] + state_args
@@ -466,23 +468,25 @@ def tiled_layout(spec: ModelSpec) -> dict:
     """The engine's TiledCheckerboardCPM layout of a model, computed on the host
     (tiled_layout, tiled_checkerboard.jl:166-188; the batch schedule of
     tiled_checkerboard_device.jl:28-121): grid, reach, halo, tile colours (first axis
-    fastest), colour count, switching interval and batches per colour."""
+    fastest), colour count, switching interval, batches per colour and the positions of the
+    box a lane stages in LDS (0: the global-memory kernel)."""
     lib = load_library()
     keep: list = []
     desc = make_desc(spec, keep)
     grid = np.zeros(3, np.int32)
     reach = np.zeros(3, np.int32)
-    halo, count, interval, batches = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    halo, count, interval, batches, staged = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     rc = lib.cpm_tiled_layout(C.byref(desc), grid.ctypes.data, reach.ctypes.data, C.byref(halo), None,
-                              C.byref(count), C.byref(interval), C.byref(batches))
+                              C.byref(count), C.byref(interval), C.byref(batches), C.byref(staged))
     if rc != 0:
         raise EngineError(rc, lib.cpm_last_error(None).decode())
     colors = np.zeros(int(np.prod(grid)), np.uint16)
-    rc = lib.cpm_tiled_layout(C.byref(desc), None, None, None, colors.ctypes.data, None, None, None)
+    rc = lib.cpm_tiled_layout(C.byref(desc), None, None, None, colors.ctypes.data, None, None, None, None)
     if rc != 0:
         raise EngineError(rc, lib.cpm_last_error(None).decode())
     return dict(grid=tuple(int(v) for v in grid), reach=tuple(int(v) for v in reach), halo=halo.value,
-                colors=colors, color_count=count.value, interval=interval.value, batches=batches.value)
+                colors=colors, color_count=count.value, interval=interval.value, batches=batches.value,
+                staged=staged.value)
 
 
 def copy_probe(nbytes: int, reps: int = 10) -> float:

@@ -154,12 +154,17 @@ struct RepDev {
     // mutable sites of each tile (x fastest inside a tile), the tiles of each colour, the
     // per-cell tracker changes of one colour batch and the counters of the current MCS
     int32_t tiled_interval, tiled_batches;
+    int32_t tile_edge[3], tile_grid[3];
+    int32_t tiled_halo;             // radius of the declared accesses
+    int32_t tiled_stage[3];         // staged box per tile: edge + 2 halo (0 when not staged)
     const uint32_t *tile_sites;     // mutable sites (1-based), grouped by tile
     const uint32_t *tile_offsets;   // tiles + 1 ranges into tile_sites
     const uint32_t *color_tiles;    // tiles (1-based) grouped by colour, ascending
     const uint32_t *color_offsets;  // colours + 1 ranges into color_tiles
     int32_t *tile_dvol;             // cap + 1 volume changes (summed over the batch's tiles)
     int32_t *tile_dbnd;             // cap + 1 boundary-measure changes
+    int32_t *tile_spill;            // touched-cell entries past the LDS list, per tile of a colour
+    int32_t tile_spill_stride;      // words per tile: 2 * (2 * interval - LDS entries), or 0
     unsigned long long *tiled_counts;  // realized, same owner, boundary null, rejected, accepted
 };
 

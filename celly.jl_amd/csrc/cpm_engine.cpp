@@ -29,7 +29,7 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
 extern "C" hipError_t cpm_internal_launch_commit(const cpm::PassArgs *a, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
                                                       uint32_t ordinal, uint32_t batch, uint32_t max_tiles,
-                                                      uint32_t interval, hipStream_t stream);
+                                                      uint32_t interval, uint32_t staged, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
                                                           uint32_t capacity, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
@@ -139,6 +139,8 @@ struct Replica {
     // interval, batches per colour and the most tiles of any colour
     int tile_edge[3] = {1, 1, 1}, tile_grid[3] = {1, 1, 1}, tile_reach[3] = {0, 0, 0};
     int tiled_halo = 0, tiled_interval = 0, tiled_batches = 0;
+    int tiled_stage[3] = {0, 0, 0};  // staged box per tile (0: the global-memory kernel)
+    uint32_t tiled_staged = 0;       // positions of the staged box
     uint32_t tiled_max_tiles = 0;
     std::vector<uint16_t> tile_colors;
     std::vector<uint32_t> tile_sites, tile_offsets, color_tiles, color_offsets;
@@ -152,7 +154,7 @@ struct Replica {
     cpm::CellRec *d_cells = nullptr;
     uint32_t *d_ctype = nullptr;
     uint32_t *d_tile_sites = nullptr, *d_tile_offsets = nullptr, *d_color_tiles = nullptr, *d_color_offsets = nullptr;
-    int32_t *d_tile_dvol = nullptr, *d_tile_dbnd = nullptr;
+    int32_t *d_tile_dvol = nullptr, *d_tile_dbnd = nullptr, *d_tile_spill = nullptr;
     unsigned long long *d_tiled_counts = nullptr;
     uint64_t bytes = 0;
     uint64_t estimate = 0;      // estimate_replica_bytes at admission
@@ -167,7 +169,7 @@ struct Replica {
         dev_free(d_clist); dev_free(d_coff);
         dev_free(d_cells); dev_free(d_ctype); dev_free(d_mvol);
         dev_free(d_tile_sites); dev_free(d_tile_offsets); dev_free(d_color_tiles); dev_free(d_color_offsets);
-        dev_free(d_tile_dvol); dev_free(d_tile_dbnd); dev_free(d_tiled_counts);
+        dev_free(d_tile_dvol); dev_free(d_tile_dbnd); dev_free(d_tile_spill); dev_free(d_tiled_counts);
     }
 };
 
@@ -190,6 +192,10 @@ std::atomic<int> g_shape_pref[3][4][3];
 // Process-wide switch of the row kernels (2D periodic von Neumann checkerboards with a
 // 2 x 2 colour pattern); on by default.
 std::atomic<int> g_row_kernels{1};
+
+// Process-wide switch of the staged TiledCheckerboardCPM kernel (tile box and halo in LDS);
+// on by default, read when a model is compiled.
+std::atomic<int> g_tiled_staging{1};
 
 int default_shape(int ndim, int halo) { return ndim == 2 ? (halo == 1 ? 0 : 1) : (halo == 1 ? 2 : 3); }
 
@@ -281,6 +287,8 @@ int conflict_neighbours(const HostGeom &g, const std::vector<int32_t> &offs, int
 
 // Group configuration of TiledCheckerboardCPM replicas (outside the tile-shape ids 0..63).
 constexpr int kTiledConfig = 64;
+// Touched-cell entries the tiled kernels keep in LDS per tile (as cpm::kListLds).
+constexpr uint32_t kTiledListLds = 8;
 
 // Largest per-axis offset of a relation (its radius).
 int relation_radius(const cpm_relation_desc &rel, int ndim) { return max_abs_offset(rel, ndim); }
@@ -433,6 +441,22 @@ void compile_tiled(Replica &r, const HostGeom &g) {
                            ? d.switching_interval : (int)maxsites;
     if (r.tiled_interval > 64) fail(CPM_E_UNSUPPORTED, "switching intervals above 64 proposals per tile");
     r.tiled_batches = (int)((maxsites + (uint32_t)r.tiled_interval - 1) / (uint32_t)r.tiled_interval);
+    // staged kernel: box and halo of at most 64 positions, no site staged twice (a periodic axis
+    // at least as long as the staged extent), LDS within 64 KiB per workgroup
+    {
+        uint32_t positions = 1;
+        bool ok = g_tiled_staging.load() != 0;
+        for (int a = 0; a < 3; ++a) {
+            const int ext = a < nd ? r.tile_edge[a] + 2 * halo : 1;
+            r.tiled_stage[a] = ext;
+            positions *= (uint32_t)ext;
+            if (a < nd && periodic[a] && ext > g.dims[a]) ok = false;
+        }
+        const uint64_t lds = 3584 + (uint64_t)(positions + 2 * kTiledListLds) * 64 * 4;
+        ok = ok && positions <= 64 && lds <= 65536;
+        r.tiled_staged = ok ? positions : 0;
+        if (!ok) r.tiled_stage[0] = r.tiled_stage[1] = r.tiled_stage[2] = 0;
+    }
     if ((int64_t)maxc * r.tiled_batches > 255) fail(CPM_E_UNSUPPORTED, "tile colours times batches exceed 255 subrounds");
     // realized colour of every site: the colour of its tile (0 for immutable sites)
     r.colors.assign((size_t)r.n, 0);
@@ -865,6 +889,12 @@ void compile_replica(Replica &r) {
     v.capacity = d.capacity;
     v.tiled_interval = r.tiled_interval;
     v.tiled_batches = r.tiled_batches;
+    for (int a = 0; a < 3; ++a) {
+        v.tile_edge[a] = r.tile_edge[a];
+        v.tile_grid[a] = r.tile_grid[a];
+        v.tiled_stage[a] = r.tiled_stage[a];
+    }
+    v.tiled_halo = r.tiled_halo;
     v.n_mech = d.n_mechanics;
     for (int k = 0; k < d.n_mechanics; ++k) {
         const cpm_mechanics_desc &m = d.mechanics[k];
@@ -1027,6 +1057,11 @@ void allocate_replica(Replica &r, hipStream_t stream) {
         r.d_tile_dvol = dev_alloc<int32_t>(cap1, r.bytes);
         r.d_tile_dbnd = dev_alloc<int32_t>(cap1, r.bytes);
         r.d_tiled_counts = dev_alloc<unsigned long long>(8, r.bytes);
+        // touched-cell entries beyond the kernels' LDS list (at most two per proposal)
+        const uint32_t entries = 2 * (uint32_t)r.tiled_interval;
+        r.dev.tile_spill_stride = entries > kTiledListLds ? (int32_t)(2 * (entries - kTiledListLds)) : 0;
+        r.d_tile_spill = dev_alloc<int32_t>(std::max<size_t>((size_t)r.tiled_max_tiles * r.dev.tile_spill_stride, 1),
+                                            r.bytes);
         hip_check(hipMemsetAsync(r.d_tile_dvol, 0, cap1 * 4, stream), "memset");
         hip_check(hipMemsetAsync(r.d_tile_dbnd, 0, cap1 * 4, stream), "memset");
         hip_check(hipMemsetAsync(r.d_tiled_counts, 0, 8 * 8, stream), "memset");
@@ -1050,6 +1085,7 @@ void allocate_replica(Replica &r, hipStream_t stream) {
     v.color_offsets = r.d_color_offsets;
     v.tile_dvol = r.d_tile_dvol;
     v.tile_dbnd = r.d_tile_dbnd;
+    v.tile_spill = r.d_tile_spill;
     v.tiled_counts = r.d_tiled_counts;
 }
 
@@ -1473,7 +1509,8 @@ struct Engine {
                 for (int b = 0; b < r->tiled_batches; ++b) {
                     mark(0);
                     hip_check(cpm_internal_launch_tiled_batch(d_reps, d_dyn, s, (uint32_t)o, (uint32_t)b,
-                                                              r->tiled_max_tiles, (uint32_t)r->tiled_interval, stream),
+                                                              r->tiled_max_tiles, (uint32_t)r->tiled_interval,
+                                                              r->tiled_staged, stream),
                               "tiled batch");
                     mark(1);
                     hip_check(cpm_internal_launch_tiled_reconcile(d_reps, d_dyn, s, r->desc.capacity, stream),
@@ -1991,7 +2028,8 @@ int32_t cpm_model_colors(const cpm_model_desc *desc, uint8_t *colors, int32_t *c
 }
 
 int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t reach[3], int32_t *halo,
-                         uint16_t *tile_colors, int32_t *color_count, int32_t *interval, int32_t *batches) {
+                         uint16_t *tile_colors, int32_t *color_count, int32_t *interval, int32_t *batches,
+                         int32_t *staged) {
     if (!desc) {
         g_create_error = "null argument";
         return CPM_E_INVALID;
@@ -2009,6 +2047,7 @@ int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t re
         if (tile_colors) std::memcpy(tile_colors, r.tile_colors.data(), r.tile_colors.size() * 2);
         if (color_count) *color_count = r.rounds;
         if (interval) *interval = r.tiled_interval;
+        if (staged) *staged = (int32_t)r.tiled_staged;
         if (batches) *batches = r.tiled_batches;
     });
 }
@@ -2073,6 +2112,11 @@ int32_t cpm_set_tile_shape(int32_t algorithm, int32_t ndim, int32_t halo, int32_
             return CPM_E_INVALID;
     }
     g_shape_pref[algorithm][ndim][halo].store(shape >= 0 ? shape + 1 : 0);
+    return CPM_OK;
+}
+
+int32_t cpm_set_tiled_staging(int32_t enable) {
+    g_tiled_staging.store(enable ? 1 : 0);
     return CPM_OK;
 }
 

@@ -1039,67 +1039,24 @@ struct View {
     int32_t dbl, dbg;  // boundary-measure changes
 };
 
-// evaluate_copy of one actionable attempt (recipient `site`, losing owner l, gaining owner g)
-// against the trackers plus `view`: delta_h folded over the energy terms, plus the
+// evaluate_copy of one actionable attempt (losing owner l, gaining owner g) from its gathered
+// inputs: both cells' records and mechanical states, the first kNb surface and contact
+// neighbour words with the contact neighbours' types, `more(role, k)` for the words of
+// directions k >= kNb (role 0 surface, 1 contact) and `jf` / `jd` for the contact matrix.
+// Trackers are the records plus `view`: delta_h folded over the energy terms, plus the
 // mechanical work (scientific_inner_loop.jl:213-269). Fills the tracker values and boundary
 // changes a commit writes.
-template <class D>
-__device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t l, int32_t g, const View &view,
-                                             Staged *st) {
-    const uint32_t idx0 = site - 1;
-    const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
-    const uint32_t zq = fast_div(idx0, div_xy);
-    const uint32_t rem = idx0 - zq * d.div_xy.d;
-    const uint32_t yq = fast_div(rem, div_x);
-    const int x = (int)(rem - yq * d.div_x.d);
-    const int y = (int)yq;
-    const int z = (int)zq;
-    // slot 0 of every per-cell array is a spare, read in place of a medium owner
-    const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
-    const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
-    // mechanical state of both cells (volume_pressure, surface_tension)
-    float qpl = 0.0f, qtl = 0.0f, qpg = 0.0f, qtg = 0.0f;
-    if (d.n_mech > 0) {
-        const v4i ml = ((const CPM_GLOBAL v4i *)(gp(d.cells) + li))[3];
-        const v4i mg = ((const CPM_GLOBAL v4i *)(gp(d.cells) + gi))[3];
-        qpl = __int_as_float(ml.x);
-        qtl = __int_as_float(ml.y);
-        qpg = __int_as_float(mg.x);
-        qtg = __int_as_float(mg.y);
-    }
+template <class D, class More, class JF, class JD>
+__device__ __forceinline__ float copy_energy_from(const D &d, int32_t l, int32_t g, const View &view,
+                                                  const CellHot &cl, const CellHot &cg, float qpl, float qtl,
+                                                  float qpg, float qtg, const int32_t (&sw)[kNb],
+                                                  const int32_t (&cw)[kNb], const uint32_t (&ct)[kNb], More &&more,
+                                                  JF &&jf, JD &&jd, Staged *st) {
     const int32_t vl0 = cl.vol + view.dvl, vg0 = cg.vol + view.dvg;
     const double vs_l = cl.vs, tv_l = cl.tv, vs_g = cg.vs, tv_g = cg.tv;
     const double bs_l = cl.bs, bs_g = cg.bs;
     const int64_t tb_l = cl.tb, tb_g = cg.tb, bm_l = cl.bnd + view.dbl, bm_g = cg.bnd + view.dbg;
     const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
-    // sites at least `halo` from every face of a periodic, obstacle-free lattice reach
-    // their neighbours by a linear delta
-    const int h = d.halo;
-    const bool inner = d.simple && x >= h && x < d.dims[0] - h && y >= h && y < d.dims[1] - h &&
-                       (d.ndim == 2 || (z >= h && z < d.dims[2] - h));
-    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
-    auto nb = [&](const auto *o, int32_t lin) __attribute__((always_inline)) {
-        return inner ? lat[(int32_t)idx0 + lin] : neighbour_word(d, x, y, z, o);
-    };
-    // neighbour words: one unconditional load per direction (lanes off the fast path load
-    // their own site), so every load is in flight at once; lanes within `halo` of a face
-    // or on a lattice with walls or obstacles then realize their neighbours one by one
-    int32_t sw[kNb], cw[kNb];
-#pragma unroll
-    for (int u = 0; u < kNb; ++u) sw[u] = u < ns ? lat[(int32_t)idx0 + (inner ? d.surf_lin[u] : 0)] : 0;
-#pragma unroll
-    for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? lat[(int32_t)idx0 + (inner ? d.cont_lin[u] : 0)] : 0;
-    if (!inner) {
-#pragma unroll
-        for (int u = 0; u < kNb; ++u)
-            if (u < ns) sw[u] = neighbour_word(d, x, y, z, d.surf.off[u]);
-#pragma unroll
-        for (int u = 0; u < kNb; ++u)
-            if (u < nc) cw[u] = neighbour_word(d, x, y, z, d.cont.off[u]);
-    }
-    uint32_t ct[kNb];
-#pragma unroll
-    for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
     const int32_t vl = l > 0 ? vl0 : 0, vg = g > 0 ? vg0 : 0;
 
     // boundary tracker deltas (BoundaryEdgeCount, scientific_components.jl:466-481)
@@ -1111,8 +1068,7 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
     };
 #pragma unroll
     for (int u = 0; u < kNb; ++u) surface_word(sw[u]);
-    for (int k = kNb; k < ns; ++k)
-        surface_word(k < 32 ? nb(d.surf.off[k], d.surf_lin[k]) : neighbour_word(d, x, y, z, d.surf.off[k]));
+    for (int k = kNb; k < ns; ++k) surface_word(more(0, k));
     st->dl = dl;
     st->dg = dg;
     st->vl = vl;
@@ -1157,19 +1113,19 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
                 const uint32_t il = (lt - 1) * nt + (wt - 1), ig = (gt - 1) * nt + (wt - 1);
                 if (v64) {
                     const double wk = d.cont.wd[k];
-                    if (l != w) dd = __dsub_rn(dd, __dmul_rn(wk, d.Jd[il]));
-                    if (g != w) dd = __dadd_rn(dd, __dmul_rn(wk, d.Jd[ig]));
+                    if (l != w) dd = __dsub_rn(dd, __dmul_rn(wk, jd(il)));
+                    if (g != w) dd = __dadd_rn(dd, __dmul_rn(wk, jd(ig)));
                 } else {
                     const float wk = d.cont.wf[k];
-                    if (l != w) df = __fsub_rn(df, __fmul_rn(wk, d.Jf[il]));
-                    if (g != w) df = __fadd_rn(df, __fmul_rn(wk, d.Jf[ig]));
+                    if (l != w) df = __fsub_rn(df, __fmul_rn(wk, jf(il)));
+                    if (g != w) df = __fadd_rn(df, __fmul_rn(wk, jf(ig)));
                 }
             };
 #pragma unroll
             for (int u = 0; u < kNb; ++u)
                 if (u < nc) contact_term(u, cw[u], ct[u]);
             for (int k = kNb; k < nc; ++k) {
-                const int32_t w = k < 32 ? nb(d.cont.off[k], d.cont_lin[k]) : neighbour_word(d, x, y, z, d.cont.off[k]);
+                const int32_t w = more(1, k);
                 contact_term(k, w, gp(d.ctype)[w > 0 ? w : 0]);
             }
             v = v64 ? dd : (double)df;
@@ -1211,6 +1167,71 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
         mech = __fadd_rn(mech, r);
     }
     return __fadd_rn(dh, mech);
+}
+
+// evaluate_copy of one actionable attempt (recipient `site`, losing owner l, gaining owner g)
+// against the trackers in the cell records plus `view`, reading the lattice in global memory.
+template <class D>
+__device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t l, int32_t g, const View &view,
+                                             Staged *st) {
+    const uint32_t idx0 = site - 1;
+    const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
+    const uint32_t zq = fast_div(idx0, div_xy);
+    const uint32_t rem = idx0 - zq * d.div_xy.d;
+    const uint32_t yq = fast_div(rem, div_x);
+    const int x = (int)(rem - yq * d.div_x.d);
+    const int y = (int)yq;
+    const int z = (int)zq;
+    // slot 0 of every per-cell array is a spare, read in place of a medium owner
+    const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
+    const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
+    // mechanical state of both cells (volume_pressure, surface_tension)
+    float qpl = 0.0f, qtl = 0.0f, qpg = 0.0f, qtg = 0.0f;
+    if (d.n_mech > 0) {
+        const v4i ml = ((const CPM_GLOBAL v4i *)(gp(d.cells) + li))[3];
+        const v4i mg = ((const CPM_GLOBAL v4i *)(gp(d.cells) + gi))[3];
+        qpl = __int_as_float(ml.x);
+        qtl = __int_as_float(ml.y);
+        qpg = __int_as_float(mg.x);
+        qtg = __int_as_float(mg.y);
+    }
+    const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
+    // sites at least `halo` from every face of a periodic, obstacle-free lattice reach
+    // their neighbours by a linear delta
+    const int h = d.halo;
+    const bool inner = d.simple && x >= h && x < d.dims[0] - h && y >= h && y < d.dims[1] - h &&
+                       (d.ndim == 2 || (z >= h && z < d.dims[2] - h));
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    auto nb = [&](const auto *o, int32_t lin) __attribute__((always_inline)) {
+        return inner ? lat[(int32_t)idx0 + lin] : neighbour_word(d, x, y, z, o);
+    };
+    // neighbour words: one unconditional load per direction (lanes off the fast path load
+    // their own site), so every load is in flight at once; lanes within `halo` of a face
+    // or on a lattice with walls or obstacles then realize their neighbours one by one
+    int32_t sw[kNb], cw[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) sw[u] = u < ns ? lat[(int32_t)idx0 + (inner ? d.surf_lin[u] : 0)] : 0;
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? lat[(int32_t)idx0 + (inner ? d.cont_lin[u] : 0)] : 0;
+    if (!inner) {
+#pragma unroll
+        for (int u = 0; u < kNb; ++u)
+            if (u < ns) sw[u] = neighbour_word(d, x, y, z, d.surf.off[u]);
+#pragma unroll
+        for (int u = 0; u < kNb; ++u)
+            if (u < nc) cw[u] = neighbour_word(d, x, y, z, d.cont.off[u]);
+    }
+    uint32_t ct[kNb];
+#pragma unroll
+    for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
+    auto more = [&](int role, int k) __attribute__((always_inline)) {
+        if (role == 0) return k < 32 ? nb(d.surf.off[k], d.surf_lin[k]) : neighbour_word(d, x, y, z, d.surf.off[k]);
+        return k < 32 ? nb(d.cont.off[k], d.cont_lin[k]) : neighbour_word(d, x, y, z, d.cont.off[k]);
+    };
+    return copy_energy_from(
+        d, l, g, view, cl, cg, qpl, qtl, qpg, qtg, sw, cw, ct, more,
+        [&](uint32_t i) __attribute__((always_inline)) { return d.Jf[i]; },
+        [&](uint32_t i) __attribute__((always_inline)) { return d.Jd[i]; }, st);
 }
 
 // ConventionalMetropolis acceptance probability of an energy change at temperature T
@@ -1531,40 +1552,119 @@ __device__ __forceinline__ int32_t tiled_volume_part(int32_t packed) {
     return (packed - tiled_boundary_part(packed)) >> 16;
 }
 
+// The touched cells of one tile and their packed changes: one LDS column per lane (stride
+// kTiledBlock), at most two entries per proposal.
+constexpr uint32_t kListLds = 8;  // touched cells kept in LDS; the rest spill to global memory
+
+struct TileList {
+    int32_t *cell, *chg;              // LDS columns (stride kTiledBlock)
+    CPM_GLOBAL int32_t *spill;        // (cell, change) pairs of entries kListLds onward
+    uint32_t touched;
+
+    __device__ __forceinline__ int find(int32_t c) const {
+        const uint32_t nl = min(touched, kListLds);
+        for (uint32_t j = 0; j < nl; ++j)
+            if (cell[j * kTiledBlock] == c) return (int)j;
+        for (uint32_t j = kListLds; j < touched; ++j)
+            if (spill[2 * (j - kListLds)] == c) return (int)j;
+        return -1;
+    }
+    __device__ __forceinline__ int32_t change(int j) const {
+        return (uint32_t)j < kListLds ? chg[j * kTiledBlock] : spill[2 * ((uint32_t)j - kListLds) + 1];
+    }
+    __device__ __forceinline__ void add(int32_t c, int32_t dv, int32_t db) {
+        int j = find(c);
+        const int32_t q = dv * 65536 + db;
+        if (j < 0) {
+            j = (int)touched++;
+            if ((uint32_t)j < kListLds) {
+                cell[j * kTiledBlock] = c;
+                chg[j * kTiledBlock] = q;
+            } else {
+                spill[2 * ((uint32_t)j - kListLds)] = c;
+                spill[2 * ((uint32_t)j - kListLds) + 1] = q;
+            }
+        } else if ((uint32_t)j < kListLds) {
+            chg[j * kTiledBlock] += q;
+        } else {
+            spill[2 * ((uint32_t)j - kListLds) + 1] += q;
+        }
+    }
+    // the tile-local view of the attempt's cells (media carry no trackers)
+    __device__ __forceinline__ View view(int32_t l, int32_t g) const {
+        View v{0, 0, 0, 0};
+        if (l > 0) {
+            const int j = find(l);
+            if (j >= 0) {
+                const int32_t q = change(j);
+                v.dvl = tiled_volume_part(q);
+                v.dbl = tiled_boundary_part(q);
+            }
+        }
+        if (g > 0) {
+            const int j = find(g);
+            if (j >= 0) {
+                const int32_t q = change(j);
+                v.dvg = tiled_volume_part(q);
+                v.dbg = tiled_boundary_part(q);
+            }
+        }
+        return v;
+    }
+    // the tile's changes join the colour batch's per-cell sums
+    template <class D>
+    __device__ __forceinline__ void flush(const D &d) const {
+        for (uint32_t j = 0; j < touched; ++j) {
+            const int32_t c = j < kListLds ? cell[j * kTiledBlock] : spill[2 * (j - kListLds)];
+            const int32_t q = change((int)j);
+            const int32_t dv = tiled_volume_part(q), db = tiled_boundary_part(q);
+            if (dv != 0) atomicAdd(&d.tile_dvol[c], dv);
+            if (db != 0) atomicAdd(&d.tile_dbnd[c], db);
+        }
+    }
+};
+
+// Wave sums of the per-lane proposal counters into the replica's MCS counters.
+template <class D>
+__device__ __forceinline__ void tiled_count(const D &d, uint32_t (&v)[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 5; ++i)
+            if (v[i]) atomicAdd(&d.tiled_counts[i], (unsigned long long)v[i]);
+}
+
+// Ownership change and tracker bookkeeping of an accepted tile copy.
+template <class D>
+__device__ __forceinline__ void tiled_commit(const D &d, TileList &list, int32_t l, int32_t g, const Staged &st) {
+    if (l > 0) list.add(l, -1, st.dl);
+    else atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
+    if (g > 0) list.add(g, 1, st.dg);
+    else atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
+}
+
+// Any domain: recipients from the tile's site list, the lattice read in global memory.
 __global__ __launch_bounds__(kTiledBlock) void tiled_batch_kernel(const RepDev *reps, const RepDyn *dyn, int slot,
                                                                   uint32_t ordinal, uint32_t batch) {
-    extern __shared__ int32_t s_touched[];  // [2 * interval] cell ids, then as many packed changes
-    const RepDev &d = reps[slot];
-    const RepDyn &y = dyn[slot];
+    extern __shared__ int32_t s_touched[];  // kListLds cell ids, then as many packed changes
+    // descriptor and MCS state are not written by this kernel: constant address space, so
+    // they stay in scalar loads however the lattice is written
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)dyn)[slot];
     if (!y.active) return;
     const uint32_t color = y.order[ordinal];
     const uint32_t t0 = d.color_offsets[color - 1], t1 = d.color_offsets[color];
     const uint32_t k = t0 + blockIdx.x * kTiledBlock + threadIdx.x;
     const uint32_t interval = (uint32_t)d.tiled_interval;
-    uint32_t n_real = 0, n_same = 0, n_null = 0, n_rej = 0, n_acc = 0;
+    uint32_t v[5] = {0, 0, 0, 0, 0};  // realized, same owner, boundary null, rejected, accepted
     if (k < t1) {
         const uint32_t tile = d.color_tiles[k];
         const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
         const uint32_t start = batch * interval + 1;
         const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
-        const uint32_t entries = 2 * interval;
-        int32_t *s_cell = s_touched + threadIdx.x;
-        int32_t *s_chg = s_touched + entries * kTiledBlock + threadIdx.x;
-        uint32_t touched = 0;
-        auto find = [&](int32_t c) -> int {
-            for (uint32_t j = 0; j < touched; ++j)
-                if (s_cell[j * kTiledBlock] == c) return (int)j;
-            return -1;
-        };
-        auto add = [&](int32_t c, int32_t dv, int32_t db) {
-            int j = find(c);
-            if (j < 0) {
-                j = (int)touched++;
-                s_cell[j * kTiledBlock] = c;
-                s_chg[j * kTiledBlock] = 0;
-            }
-            s_chg[j * kTiledBlock] += dv * 65536 + db;
-        };
+        TileList list{s_touched + threadIdx.x, s_touched + kListLds * kTiledBlock + threadIdx.x,
+                      gp(d.tile_spill) + (size_t)(k - t0) * (uint32_t)d.tile_spill_stride, 0};
         CPM_GLOBAL int32_t *lat = gp(d.lattice);
         const uint32_t stop = min(start + interval - 1, count);
         for (uint32_t lp = start; lp <= stop; ++lp) {
@@ -1580,66 +1680,180 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_batch_kernel(const RepDev *
                                                     z + d.prop.off[dir][2]);
             const int32_t l = lat[idx0];
             if (donor.site == 0 || !donor.mut) {
-                ++n_null;
+                ++v[2];
                 continue;
             }
             const int32_t g = donor.word;
             if (g == l) {
-                ++n_same;
+                ++v[1];
                 continue;
             }
-            ++n_real;
-            View view{0, 0, 0, 0};
-            if (l > 0) {
-                const int j = find(l);
-                if (j >= 0) {
-                    view.dvl = tiled_volume_part(s_chg[j * kTiledBlock]);
-                    view.dbl = tiled_boundary_part(s_chg[j * kTiledBlock]);
-                }
-            }
-            if (g > 0) {
-                const int j = find(g);
-                if (j >= 0) {
-                    view.dvg = tiled_volume_part(s_chg[j * kTiledBlock]);
-                    view.dbg = tiled_boundary_part(s_chg[j * kTiledBlock]);
-                }
-            }
+            ++v[0];
             Staged st;
-            const float p = metropolis_p(d.temperature, copy_energy(d, rec, l, g, view, &st));
+            const float p = metropolis_p(d.temperature, copy_energy(d, rec, l, g, list.view(l, g), &st));
             const Words w = rng_words(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity, tile, 0, 2);
             if (!(uniform_open01(w.w0) < p)) {
-                ++n_rej;
+                ++v[3];
                 continue;
             }
-            ++n_acc;
+            ++v[4];
             lat[idx0] = g;
-            if (l > 0) add(l, -1, st.dl);
-            else atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
-            if (g > 0) add(g, 1, st.dg);
-            else atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
+            tiled_commit(d, list, l, g, st);
         }
-        for (uint32_t j = 0; j < touched; ++j) {
-            const int32_t c = s_cell[j * kTiledBlock], q = s_chg[j * kTiledBlock];
-            const int32_t dv = tiled_volume_part(q), db = tiled_boundary_part(q);
-            if (dv != 0) atomicAdd(&d.tile_dvol[c], dv);
-            if (db != 0) atomicAdd(&d.tile_dbnd[c], db);
-        }
+        list.flush(d);
     }
-    uint32_t v[5] = {n_real, n_same, n_null, n_rej, n_acc};
+    tiled_count(d, v);
+}
+
+// LDS bytes of the staged kernel before the per-lane columns: the contact matrix (Float32
+// and Float64) and the proposal directions as offsets inside a staged box.
+constexpr int kTiledStagedHead = CPM_MAX_TYPES * CPM_MAX_TYPES * (4 + 8) + CPM_MAX_DIRS * 4;
+
+// Tiles whose box plus halo holds at most 64 positions, none of them the same site twice: the
+// lane stages its box and halo in LDS once (owner words; a bit mask of the positions a
+// proposal may copy from), reads every neighbour there and writes its own copies there and
+// to the lattice. Global reads per realized proposal: the two cell records and the contact
+// neighbours' types. Recipients of an obstacle-free tile come from the box arithmetic.
+__global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev *reps, const RepDyn *dyn, int slot,
+                                                                   uint32_t ordinal, uint32_t batch) {
+    extern __shared__ int32_t s_mem[];
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)dyn)[slot];
+    if (!y.active) return;
+    float *s_jf = (float *)s_mem;
+    double *s_jd = (double *)(s_mem + CPM_MAX_TYPES * CPM_MAX_TYPES);
+    int32_t *s_pd = s_mem + CPM_MAX_TYPES * CPM_MAX_TYPES * 3;
+    int32_t *s_lane = s_mem + kTiledStagedHead / 4;
+    const int SX = d.tiled_stage[0], SY = d.tiled_stage[1], SZ = d.tiled_stage[2];
+    const int H = d.tiled_halo, HZ = d.ndim == 3 ? H : 0;
+    for (int i = threadIdx.x; i < d.n_types * d.n_types; i += kTiledBlock) {
+        s_jf[i] = d.Jf[i];
+        s_jd[i] = d.Jd[i];
+    }
+    for (int i = threadIdx.x; i < d.prop.n; i += kTiledBlock)
+        s_pd[i] = d.prop.off[i][0] + SX * (d.prop.off[i][1] + SY * d.prop.off[i][2]);
+    __syncthreads();
+    const uint32_t color = y.order[ordinal];
+    const uint32_t t0 = d.color_offsets[color - 1], t1 = d.color_offsets[color];
+    const uint32_t k = t0 + blockIdx.x * kTiledBlock + threadIdx.x;
+    const uint32_t interval = (uint32_t)d.tiled_interval;
+    const uint32_t S = (uint32_t)(SX * SY * SZ);
+    uint32_t v[5] = {0, 0, 0, 0, 0};  // realized, same owner, boundary null, rejected, accepted
+    if (k < t1) {
+        const uint32_t tile = d.color_tiles[k];
+        const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
+        const uint32_t start = batch * interval + 1;
+        const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
+        const int X = d.dims[0], Y = d.dims[1];
+        const uint32_t tc = tile - 1, gx = (uint32_t)d.tile_grid[0], gy = (uint32_t)d.tile_grid[1];
+        const int ox = (int)(tc % gx) * d.tile_edge[0], oy = (int)((tc / gx) % gy) * d.tile_edge[1],
+                  oz = (int)(tc / (gx * gy)) * d.tile_edge[2];
+        const int bx = min(d.tile_edge[0], X - ox), by = min(d.tile_edge[1], Y - oy);
+        int32_t *s_w = s_lane + threadIdx.x;
+        TileList list{s_lane + S * kTiledBlock + threadIdx.x, s_lane + (S + kListLds) * kTiledBlock + threadIdx.x,
+                      gp(d.tile_spill) + (size_t)(k - t0) * (uint32_t)d.tile_spill_stride, 0};
+        // stage the box and its halo: eight positions' loads in flight at a time
+        uint64_t donors = 0;
+        for (uint32_t p0 = 0; p0 < S; p0 += 8) {
+            int32_t w[8];
+            bool m[8];
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
-    if (threadIdx.x == 0)
-        for (int i = 0; i < 5; ++i)
-            if (v[i]) atomicAdd(&d.tiled_counts[i], (unsigned long long)v[i]);
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t p = p0 + j;
+                w[j] = 0;
+                m[j] = false;
+                if (p < S) {
+                    const int sx = (int)(p % (uint32_t)SX), sy = (int)((p / (uint32_t)SX) % (uint32_t)SY),
+                              sz = (int)(p / (uint32_t)(SX * SY));
+                    const Realized rz = realize_position(d, ox - H + sx, oy - H + sy, oz - HZ + sz);
+                    w[j] = rz.word;
+                    m[j] = rz.site != 0 && rz.mut;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (p0 + j < S) {
+                    s_w[(p0 + j) * kTiledBlock] = w[j];
+                    if (m[j]) donors |= 1ull << (p0 + j);
+                }
+        }
+        const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
+        auto delta = [&](const auto *o) __attribute__((always_inline)) { return o[0] + SX * (o[1] + SY * o[2]); };
+        CPM_GLOBAL int32_t *lat = gp(d.lattice);
+        const uint32_t stop = min(start + interval - 1, count);
+        for (uint32_t lp = start; lp <= stop; ++lp) {
+            const uint32_t sel = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
+                                              tile, 0, count);
+            int lx, ly, lz;
+            uint32_t rec;
+            if (d.mask == nullptr) {
+                // the tile's sites in order, first axis fastest
+                lx = (int)(sel % (uint32_t)bx);
+                ly = (int)((sel / (uint32_t)bx) % (uint32_t)by);
+                lz = (int)(sel / (uint32_t)(bx * by));
+                rec = 1u + (uint32_t)(ox + lx) + (uint32_t)X * ((uint32_t)(oy + ly) + (uint32_t)Y * (uint32_t)(oz + lz));
+            } else {
+                rec = d.tile_sites[first + sel];
+                const uint32_t i0 = rec - 1, XY = (uint32_t)X * (uint32_t)Y;
+                lz = (int)(i0 / XY) - oz;
+                ly = (int)((i0 % XY) / (uint32_t)X) - oy;
+                lx = (int)(i0 % (uint32_t)X) - ox;
+            }
+            const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
+                                              tile, 1, (uint32_t)d.prop.n);
+            const int pr = (lx + H) + SX * ((ly + H) + SY * (lz + HZ));
+            const int pd = pr + s_pd[dir];
+            const int32_t l = s_w[pr * kTiledBlock];
+            if (!((donors >> pd) & 1ull)) {
+                ++v[2];
+                continue;
+            }
+            const int32_t g = s_w[pd * kTiledBlock];
+            if (g == l) {
+                ++v[1];
+                continue;
+            }
+            ++v[0];
+            const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
+            const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
+            int32_t sw[kNb], cw[kNb];
+            uint32_t ct[kNb];
+#pragma unroll
+            for (int u = 0; u < kNb; ++u) sw[u] = u < ns ? s_w[(pr + delta(d.surf.off[u])) * kTiledBlock] : 0;
+#pragma unroll
+            for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? s_w[(pr + delta(d.cont.off[u])) * kTiledBlock] : 0;
+#pragma unroll
+            for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
+            auto more = [&](int role, int q) __attribute__((always_inline)) {
+                return s_w[(pr + delta(role == 0 ? d.surf.off[q] : d.cont.off[q])) * kTiledBlock];
+            };
+            Staged st;
+            const float dd = copy_energy_from(
+                d, l, g, list.view(l, g), cl, cg, 0.0f, 0.0f, 0.0f, 0.0f, sw, cw, ct, more,
+                [&](uint32_t i) __attribute__((always_inline)) { return s_jf[i]; },
+                [&](uint32_t i) __attribute__((always_inline)) { return s_jd[i]; }, &st);
+            const float p = metropolis_p(d.temperature, dd);
+            const Words w = rng_words(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity, tile, 0, 2);
+            if (!(uniform_open01(w.w0) < p)) {
+                ++v[3];
+                continue;
+            }
+            ++v[4];
+            lat[rec - 1] = g;
+            s_w[pr * kTiledBlock] = g;
+            tiled_commit(d, list, l, g, st);
+        }
+        list.flush(d);
+    }
+    tiled_count(d, v);
 }
 
 // After each colour batch: trackers = snapshot + the summed tile changes, accumulators
 // cleared, and active cells left with zero volume retire (tiled_checkerboard_device.jl:
 // 688-705; _retire_zero_volume_cells!).
 __global__ __launch_bounds__(kBlock) void tiled_reconcile_kernel(const RepDev *reps, const RepDyn *dyn, int slot) {
-    const RepDev &d = reps[slot];
-    if (!dyn[slot].active) return;
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
+    if (!((const CPM_CONST RepDyn *)dyn)[slot].active) return;
     const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
     if (cell > d.capacity) return;
     CPM_GLOBAL CellRec *c = gp(d.cells) + cell;
@@ -1840,14 +2054,22 @@ extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::Re
     return hipGetLastError();
 }
 
+// staged = positions of a tile's staged box (0: the kernel reading the lattice in global memory)
 extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
                                                       uint32_t ordinal, uint32_t batch, uint32_t max_tiles,
-                                                      uint32_t interval, hipStream_t stream) {
+                                                      uint32_t interval, uint32_t staged, hipStream_t stream) {
     const uint32_t blocks = (max_tiles + cpm::kTiledBlock - 1) / cpm::kTiledBlock;
     if (blocks == 0) return hipSuccess;
-    const size_t lds = (size_t)4 * interval * cpm::kTiledBlock * sizeof(int32_t);
-    hipLaunchKernelGGL(cpm::tiled_batch_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn, slot,
-                       ordinal, batch);
+    (void)interval;
+    if (staged) {
+        const size_t lds = cpm::kTiledStagedHead + (size_t)(staged + 2 * cpm::kListLds) * cpm::kTiledBlock * sizeof(int32_t);
+        hipLaunchKernelGGL(cpm::tiled_staged_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn,
+                           slot, ordinal, batch);
+    } else {
+        const size_t lds = (size_t)2 * cpm::kListLds * cpm::kTiledBlock * sizeof(int32_t);
+        hipLaunchKernelGGL(cpm::tiled_batch_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn,
+                           slot, ordinal, batch);
+    }
     return hipGetLastError();
 }
 

@@ -202,9 +202,16 @@ int32_t cpm_model_colors(const cpm_model_desc *desc, uint8_t *colors, int32_t *c
  * the halo of _tiled_required_halo (:258-296) and the device batch schedule
  * (tiled_checkerboard_device.jl:28-121). grid and reach per axis, halo radius, the halo
  * colour of every tile (grid[0]*grid[1]*grid[2] entries, first axis fastest; may be NULL),
- * colour count, switching interval and batches per colour. Errors as cpm_create. */
+ * colour count, switching interval, batches per colour and the positions of the box a lane
+ * stages in LDS (0: the kernel that reads the lattice in global memory). Errors as cpm_create. */
 int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t reach[3], int32_t *halo,
-                         uint16_t *tile_colors, int32_t *color_count, int32_t *interval, int32_t *batches);
+                         uint16_t *tile_colors, int32_t *color_count, int32_t *interval, int32_t *batches,
+                         int32_t *staged);
+/* Process-wide choice of the TiledCheckerboardCPM batch kernel for models compiled after the
+ * call: nonzero (default) stages each tile's box and halo in LDS where it holds at most 64
+ * positions, zero always reads the lattice in global memory. Both give identical results. No
+ * reference counterpart (an implementation choice, like cpm_set_row_kernels). */
+int32_t cpm_set_tiled_staging(int32_t enable);
 /* Realized colour of every site (0 for immutable sites), N bytes: the site's own colour
  * (checkerboard) or the halo colour of its tile (tiled). Not for lottery. */
 int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors);

@@ -41,7 +41,11 @@ def test_engine_layout_matches_the_oracle(case):
     # internal rounds of one MCS: colours x batches (tiled_checkerboard_device.jl:790-806)
     sim.step(1)
     assert sim.report()[1] == eng["color_count"] * eng["batches"]
-    tile_sites = int(np.prod(tile or ((4, 4) if len(dims) == 2 else (2, 2, 2))))  # every tile can be full
+    edges = tile or ((4, 4) if len(dims) == 2 else (2, 2, 2))
+    # the staged kernel holds a box plus halo of at most 64 positions
+    box = int(np.prod([e + 2 * halo for e in edges]))
+    assert eng["staged"] == (box if box <= 64 else 0)
+    tile_sites = int(np.prod(edges))  # every tile can be full
     expected_interval = min(interval, tile_sites) if interval else tile_sites
     assert eng["interval"] == expected_interval
 
