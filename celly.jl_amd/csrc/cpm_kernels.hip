@@ -102,7 +102,7 @@ __device__ inline Realized realize_position(const D &d, int gx, int gy, int gz) 
 }
 
 template <int DIM, int H, int TX, int TY, int TZ, int NT = kBlock>
-__device__ inline void load_tile(const RepDev &d, int x0, int y0, int z0, int32_t *s_own,
+__device__ inline void load_tile(const CPM_CONST RepDev &d, int x0, int y0, int z0, int32_t *s_own,
                                  uint8_t *s_mut, uint32_t *s_sid) {
     using G = Geom<DIM, H, TX, TY, TZ>;
     const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
@@ -214,6 +214,11 @@ __device__ __forceinline__ void block_sum4(uint32_t v[4], uint32_t (*s_red)[kBlo
 // Energy terms (scientific_components.jl, scientific_trackers.jl)
 // ---------------------------------------------------------------------------
 
+// A bounded-draw constant read from the descriptor in the constant address space.
+__device__ __forceinline__ Bound const_bound(const CPM_CONST Bound &b) {
+    return Bound{b.threshold, FastDiv{b.div.d, b.div.m}};
+}
+
 // Cell type or declared medium type, clamped into 1..n_types (the host rejects models
 // and states that could reach an undeclared type).
 template <class D>
@@ -277,7 +282,7 @@ __device__ __forceinline__ double bnd_change_f64(double s, double t, double m, i
 // Publish the claim key of an actionable attempt on both cells (one 64-bit atomic max
 // per cell replaces the max-priority and min-identity claims, checkerboard.jl:281-335)
 // and append the attempt to the tile segment.
-__device__ __forceinline__ void stage_claim(bool act, const Entry &ent, const RepDev &d,
+__device__ __forceinline__ void stage_claim(bool act, const Entry &ent, const CPM_CONST RepDev &d,
                                             unsigned long long *claims, uint32_t *s_nseg,
                                             Entry *seg) {
     if (act) {
@@ -383,8 +388,10 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
     const uint32_t r = ti.rep;
-    const RepDev &d = a.reps[r];
-    const RepDyn &y = a.dyn[r];
+    // descriptor and MCS state are not written by this kernel (constant address space:
+    // scalar loads that stay valid across the kernel's own stores)
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[r];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -514,7 +521,7 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
             const uint32_t site = 1u + (uint32_t)(x0 + ix) +
                                   (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz));
             dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity, site, 0,
-                               d.prop_bound);
+                               const_bound(d.prop_bound));
             const int q = pos + direction_offset<KF>(dir, po, s_ploff);
             if (!simple && !s_mut[q]) {
                 c23 += 1u;
@@ -671,7 +678,7 @@ __device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand,
 // strip runs its direction draws (ProposalDirectionStream) over its compacted boundary
 // sites and its priority draws and claims over its compacted actionable attempts.
 template <int RX, int TX, int TY>
-__device__ __forceinline__ void row_wave(const PassArgs &a, const RepDev &d, const TileInfo &ti, uint32_t tile,
+__device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti, uint32_t tile,
                                          uint64_t mcs, int ry, int wave, uint32_t *s_site, int32_t *s_own,
                                          int32_t (*s_nb)[2 * 64], uint32_t &nsites, uint32_t &nsame,
                                          uint32_t &nact) {
@@ -741,8 +748,8 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
-    const RepDev &d = a.reps[ti.rep];
-    const RepDyn &y = a.dyn[ti.rep];
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[ti.rep];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[ti.rep];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -804,8 +811,10 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
     const uint32_t r = ti.rep;
-    const RepDev &d = a.reps[r];
-    const RepDyn &y = a.dyn[r];
+    // descriptor and MCS state are not written by this kernel (constant address space:
+    // scalar loads that stay valid across the kernel's own stores)
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[r];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -897,7 +906,7 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
                         if (!d.uniform_degree) {
                             const uint32_t degree = gp(d.site_degree)[site - 1];
                             eligible = bounded_draw(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs,
-                                                    sub, 0, kSiteEntity, site, 0, d.round_bound) < degree + 1;
+                                                    sub, 0, kSiteEntity, site, 0, const_bound(d.round_bound)) < degree + 1;
                         }
                         if (eligible) {
                             c01 += 1u;
@@ -930,7 +939,7 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
             const int pos = G::pos(ix, iy, iz);
             const uint32_t site = s_sid[pos];
             const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
-                                              kSiteEntity, site, 0, d.prop_bound);
+                                              kSiteEntity, site, 0, const_bound(d.prop_bound));
             const int q = pos + direction_offset<KF>(dir, po, s_ploff);
             if (!s_mut[q]) {
                 c23 += 1u;
@@ -1406,7 +1415,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
                 const uint32_t r = one ? r_first : a.tiles[t0 + k].rep;
                 if (w < nw && (one || kw == (int)k)) {
                     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
-                    const RepDyn &y = a.dyn[r];
+                    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[r];
                     const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
                     const Entry ent{(uint32_t)rec.x, 0u, rec.z, rec.w};
                     Staged st;
@@ -1465,7 +1474,7 @@ __device__ inline float ou_transition(float q, float disp, float strength, float
 // (_mechanical_mean_inputs, scientific_mechanics.jl:47-56): finite volume against
 // target_volume / volume_strength, or boundary measure against target_boundary /
 // boundary_strength, converted to Float32.
-__device__ inline void mech_inputs(const RepDev &d, int k, const CPM_GLOBAL CellRec *c, float *disp, float *strength) {
+__device__ inline void mech_inputs(const CPM_CONST RepDev &d, int k, const CPM_GLOBAL CellRec *c, float *disp, float *strength) {
     if (d.mech_kind[k] == CPM_MECH_PRESSURE) {
         *disp = __fsub_rn((float)c->vol, (float)c->tv);
         *strength = (float)c->vs;
@@ -1480,8 +1489,8 @@ __device__ inline void mech_inputs(const RepDev &d, int k, const CPM_GLOBAL Cell
 // scientific_mechanics.jl:170-196; lottery.jl:353-391 through _advance_mechanics!).
 __global__ __launch_bounds__(kBlock) void mech_advance_kernel(const RepDev *reps, const RepDyn *dyn, int slot,
                                                               uint32_t pass, uint32_t phase) {
-    const RepDev &d = reps[slot];
-    const RepDyn &y = dyn[slot];
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)dyn)[slot];
     if (!y.active || pass >= (uint32_t)d.rounds) return;
     const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
     if (cell > d.capacity) return;
@@ -1507,7 +1516,7 @@ __global__ __launch_bounds__(kBlock) void mech_advance_kernel(const RepDev *reps
 // _initialize_mechanics! (scientific_mechanics.jl:63-103): inactive cells hold zero;
 // constitutive mean 2 strength displacement, or the stationary draw around it.
 __global__ __launch_bounds__(kBlock) void mech_init_kernel(const RepDev *reps, int slot) {
-    const RepDev &d = reps[slot];
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
     const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
     if (cell > d.capacity) return;
     CPM_GLOBAL CellRec *c = gp(d.cells) + cell;
