@@ -1298,7 +1298,8 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &
 
 
 // Claim check, evaluation, acceptance and commit of one pass (checkerboard.jl:337-457) in
-// one wave per region of kCommitTiles tile segments. An attempt wins iff its key is the
+// one workgroup per region of CT tile segments (4, or 2 for tiles of 4096 sites and more, so
+// that a region holds a similar number of sites). An attempt wins iff its key is the
 // maximum on both of its cells. Entries are read in batches of kCheckBatch chunks whose
 // entry loads, then claim loads, are all in flight together; winners are compacted into
 // LDS and evaluated against the pass snapshot in full chunks, and commit at once (winners
@@ -1308,15 +1309,15 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &
 // otherwise one round per tile. The replica index is uniform in every round, so its
 // descriptor is read through the scalar cache.
 constexpr int kCheckBatch = 4;
-template <int NWV>
+template <int NWV, int CT>
 __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     constexpr int NT = 64 * NWV;
-    constexpr int NS = kCommitTiles * kWaves;
+    constexpr int NS = CT * kWaves;
     constexpr int WCAP = kCheckBatch * NT;
     static_assert((NS & (NS - 1)) == 0 && NS <= 64, "stretch search halves a power of two");
     __shared__ uint32_t s_base[NS + 1];
-    __shared__ uint32_t s_cbase[kCommitTiles];
-    __shared__ uint32_t s_cnt[kCommitTiles][3];  // conflicts, rejections, acceptances
+    __shared__ uint32_t s_cbase[CT];
+    __shared__ uint32_t s_cnt[CT][3];  // conflicts, rejections, acceptances
     __shared__ v4i s_win[WCAP];
     __shared__ uint32_t s_nw;
     const uint32_t nwg = gridDim.x;
@@ -1331,8 +1332,8 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
         const uint32_t hi = min(lo + per, a.total_claims);
         for (uint32_t j = lo + tid; j < hi; j += NT) next[j] = 0ull;
     }
-    const uint32_t t0 = b * kCommitTiles;
-    const uint32_t nt = min((uint32_t)kCommitTiles, a.ntiles - t0);
+    const uint32_t t0 = b * CT;
+    const uint32_t nt = min((uint32_t)CT, a.ntiles - t0);
     // stretch counts (idle tiles count zero) and their inclusive scan
     uint32_t n = 0;
     bool live = false;
@@ -1353,7 +1354,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     }
     const uint64_t live_tiles = __ballot(live && lane % kWaves == 0);  // wave 0: stretch 0 carries the flag
     if (tid < nt) s_cbase[tid] = a.tiles[t0 + tid].claim_base;
-    if (tid < (uint32_t)kCommitTiles * 3) s_cnt[tid / 3][tid % 3] = 0;
+    if (tid < (uint32_t)CT * 3) s_cnt[tid / 3][tid % 3] = 0;
     __syncthreads();
     const uint32_t total = s_base[NS];
     const uint32_t Q = a.seg_cap / kWaves;
@@ -1973,8 +1974,14 @@ static hipError_t launch_propose_t(int algorithm, int generic, const PassArgs &a
 
 constexpr int kCommitWaves = 2;  // claim check on both waves, winners evaluated in full chunks
 static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
-    const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
-    hipLaunchKernelGGL((commit_kernel<kCommitWaves>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
+    if (a.seg_cap >= 4096) {
+        const uint32_t wgs = (a.ntiles + 1) / 2;
+        hipLaunchKernelGGL((commit_kernel<kCommitWaves, 2>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
+    } else {
+        const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
+        hipLaunchKernelGGL((commit_kernel<kCommitWaves, kCommitTiles>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream,
+                           a);
+    }
     return hipGetLastError();
 }
 
