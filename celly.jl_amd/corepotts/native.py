@@ -400,7 +400,8 @@ class ScientificPottsIntegrator:
         return count.value, order[:count.value]
 
     def kernel_config(self) -> int:
-        """Tile shape id | proposal-count kernels << 4 | row propose kernel << 5."""
+        """Tile shape id | proposal-count kernels << 4 | row propose kernel << 5 | plain lottery kernel << 6;
+        256 for TiledCheckerboardCPM."""
         v = C.c_int32()
         self._check(self.lib.cpm_kernel_config(self._ctx, C.byref(v)))
         return int(v.value)

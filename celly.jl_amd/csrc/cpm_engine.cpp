@@ -285,8 +285,8 @@ int conflict_neighbours(const HostGeom &g, const std::vector<int32_t> &offs, int
     return count;
 }
 
-// Group configuration of TiledCheckerboardCPM replicas (outside the tile-shape ids 0..63).
-constexpr int kTiledConfig = 64;
+// Group configuration of TiledCheckerboardCPM replicas (outside the tile configurations 0..127).
+constexpr int kTiledConfig = 256;
 // Touched-cell entries the tiled kernels keep in LDS per tile (as cpm::kListLds).
 constexpr uint32_t kTiledListLds = 8;
 
@@ -886,6 +886,9 @@ void compile_replica(Replica &r) {
         for (int c = 1; ok && c <= 4; ++c) ok = v.col_count[c] == 1;
         if (ok) r.config |= 32;
     }
+    // lottery on a periodic obstacle-free lattice with at least 2 halo + 1 sites per axis:
+    // winners follow from the tickets alone
+    if (d.algorithm == CPM_ALG_LOTTERY && all_periodic && wide) r.config |= 64;
     v.capacity = d.capacity;
     v.tiled_interval = r.tiled_interval;
     v.tiled_batches = r.tiled_batches;

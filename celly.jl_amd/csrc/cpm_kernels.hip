@@ -792,7 +792,9 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
 // ---------------------------------------------------------------------------
 // Lottery round, propose + claim (lottery.jl:153-256)
 // ---------------------------------------------------------------------------
-template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT>
+// PLAIN: periodic, obstacle-free and at least 2 halo + 1 sites along every axis, so every
+// conflict neighbour position of a tile site is a mutable site other than itself.
+template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT, bool PLAIN>
 __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     using G = Geom<DIM, H, TX, TY, TZ>;
     constexpr int NW = NT / 64;
@@ -877,7 +879,7 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
 #pragma unroll 8
                     for (int k = 0; k < KC; ++k) {
                         const int q = pos + s_cloff[k];
-                        const bool other_site = s_mut[q] && s_sid[q] != site;
+                        const bool other_site = PLAIN || (s_mut[q] && s_sid[q] != site);
                         const uint32_t other = s_tk[q];
                         beaten |= other_site && other > mine;
                         tied |= other_site && other == mine;
@@ -1950,16 +1952,23 @@ __global__ __launch_bounds__(kBlock) void report_kernel(const RepDev *reps, cons
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-// generic == 0: the proposal relation has 2 * DIM directions (offsets in registers)
+// generic == 0: the proposal relation has 2 * DIM directions (offsets in registers);
+// plain: lottery winners are decided from the tickets alone (see lot_propose_kernel)
 template <int DIM, int H, int TX, int TY, int TZ, int NT = kBlock>
-static hipError_t launch_propose_t(int algorithm, int generic, const PassArgs &a, hipStream_t stream) {
+static hipError_t launch_propose_t(int algorithm, int generic, int plain, const PassArgs &a, hipStream_t stream) {
     constexpr int KF = 2 * DIM;
     if (algorithm == CPM_ALG_LOTTERY) {
-        if (generic)
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT>), dim3(a.ntiles), dim3(NT), 0,
+        if (generic && plain)
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, true>), dim3(a.ntiles), dim3(NT), 0,
+                               stream, a);
+        else if (generic)
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, false>), dim3(a.ntiles), dim3(NT), 0,
+                               stream, a);
+        else if (plain)
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, true>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT>), dim3(a.ntiles), dim3(NT), 0,
+            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, false>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
     } else {
         if (generic)
@@ -1989,9 +1998,10 @@ static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
 
 // Tile configurations: id & 15 -> (dim, halo, TX, TY, TZ); bit 4 of id selects the kernels
 // for proposal relations whose direction count differs from 2 * dim; bit 5 the row kernel
-// of 2D periodic von Neumann checkerboards with a 2 x 2 colour pattern.
+// of 2D periodic von Neumann checkerboards with a 2 x 2 colour pattern; bit 6 the lottery
+// kernel for periodic obstacle-free lattices of at least 2 halo + 1 sites per axis.
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz) {
-    if (id < 0 || id > 63) return 0;
+    if (id < 0 || id > 127) return 0;
     id &= 15;
     static const int table[][5] = {
         {2, 1, 128, 16, 1}, {2, 2, 128, 16, 1}, {3, 1, 32, 8, 8}, {3, 2, 32, 8, 8},
@@ -2015,7 +2025,7 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream) {
     using namespace cpm;
-    if (config < 0 || config > 63) return hipErrorInvalidValue;
+    if (config < 0 || config > 127) return hipErrorInvalidValue;
     if (config & 32) {
         // row kernels: checkerboard, 2D periodic von Neumann proposal, 2 x 2 colour pattern
         if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16)) return hipErrorInvalidValue;
@@ -2033,21 +2043,23 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
         return hipGetLastError();
     }
     const int g = (config >> 4) & 1;
+    const int plain = (config >> 6) & 1;
+    if (plain && algorithm != CPM_ALG_LOTTERY) return hipErrorInvalidValue;
     switch (config & 15) {
-    case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, g, *a, stream);
-    case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, g, *a, stream);
-    case 2: return launch_propose_t<3, 1, 32, 8, 8>(algorithm, g, *a, stream);
-    case 3: return launch_propose_t<3, 2, 32, 8, 8>(algorithm, g, *a, stream);
-    case 4: return launch_propose_t<2, 1, 64, 32, 1>(algorithm, g, *a, stream);
-    case 5: return launch_propose_t<2, 1, 256, 8, 1>(algorithm, g, *a, stream);
-    case 6: return launch_propose_t<2, 1, 64, 16, 1>(algorithm, g, *a, stream);
-    case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, g, *a, stream);
-    case 8: return launch_propose_t<3, 1, 64, 8, 4>(algorithm, g, *a, stream);
-    case 9: return launch_propose_t<3, 1, 32, 16, 4>(algorithm, g, *a, stream);
-    case 10: return launch_propose_t<2, 1, 32, 16, 1, 64>(algorithm, g, *a, stream);
-    case 11: return launch_propose_t<2, 1, 64, 8, 1, 64>(algorithm, g, *a, stream);
-    case 12: return launch_propose_t<2, 1, 256, 16, 1>(algorithm, g, *a, stream);
-    case 13: return launch_propose_t<2, 1, 128, 32, 1>(algorithm, g, *a, stream);
+    case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, g, plain, *a, stream);
+    case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, g, plain, *a, stream);
+    case 2: return launch_propose_t<3, 1, 32, 8, 8>(algorithm, g, plain, *a, stream);
+    case 3: return launch_propose_t<3, 2, 32, 8, 8>(algorithm, g, plain, *a, stream);
+    case 4: return launch_propose_t<2, 1, 64, 32, 1>(algorithm, g, plain, *a, stream);
+    case 5: return launch_propose_t<2, 1, 256, 8, 1>(algorithm, g, plain, *a, stream);
+    case 6: return launch_propose_t<2, 1, 64, 16, 1>(algorithm, g, plain, *a, stream);
+    case 7: return launch_propose_t<2, 1, 32, 32, 1>(algorithm, g, plain, *a, stream);
+    case 8: return launch_propose_t<3, 1, 64, 8, 4>(algorithm, g, plain, *a, stream);
+    case 9: return launch_propose_t<3, 1, 32, 16, 4>(algorithm, g, plain, *a, stream);
+    case 10: return launch_propose_t<2, 1, 32, 16, 1, 64>(algorithm, g, plain, *a, stream);
+    case 11: return launch_propose_t<2, 1, 64, 8, 1, 64>(algorithm, g, plain, *a, stream);
+    case 12: return launch_propose_t<2, 1, 256, 16, 1>(algorithm, g, plain, *a, stream);
+    case 13: return launch_propose_t<2, 1, 128, 32, 1>(algorithm, g, plain, *a, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -115,6 +115,8 @@ def test_config1_plain(gpu_lib, alg):
     gpu, sim, reps = run_parity(compiled, proposal, components, algorithm, seed, [1, 10, 25, 100])
     for rep in reps:
         _partition(rep)
+    # periodic and obstacle-free: the lottery kernel that decides winners from tickets alone
+    assert bool(gpu.kernel_config() & 64) == (alg == "lottery")
 
 
 @pytest.mark.parametrize("alg", ["checkerboard", "lottery"])
@@ -171,7 +173,8 @@ def _walled_domain(alg_type, obstacles=True):
 @pytest.mark.parametrize("obstacles", [False, True])
 def test_closed_fixed_obstacle_domain(gpu_lib, alg_type, obstacles):
     compiled, proposal, components, alg = _walled_domain(alg_type, obstacles)
-    run_parity(compiled, proposal, components, alg, 31337, [1, 5, 15])
+    gpu, _, _ = run_parity(compiled, proposal, components, alg, 31337, [1, 5, 15])
+    assert not gpu.kernel_config() & 64  # walls: the lottery kernel that checks every neighbour
 
 
 def test_replay_and_metrics(gpu_lib):
