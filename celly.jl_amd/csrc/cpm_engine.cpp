@@ -886,6 +886,19 @@ void compile_replica(Replica &r) {
         for (int c = 1; ok && c <= 4; ++c) ok = v.col_count[c] == 1;
         if (ok) r.config |= 32;
     }
+    // 3D row kernel: periodic and obstacle-free, canonical von Neumann proposal (six
+    // directions), one residue class of a 2 x 2 x 2 pattern per colour, whole tiles
+    if (g_row_kernels.load() && d.algorithm == CPM_ALG_CHECKERBOARD && simple && d.ndim == 3 && v.pat_fast &&
+        (r.config & 16) == 0 && v.pat[0] == 2 && v.pat[1] == 2 && v.pat[2] == 2 && r.rounds == 8 &&
+        r.desc.dims[0] % tx == 0 && r.desc.dims[1] % ty == 0 && r.desc.dims[2] % tz == 0 &&
+        (shape == 2 || shape == 8 || shape == 9)) {
+        static const int vn3[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
+        bool ok = d.proposal.count == 6;
+        for (int k = 0; ok && k < 6; ++k)
+            for (int a = 0; a < 3; ++a) ok = ok && d.proposal.offsets[k][a] == vn3[k][a];
+        for (int c = 1; ok && c <= 8; ++c) ok = v.col_count[c] == 1;
+        if (ok) r.config |= 32;
+    }
     // lottery on a periodic obstacle-free lattice with at least 2 halo + 1 sites per axis:
     // winners follow from the tickets alone
     if (d.algorithm == CPM_ALG_LOTTERY && all_periodic && wide) r.config |= 64;

@@ -790,6 +790,216 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// 3D row propose kernel: 3D periodic obstacle-free checkerboards whose proposal relation is
+// the canonical von Neumann shell (6 directions) and whose colouring is a 2 x 2 x 2 pattern
+// with one residue class per colour (checkerboard.jl:244-310). A colour's sites lie in every
+// other row of every other plane; a wave reads its colour rows and their four neighbour rows
+// (y - 1, y + 1, z - 1, z + 1) straight into registers.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, int TZ>
+struct RowGeom3 {
+    static constexpr int LPR = TX / 4;                      // lanes per lattice row
+    static constexpr int RPI = 64 / LPR;                    // rows per wave instruction
+    static constexpr int CRY = TY / 2, CRZ = TZ / 2;        // colour rows along y and z
+    static constexpr int CROWS = CRY * CRZ / kWaves;        // colour rows of one wave
+    static constexpr int ITERS = (CROWS + RPI - 1) / RPI;   // row strips of one wave
+    static constexpr int CAP = 2 * 64;                      // colour sites of one strip
+    static constexpr int Q = TX * TY * TZ / kWaves;         // segment stretch capacity
+    static_assert(TX % 4 == 0 && 64 % LPR == 0 && TY % 2 == 0 && TZ % 2 == 0 && (CRY * CRZ) % kWaves == 0,
+                  "3D row tile shape");
+    static_assert(CAP * ITERS <= Q, "stretch holds every colour site of the wave");
+};
+
+// One row strip of a wave: 16 bytes of the colour row and of its four neighbour rows, and
+// the x neighbour outside the 16 bytes (x - 1 for RX == 0, x + 4 else).
+struct RowStrip3 {
+    v4i mid, dn, up, bk, fr;
+    int32_t side;
+    uint32_t site0;  // 1-based site of the first of the 16 bytes
+    bool valid;
+};
+
+template <int RX, int TX, int TY, int TZ>
+__device__ __forceinline__ RowStrip3 row_load3(const CPM_GLOBAL int32_t *lat, int X, int Y, int Z, const TileInfo &ti,
+                                               int ry, int rz, int wave, int it) {
+    using R = RowGeom3<TX, TY, TZ>;
+    const int lane = (int)__lane_id();
+    const int k = it * R::RPI + lane / R::LPR;
+    RowStrip3 st;
+    st.valid = k < R::CROWS;
+    const int j = wave * R::CROWS + (st.valid ? k : 0);  // colour row of the tile
+    const int gx = ti.x0 + 4 * (lane % R::LPR);
+    const int gy = ti.y0 + 2 * (j % R::CRY) + ry;
+    const int gz = ti.z0 + 2 * (j / R::CRY) + rz;
+    const uint32_t XY = (uint32_t)X * (uint32_t)Y;
+    st.mid = st.dn = st.up = st.bk = st.fr = v4i{0, 0, 0, 0};
+    st.side = 0;
+    st.site0 = 1u + (uint32_t)gx + (uint32_t)gy * (uint32_t)X + (uint32_t)gz * XY;
+    if (st.valid) {
+        const int gym = gy == 0 ? Y - 1 : gy - 1, gyp = gy + 1 == Y ? 0 : gy + 1;
+        const int gzm = gz == 0 ? Z - 1 : gz - 1, gzp = gz + 1 == Z ? 0 : gz + 1;
+        const uint32_t plane = (uint32_t)gz * XY, row = plane + (uint32_t)gy * (uint32_t)X;
+        st.mid = *(const CPM_GLOBAL v4i *)(lat + row + (uint32_t)gx);
+        st.dn = *(const CPM_GLOBAL v4i *)(lat + plane + (uint32_t)gym * (uint32_t)X + (uint32_t)gx);
+        st.up = *(const CPM_GLOBAL v4i *)(lat + plane + (uint32_t)gyp * (uint32_t)X + (uint32_t)gx);
+        st.bk = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gzm * XY + (uint32_t)gy * (uint32_t)X + (uint32_t)gx);
+        st.fr = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gzp * XY + (uint32_t)gy * (uint32_t)X + (uint32_t)gx);
+        const int sx = RX == 0 ? (gx == 0 ? X - 1 : gx - 1) : (gx + 4 == X ? 0 : gx + 4);
+        st.side = lat[row + (uint32_t)sx];
+    }
+    return st;
+}
+
+// Colour sites of one strip: interior sites (all six neighbours share the owner) are
+// same-owner no-ops; boundary sites are appended with their neighbours in the canonical
+// direction order (-x, -y, -z, +z, +y, +x).
+template <int RX>
+__device__ __forceinline__ void row_collect3(const RowStrip3 &st, uint32_t &ncand, uint32_t &nsites, uint32_t &nsame,
+                                             uint32_t *s_site, int32_t *s_own, int32_t (*s_nb)[2 * 64]) {
+    nsites += 2u * (uint32_t)__popcll(__ballot(st.valid));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = RX + 2 * h;
+        const v4i mid = st.mid, dn = st.dn, up = st.up, bk = st.bk, fr = st.fr;
+        const int32_t own = p == 0 ? mid.x : p == 1 ? mid.y : p == 2 ? mid.z : mid.w;
+        const int32_t left = p == 0 ? st.side : p == 1 ? mid.x : p == 2 ? mid.y : mid.z;
+        const int32_t right = p == 0 ? mid.y : p == 1 ? mid.z : p == 2 ? mid.w : st.side;
+        const int32_t below = p == 0 ? dn.x : p == 1 ? dn.y : p == 2 ? dn.z : dn.w;
+        const int32_t above = p == 0 ? up.x : p == 1 ? up.y : p == 2 ? up.z : up.w;
+        const int32_t back = p == 0 ? bk.x : p == 1 ? bk.y : p == 2 ? bk.z : bk.w;
+        const int32_t front = p == 0 ? fr.x : p == 1 ? fr.y : p == 2 ? fr.z : fr.w;
+        const bool interior = own == left && own == below && own == above && own == right && own == back &&
+                              own == front;
+        const bool boundary = st.valid && !interior;
+        nsame += (uint32_t)__popcll(__ballot(st.valid && interior));
+        const uint64_t m = __ballot(boundary);
+        if (boundary) {
+            const uint32_t e = ncand + __popcll(m & __lanemask_lt());
+            s_site[e] = st.site0 + (uint32_t)p;
+            s_own[e] = own;
+            s_nb[0][e] = left;
+            s_nb[1][e] = below;
+            s_nb[2][e] = back;
+            s_nb[3][e] = front;
+            s_nb[4][e] = above;
+            s_nb[5][e] = right;
+        }
+        ncand += (uint32_t)__popcll(m);
+    }
+}
+
+template <int RX, int TX, int TY, int TZ>
+__device__ __forceinline__ void row_wave3(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti,
+                                          uint32_t tile, uint64_t mcs, int ry, int rz, int wave, uint32_t *s_site,
+                                          int32_t *s_own, int32_t (*s_nb)[2 * 64], uint32_t &nsites, uint32_t &nsame,
+                                          uint32_t &nact) {
+    using R = RowGeom3<TX, TY, TZ>;
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    const int lane = (int)__lane_id();
+    RowStrip3 st[R::ITERS];
+#pragma unroll
+    for (int it = 0; it < R::ITERS; ++it)
+        st[it] = row_load3<RX, TX, TY, TZ>(lat, d.dims[0], d.dims[1], d.dims[2], ti, ry, rz, wave, it);
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
+    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
+    const Bound dirs = const_bound(d.prop_bound);
+#pragma unroll
+    for (int it = 0; it < R::ITERS; ++it) {
+        uint32_t ncand = 0;
+        row_collect3<RX>(st[it], ncand, nsites, nsame, s_site, s_own, s_nb);
+        // direction draws (bounded_uint(6) may retry); actionable attempts compacted in place
+        uint32_t na = 0;
+        for (uint32_t e0 = 0; e0 < ncand; e0 += 64) {
+            const uint32_t e = e0 + (uint32_t)lane;
+            bool act = false;
+            uint32_t site = 0;
+            int32_t own = 0, gain = 0;
+            if (e < ncand) {
+                site = s_site[e];
+                own = s_own[e];
+                const uint32_t dir =
+                    bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity, site, 0, dirs);
+                gain = s_nb[dir][e];
+                act = gain != own;
+            }
+            const uint64_t valid = __ballot(e < ncand);
+            const uint64_t m = __ballot(act);
+            nsame += (uint32_t)__popcll(valid & ~m);
+            if (act) {
+                const uint32_t at = na + __popcll(m & __lanemask_lt());
+                s_site[at] = site;
+                s_own[at] = own;
+                s_nb[0][at] = gain;
+            }
+            na += (uint32_t)__popcll(m);
+        }
+        // priority draws and claims (CheckerboardPriorityStream); stretch `wave` of the segment
+        for (uint32_t e = (uint32_t)lane; e < na; e += 64) {
+            const uint32_t site = s_site[e];
+            const int32_t losing = s_own[e];
+            const int32_t gaining = s_nb[0][e];
+            const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
+                                            kSiteEntity, site, 0, 0).w0;
+            const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
+            if (losing > 0) atomicMax(&claims[(uint32_t)losing], key);
+            if (gaining > 0) atomicMax(&claims[(uint32_t)gaining], key);
+            seg[nact + e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
+        }
+        nact += na;
+    }
+}
+
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(kBlock) void cb_propose_rows3_kernel(PassArgs a) {
+    __shared__ uint32_t s_site[kWaves][2 * 64];
+    __shared__ int32_t s_own[kWaves][2 * 64];
+    __shared__ int32_t s_nb[kWaves][6][2 * 64];
+    __shared__ uint32_t s_cnt[kWaves][3];
+
+    const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
+    const TileInfo ti = a.tiles[tile];
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[ti.rep];
+    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[ti.rep];
+    if (!y.active || a.pass >= (uint32_t)d.rounds) {
+        if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
+        return;
+    }
+    const uint64_t mcs = y.mcs;
+    const uint32_t colour = y.order[a.pass];
+    const int res = d.col_res[d.col_first[colour]];
+    const int rx = d.pat_coord[res][0], ry = d.pat_coord[res][1], rz = d.pat_coord[res][2];
+    const int wave = (int)(threadIdx.x >> 6);
+    uint32_t nsites = 0, nsame = 0, nact = 0;
+    if (rx == 0)
+        row_wave3<0, TX, TY, TZ>(a, d, ti, tile, mcs, ry, rz, wave, s_site[wave], s_own[wave], s_nb[wave], nsites,
+                                 nsame, nact);
+    else
+        row_wave3<1, TX, TY, TZ>(a, d, ti, tile, mcs, ry, rz, wave, s_site[wave], s_own[wave], s_nb[wave], nsites,
+                                 nsame, nact);
+    if (__lane_id() == 0) {
+        a.seg_count[tile * kWaves + wave] = nact;
+        s_cnt[wave][0] = nsites;
+        s_cnt[wave][1] = nsame;
+        s_cnt[wave][2] = nact;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
+        uint32_t t0 = 0, t1 = 0, t3 = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            t0 += s_cnt[w][0];
+            t1 += s_cnt[w][1];
+            t3 += s_cnt[w][2];
+        }
+        slot[0] = t0;   // candidates
+        slot[1] = t1;   // same-owner no-ops
+        slot[2] = 0;    // boundary no-ops: none on a periodic obstacle-free lattice
+        slot[3] = t3;   // realized (actionable) attempts
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Lottery round, propose + claim (lottery.jl:153-256)
 // ---------------------------------------------------------------------------
 // PLAIN: periodic, obstacle-free and at least 2 halo + 1 sites along every axis, so every
@@ -2027,7 +2237,7 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
     using namespace cpm;
     if (config < 0 || config > 127) return hipErrorInvalidValue;
     if (config & 32) {
-        // row kernels: checkerboard, 2D periodic von Neumann proposal, 2 x 2 colour pattern
+        // row kernels: checkerboard, periodic von Neumann proposal, 2 x 2 (x 2) colour pattern
         if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16)) return hipErrorInvalidValue;
         switch (config & 15) {
         case 0:
@@ -2038,6 +2248,10 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
         case 7: hipLaunchKernelGGL((cb_propose_rows_kernel<32, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         case 12: hipLaunchKernelGGL((cb_propose_rows_kernel<256, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         case 13: hipLaunchKernelGGL((cb_propose_rows_kernel<128, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        // 3D: 2 x 2 x 2 colour pattern, six von Neumann directions
+        case 2: hipLaunchKernelGGL((cb_propose_rows3_kernel<32, 8, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 8: hipLaunchKernelGGL((cb_propose_rows3_kernel<64, 8, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 9: hipLaunchKernelGGL((cb_propose_rows3_kernel<32, 16, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

@@ -99,3 +99,22 @@ def test_3d_shapes(gpu_lib, shape, alg):
         _parity(workloads.block_cells(21, 3, 5, 7, algorithm=alg, seed=77), 3)
     finally:
         set_tile_shape(alg_id, 3, 1, -1)
+
+
+ROW3_SHAPES = [s for s, t in tile_shapes(3, 1) if s in (2, 8, 9)]
+
+
+@pytest.mark.parametrize("shape", ROW3_SHAPES)
+@pytest.mark.parametrize("rows", [True, False], ids=["rows", "tiles"])
+def test_3d_row_kernels(gpu_lib, shape, rows):
+    """Periodic cubes whose sides are whole tiles of every 3D shape: the 3D row propose kernel
+    (von Neumann proposal, 26-neighbour contact: a 2 x 2 x 2 colour pattern, eight colours)
+    against the oracle, and the tile kernel on the same models."""
+    set_tile_shape(ALG_CHECKERBOARD, 3, 1, shape)
+    set_row_kernels(rows)
+    try:
+        _parity(workloads.cubes_3d(64, side=8, seed=515), 3, rows)
+        _parity(workloads.block_cells(64, 3, 6, 8, seed=516), 3, rows)
+    finally:
+        set_tile_shape(ALG_CHECKERBOARD, 3, 1, -1)
+        set_row_kernels(True)
