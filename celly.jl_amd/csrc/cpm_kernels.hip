@@ -1052,19 +1052,50 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     int32_t po[KF > 0 ? KF : 1];
 #pragma unroll
     for (int k = 0; k < (KF > 0 ? KF : 1); ++k) po[k] = d.prop_lds[k];
-    load_tile<DIM, H, TX, TY, TZ, NT>(d, x0, y0, z0, s_own, s_mut, s_sid);
+    // site id of an LDS position of a plain lattice (every position a mutable site; the
+    // mutability and identity arrays are not staged)
+    auto sid = [&](int pos) __attribute__((always_inline)) {
+        const int lx = pos % G::LX, ly = (pos / G::LX) % G::LY, lz = pos / (G::LX * G::LY);
+        int gx = x0 - G::PX + lx, gy = y0 - H + ly, gz = z0 - G::HZ + lz;
+        gx += gx < 0 ? X : 0;
+        gx -= gx >= X ? X : 0;
+        gy += gy < 0 ? Y : 0;
+        gy -= gy >= Y ? Y : 0;
+        if (DIM == 3) {
+            gz += gz < 0 ? Z : 0;
+            gz -= gz >= Z ? Z : 0;
+        }
+        return 1u + (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
+    };
+    load_tile<DIM, H, TX, TY, TZ, NT>(d, x0, y0, z0, s_own, s_mut, PLAIN ? nullptr : s_sid);
     __syncthreads();
     // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1);
     // two independent draws per iteration keep two Philox chains in flight
-    for (int i = threadIdx.x; i < G::LN; i += 2 * NT) {
-        const int i2 = i + NT;
-        const bool m1 = s_mut[i] != 0, m2 = i2 < G::LN && s_mut[i2] != 0;
-        const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
-                                      s_sid[i], 0, 0).w0;
-        const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
-                                      i2 < G::LN ? s_sid[i2] : 0u, 0, 0).w0;
-        s_tk[i] = m1 ? t1 : 0u;
-        if (i2 < G::LN) s_tk[i2] = m2 ? t2 : 0u;
+    if constexpr (PLAIN) {
+        // the tile and its halo only (the x pads beyond the halo are never read)
+        constexpr int WX = TX + 2 * H, WN = WX * G::LY * G::LZ;
+        for (int k = threadIdx.x; k < WN; k += 2 * NT) {
+            const int k2 = k + NT;
+            const int i = (G::PX - H) + k % WX + G::LX * (k / WX);
+            const int i2 = (G::PX - H) + k2 % WX + G::LX * (k2 / WX);
+            const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                                          sid(i), 0, 0).w0;
+            const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                                          k2 < WN ? sid(i2) : 0u, 0, 0).w0;
+            s_tk[i] = t1;
+            if (k2 < WN) s_tk[i2] = t2;
+        }
+    } else {
+        for (int i = threadIdx.x; i < G::LN; i += 2 * NT) {
+            const int i2 = i + NT;
+            const bool m1 = s_mut[i] != 0, m2 = i2 < G::LN && s_mut[i2] != 0;
+            const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                                          s_sid[i], 0, 0).w0;
+            const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
+                                          i2 < G::LN ? s_sid[i2] : 0u, 0, 0).w0;
+            s_tk[i] = m1 ? t1 : 0u;
+            if (i2 < G::LN) s_tk[i2] = m2 ? t2 : 0u;
+        }
     }
     __syncthreads();
 
@@ -1079,8 +1110,8 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
             const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
             if (gx < X && gy < Y && gz < Z) {
                 const int pos = G::pos(ix, iy, iz);
-                if (s_mut[pos]) {
-                    const uint32_t site = s_sid[pos];
+                if (PLAIN || s_mut[pos]) {
+                    const uint32_t site = PLAIN ? sid(pos) : s_sid[pos];
                     const uint32_t mine = s_tk[pos];
                     // strict lexicographic ticket maximum over the conflict neighbours
                     // (lottery.jl:153-185): word 1 decides unless it ties, which needs the
@@ -1100,8 +1131,8 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
                                                    kSiteEntity, site, 0, 0);
                         for (int k = 0; k < KC && wins; ++k) {
                             const int q = pos + s_cloff[k];
-                            if (!s_mut[q]) continue;
-                            const uint32_t nsite = s_sid[q];
+                            if (!PLAIN && !s_mut[q]) continue;
+                            const uint32_t nsite = PLAIN ? sid(q) : s_sid[q];
                             if (nsite == site || s_tk[q] != mine) continue;
                             const Words a4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
                                                        mcs, sub, 0, kSiteEntity, nsite, 0, 0);
@@ -1123,7 +1154,7 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
                         if (eligible) {
                             c01 += 1u;
                             const int32_t own = s_own[pos];
-                            if (interior_site<KF, false>(s_own, s_mut, pos, own, po, K, s_ploff)) c01 += 1u << 16;
+                            if (interior_site<KF, PLAIN>(s_own, s_mut, pos, own, po, K, s_ploff)) c01 += 1u << 16;
                             else need = true;
                         }
                     }
@@ -1149,11 +1180,11 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
             const int i = s_work[list_slot<Q>(e, p1, p2, p3)];
             const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
             const int pos = G::pos(ix, iy, iz);
-            const uint32_t site = s_sid[pos];
+            const uint32_t site = PLAIN ? sid(pos) : s_sid[pos];
             const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
                                               kSiteEntity, site, 0, const_bound(d.prop_bound));
             const int q = pos + direction_offset<KF>(dir, po, s_ploff);
-            if (!s_mut[q]) {
+            if (!PLAIN && !s_mut[q]) {
                 c23 += 1u;
             } else {
                 const int32_t own = s_own[pos], gain = s_own[q];
