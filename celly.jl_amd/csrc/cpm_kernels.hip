@@ -1011,8 +1011,10 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     constexpr int Q = G::TN / NW;
     static_assert(G::TN <= 4096, "packed 16-bit tile counters");
     __shared__ __align__(16) int32_t s_own[G::LN];
-    __shared__ uint8_t s_mut[G::LN];
-    __shared__ uint32_t s_sid[G::LN];
+    // plain lattices stage neither mutability nor identity (smaller footprint, more
+    // workgroups per CU)
+    __shared__ uint8_t s_mut[PLAIN ? 4 : G::LN];
+    __shared__ uint32_t s_sid[PLAIN ? 1 : G::LN];
     __shared__ uint32_t s_tk[G::LN];
     __shared__ uint16_t s_work[G::TN];
     __shared__ int32_t s_ploff[KF > 0 ? KF : CPM_MAX_DIRS];
