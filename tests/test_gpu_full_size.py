@@ -1,7 +1,8 @@
 """Parity at the full sizes of the benchmark configurations: the HIP engine and the oracle
 start from the same 4096 x 4096 (163,840 cells) and 256^3 (32,768 cubes, Float32 contact
 matrix) states and must agree bit for bit on lattice, trackers and report after the
-compared MCS; the device state must also pass the tracker recount."""
+compared MCS (checkerboard, lottery and tiled at 4096^2); the device state must also pass
+the tracker recount."""
 import numpy as np
 import pytest
 
@@ -38,3 +39,8 @@ def test_config2_lottery_4096(gpu_lib):
 
 def test_config3_cubes_256(gpu_lib):
     _full(workloads.cubes_3d(256), 1)
+
+
+def test_config2_tiled_4096(gpu_lib):
+    # TiledCheckerboardCPM with the reference's default 4 x 4 tiles on the config-2 state
+    _full(workloads.config2_large(algorithm="tiled"), 2)
