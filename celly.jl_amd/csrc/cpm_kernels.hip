@@ -1913,50 +1913,64 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_batch_kernel(const RepDev *
     const uint32_t k = t0 + blockIdx.x * kTiledBlock + threadIdx.x;
     const uint32_t interval = (uint32_t)d.tiled_interval;
     uint32_t v[5] = {0, 0, 0, 0, 0};  // realized, same owner, boundary null, rejected, accepted
-    if (k < t1) {
-        const uint32_t tile = d.color_tiles[k];
-        const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
-        const uint32_t start = batch * interval + 1;
-        const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
-        TileList list{s_touched + threadIdx.x, s_touched + kListLds * kTiledBlock + threadIdx.x,
-                      gp(d.tile_spill) + (size_t)(k - t0) * (uint32_t)d.tile_spill_stride, 0};
-        CPM_GLOBAL int32_t *lat = gp(d.lattice);
-        const uint32_t stop = min(start + interval - 1, count);
-        for (uint32_t lp = start; lp <= stop; ++lp) {
+    const bool have = k < t1;
+    const uint32_t tile = have ? d.color_tiles[k] : 1u;
+    const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
+    const uint32_t start = batch * interval + 1;
+    const uint32_t stop = have ? min(start + interval - 1, count) : 0u;
+    const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
+    TileList list{s_touched + threadIdx.x, s_touched + kListLds * kTiledBlock + threadIdx.x,
+                  gp(d.tile_spill) + (size_t)(have ? k - t0 : 0u) * (uint32_t)d.tile_spill_stride, 0};
+    CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    const uint32_t X = (uint32_t)d.dims[0], XY = X * (uint32_t)d.dims[1];
+    // Each lane advances through its proposals to the next realized one (no-ops are counted on
+    // the way), then the wave evaluates all lanes' realized proposals together: evaluation
+    // passes follow the most realized proposals of any lane, not the proposal count. A lane's
+    // proposals keep their order, each seeing the lane's earlier copies.
+    uint32_t lp = start, rec = 0;
+    int32_t l = 0, g = 0;
+    bool pending = false;
+    while (true) {
+        while (!pending && lp <= stop) {
             const uint32_t sel = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
                                               tile, 0, count);
-            const uint32_t rec = d.tile_sites[first + sel];
+            rec = d.tile_sites[first + sel];
             const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
                                               tile, 1, (uint32_t)d.prop.n);
             const uint32_t idx0 = rec - 1;
-            const uint32_t X = (uint32_t)d.dims[0], XY = X * (uint32_t)d.dims[1];
             const int z = (int)(idx0 / XY), yy = (int)((idx0 % XY) / X), x = (int)(idx0 % X);
             const Realized donor = realize_position(d, x + d.prop.off[dir][0], yy + d.prop.off[dir][1],
                                                     z + d.prop.off[dir][2]);
-            const int32_t l = lat[idx0];
+            l = lat[idx0];
+            g = donor.word;
             if (donor.site == 0 || !donor.mut) {
                 ++v[2];
-                continue;
-            }
-            const int32_t g = donor.word;
-            if (g == l) {
+                ++lp;
+            } else if (g == l) {
                 ++v[1];
-                continue;
+                ++lp;
+            } else {
+                pending = true;
             }
+        }
+        if (__ballot(pending) == 0) break;
+        if (pending) {
             ++v[0];
             Staged st;
             const float p = metropolis_p(d.temperature, copy_energy(d, rec, l, g, list.view(l, g), &st));
             const Words w = rng_words(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity, tile, 0, 2);
-            if (!(uniform_open01(w.w0) < p)) {
+            if (uniform_open01(w.w0) < p) {
+                ++v[4];
+                lat[rec - 1] = g;
+                tiled_commit(d, list, l, g, st);
+            } else {
                 ++v[3];
-                continue;
             }
-            ++v[4];
-            lat[idx0] = g;
-            tiled_commit(d, list, l, g, st);
+            pending = false;
+            ++lp;
         }
-        list.flush(d);
     }
+    if (have) list.flush(d);
     tiled_count(d, v);
 }
 
@@ -1994,21 +2008,23 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev 
     const uint32_t interval = (uint32_t)d.tiled_interval;
     const uint32_t S = (uint32_t)(SX * SY * SZ);
     uint32_t v[5] = {0, 0, 0, 0, 0};  // realized, same owner, boundary null, rejected, accepted
-    if (k < t1) {
-        const uint32_t tile = d.color_tiles[k];
-        const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
-        const uint32_t start = batch * interval + 1;
-        const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
-        const int X = d.dims[0], Y = d.dims[1];
-        const uint32_t tc = tile - 1, gx = (uint32_t)d.tile_grid[0], gy = (uint32_t)d.tile_grid[1];
-        const int ox = (int)(tc % gx) * d.tile_edge[0], oy = (int)((tc / gx) % gy) * d.tile_edge[1],
-                  oz = (int)(tc / (gx * gy)) * d.tile_edge[2];
-        const int bx = min(d.tile_edge[0], X - ox), by = min(d.tile_edge[1], Y - oy);
-        int32_t *s_w = s_lane + threadIdx.x;
-        TileList list{s_lane + S * kTiledBlock + threadIdx.x, s_lane + (S + kListLds) * kTiledBlock + threadIdx.x,
-                      gp(d.tile_spill) + (size_t)(k - t0) * (uint32_t)d.tile_spill_stride, 0};
-        // stage the box and its halo: eight positions' loads in flight at a time
-        uint64_t donors = 0;
+    const bool have = k < t1;
+    const uint32_t tile = have ? d.color_tiles[k] : 1u;
+    const uint32_t first = d.tile_offsets[tile - 1], count = d.tile_offsets[tile] - first;
+    const uint32_t start = batch * interval + 1;
+    const uint32_t stop = have ? min(start + interval - 1, count) : 0u;
+    const uint32_t sub = ordinal * (uint32_t)d.tiled_batches + batch + 1;
+    const int X = d.dims[0], Y = d.dims[1];
+    const uint32_t tc = tile - 1, gx = (uint32_t)d.tile_grid[0], gy = (uint32_t)d.tile_grid[1];
+    const int ox = (int)(tc % gx) * d.tile_edge[0], oy = (int)((tc / gx) % gy) * d.tile_edge[1],
+              oz = (int)(tc / (gx * gy)) * d.tile_edge[2];
+    const int bx = min(d.tile_edge[0], X - ox), by = min(d.tile_edge[1], Y - oy);
+    int32_t *s_w = s_lane + threadIdx.x;
+    TileList list{s_lane + S * kTiledBlock + threadIdx.x, s_lane + (S + kListLds) * kTiledBlock + threadIdx.x,
+                  gp(d.tile_spill) + (size_t)(have ? k - t0 : 0u) * (uint32_t)d.tile_spill_stride, 0};
+    // stage the box and its halo: eight positions' loads in flight at a time
+    uint64_t donors = 0;
+    if (have) {
         for (uint32_t p0 = 0; p0 < S; p0 += 8) {
             int32_t w[8];
             bool m[8];
@@ -2032,15 +2048,23 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev 
                     if (m[j]) donors |= 1ull << (p0 + j);
                 }
         }
-        const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
-        auto delta = [&](const auto *o) __attribute__((always_inline)) { return o[0] + SX * (o[1] + SY * o[2]); };
-        CPM_GLOBAL int32_t *lat = gp(d.lattice);
-        const uint32_t stop = min(start + interval - 1, count);
-        for (uint32_t lp = start; lp <= stop; ++lp) {
+    }
+    const int ns = d.surf.n, nc = d.has_contact ? d.cont.n : 0;
+    auto delta = [&](const auto *o) __attribute__((always_inline)) { return o[0] + SX * (o[1] + SY * o[2]); };
+    CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    // Each lane advances through its proposals to the next realized one (no-ops are counted on
+    // the way), then the wave evaluates all lanes' realized proposals together: evaluation
+    // passes follow the most realized proposals of any lane, not the proposal count. A lane's
+    // proposals keep their order, each seeing the lane's earlier copies.
+    uint32_t lp = start, rec = 0;
+    int pr = 0;
+    int32_t l = 0, g = 0;
+    bool pending = false;
+    while (true) {
+        while (!pending && lp <= stop) {
             const uint32_t sel = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
                                               tile, 0, count);
             int lx, ly, lz;
-            uint32_t rec;
             if (d.mask == nullptr) {
                 // the tile's sites in order, first axis fastest
                 lx = (int)(sel % (uint32_t)bx);
@@ -2056,18 +2080,22 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev 
             }
             const uint32_t dir = bounded_uint(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity,
                                               tile, 1, (uint32_t)d.prop.n);
-            const int pr = (lx + H) + SX * ((ly + H) + SY * (lz + HZ));
+            pr = (lx + H) + SX * ((ly + H) + SY * (lz + HZ));
             const int pd = pr + s_pd[dir];
-            const int32_t l = s_w[pr * kTiledBlock];
+            l = s_w[pr * kTiledBlock];
+            g = s_w[pd * kTiledBlock];
             if (!((donors >> pd) & 1ull)) {
                 ++v[2];
-                continue;
-            }
-            const int32_t g = s_w[pd * kTiledBlock];
-            if (g == l) {
+                ++lp;
+            } else if (g == l) {
                 ++v[1];
-                continue;
+                ++lp;
+            } else {
+                pending = true;
             }
+        }
+        if (__ballot(pending) == 0) break;
+        if (pending) {
             ++v[0];
             const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
             const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
@@ -2089,17 +2117,19 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev 
                 [&](uint32_t i) __attribute__((always_inline)) { return s_jd[i]; }, &st);
             const float p = metropolis_p(d.temperature, dd);
             const Words w = rng_words(d.key_lo, d.key_hi, kStreamTiledProposal, y.mcs, sub, lp, kSiteEntity, tile, 0, 2);
-            if (!(uniform_open01(w.w0) < p)) {
+            if (uniform_open01(w.w0) < p) {
+                ++v[4];
+                lat[rec - 1] = g;
+                s_w[pr * kTiledBlock] = g;
+                tiled_commit(d, list, l, g, st);
+            } else {
                 ++v[3];
-                continue;
             }
-            ++v[4];
-            lat[rec - 1] = g;
-            s_w[pr * kTiledBlock] = g;
-            tiled_commit(d, list, l, g, st);
+            pending = false;
+            ++lp;
         }
-        list.flush(d);
     }
+    if (have) list.flush(d);
     tiled_count(d, v);
 }
 
