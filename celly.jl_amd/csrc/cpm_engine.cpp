@@ -874,10 +874,10 @@ void compile_replica(Replica &r) {
         }
     }
     // row kernel: 2D, periodic and obstacle-free, canonical von Neumann proposal, one
-    // residue class of a 2 x 2 pattern per colour, whole tiles
+    // residue class of a 2 x 2 pattern per colour, rows of whole 16-byte groups
     if (g_row_kernels.load() && d.algorithm == CPM_ALG_CHECKERBOARD && simple && d.ndim == 2 && v.pat_fast &&
         (r.config & 16) == 0 && v.pat[0] == 2 && v.pat[1] == 2 && v.pat[2] == 1 && r.rounds == 4 &&
-        r.desc.dims[0] % tx == 0 && r.desc.dims[1] % ty == 0 &&
+        r.desc.dims[0] % 4 == 0 &&
         (shape == 0 || shape == 1 || (shape >= 4 && shape <= 7) || shape == 12 || shape == 13)) {
         static const int vn[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
         bool ok = d.proposal.count == 4;
@@ -887,10 +887,11 @@ void compile_replica(Replica &r) {
         if (ok) r.config |= 32;
     }
     // 3D row kernel: periodic and obstacle-free, canonical von Neumann proposal (six
-    // directions), one residue class of a 2 x 2 x 2 pattern per colour, whole tiles
+    // directions), one residue class of a 2 x 2 x 2 pattern per colour, rows of whole 16-byte
+    // groups
     if (g_row_kernels.load() && d.algorithm == CPM_ALG_CHECKERBOARD && simple && d.ndim == 3 && v.pat_fast &&
         (r.config & 16) == 0 && v.pat[0] == 2 && v.pat[1] == 2 && v.pat[2] == 2 && r.rounds == 8 &&
-        r.desc.dims[0] % tx == 0 && r.desc.dims[1] % ty == 0 && r.desc.dims[2] % tz == 0 &&
+        r.desc.dims[0] % 4 == 0 &&
         (shape == 2 || shape == 8 || shape == 9)) {
         static const int vn3[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
         bool ok = d.proposal.count == 6;

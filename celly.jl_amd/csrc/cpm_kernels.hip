@@ -622,9 +622,10 @@ __device__ __forceinline__ RowStrip row_load(const CPM_GLOBAL int32_t *lat, int 
     const int lane = (int)__lane_id();
     const int k = it * R::RPI + lane / R::LPR;
     RowStrip st;
-    st.valid = k < R::CROWS;
     const int gx = x0 + 4 * (lane % R::LPR);
     const int gy = y0 + wave * R::BAND + 2 * k + ry;
+    // a tile at the upper x or y edge may extend past the lattice (whose X is a multiple of 4)
+    st.valid = k < R::CROWS && gx < X && gy < Y;
     st.mid = st.dn = st.up = v4i{0, 0, 0, 0};
     st.side = 0;
     st.site0 = 1u + (uint32_t)gx + (uint32_t)gy * (uint32_t)X;
@@ -826,11 +827,12 @@ __device__ __forceinline__ RowStrip3 row_load3(const CPM_GLOBAL int32_t *lat, in
     const int lane = (int)__lane_id();
     const int k = it * R::RPI + lane / R::LPR;
     RowStrip3 st;
-    st.valid = k < R::CROWS;
-    const int j = wave * R::CROWS + (st.valid ? k : 0);  // colour row of the tile
+    const int j = wave * R::CROWS + (k < R::CROWS ? k : 0);  // colour row of the tile
     const int gx = ti.x0 + 4 * (lane % R::LPR);
     const int gy = ti.y0 + 2 * (j % R::CRY) + ry;
     const int gz = ti.z0 + 2 * (j / R::CRY) + rz;
+    // a tile at an upper edge may extend past the lattice (whose X is a multiple of 4)
+    st.valid = k < R::CROWS && gx < X && gy < Y && gz < Z;
     const uint32_t XY = (uint32_t)X * (uint32_t)Y;
     st.mid = st.dn = st.up = st.bk = st.fr = v4i{0, 0, 0, 0};
     st.side = 0;

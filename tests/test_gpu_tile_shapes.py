@@ -118,3 +118,26 @@ def test_3d_row_kernels(gpu_lib, shape, rows):
     finally:
         set_tile_shape(ALG_CHECKERBOARD, 3, 1, -1)
         set_row_kernels(True)
+
+
+@pytest.mark.parametrize("shape", ROW_SHAPES)
+def test_2d_row_kernels_partial_tiles(gpu_lib, shape):
+    """Sides that are multiples of 4 but not of the tile: the tiles at the upper x and y edges
+    of the row kernel hang over the lattice and mask their missing rows and 16-byte groups."""
+    set_tile_shape(ALG_CHECKERBOARD, 2, 1, shape)
+    try:
+        _parity(workloads.scaled_2d(260, seed=9003), 4, True)
+        _parity(_rectangle(516, 90, seed=9004), 4, True)
+    finally:
+        set_tile_shape(ALG_CHECKERBOARD, 2, 1, -1)
+
+
+@pytest.mark.parametrize("shape", ROW3_SHAPES)
+def test_3d_row_kernels_partial_tiles(gpu_lib, shape):
+    """68^3 periodic cubes: every 3D row tile shape overhangs the lattice on at least one axis."""
+    set_tile_shape(ALG_CHECKERBOARD, 3, 1, shape)
+    try:
+        _parity(workloads.cubes_3d(68, side=4, seed=517), 3, True)
+        _parity(workloads.block_cells(68, 3, 6, 8, seed=518), 3, True)
+    finally:
+        set_tile_shape(ALG_CHECKERBOARD, 3, 1, -1)
