@@ -675,14 +675,36 @@ __device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand,
     }
 }
 
+// Claims of one row-kernel tile are first combined in a workgroup table (cell -> maximum key;
+// open addressing with linear probing, a full neighbourhood falls back to the global atomic),
+// then flushed with one global atomicMax per distinct cell: the maximum over a cell's keys is
+// the same, with fewer atomics at the memory side.
+constexpr int kClaimBits = 8;
+constexpr int kClaimSlots = 1 << kClaimBits;
+constexpr int kClaimProbes = 8;
+
+__device__ __forceinline__ void claim_local(uint32_t *s_ckey, unsigned long long *s_cval, unsigned long long *claims,
+                                            int32_t cell, unsigned long long key) {
+    uint32_t slot = ((uint32_t)cell * 0x9E3779B1u) >> (32 - kClaimBits);
+    for (int probe = 0; probe < kClaimProbes; ++probe) {
+        const uint32_t old = atomicCAS(&s_ckey[slot], 0u, (uint32_t)cell);
+        if (old == 0u || old == (uint32_t)cell) {
+            atomicMax(&s_cval[slot], key);
+            return;
+        }
+        slot = (slot + 1) & (kClaimSlots - 1);
+    }
+    atomicMax(&claims[(uint32_t)cell], key);
+}
+
 // The wave's part of one row-kernel tile: every strip's loads are issued first, then each
 // strip runs its direction draws (ProposalDirectionStream) over its compacted boundary
 // sites and its priority draws and claims over its compacted actionable attempts.
 template <int RX, int TX, int TY>
 __device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti, uint32_t tile,
                                          uint64_t mcs, int ry, int wave, uint32_t *s_site, int32_t *s_own,
-                                         int32_t (*s_nb)[2 * 64], uint32_t &nsites, uint32_t &nsame,
-                                         uint32_t &nact) {
+                                         int32_t (*s_nb)[2 * 64], uint32_t *s_ckey, unsigned long long *s_cval,
+                                         uint32_t &nsites, uint32_t &nsame, uint32_t &nact) {
     using R = RowGeom<TX, TY>;
     const CPM_GLOBAL int32_t *lat = gp(d.lattice);
     const int X = d.dims[0], Y = d.dims[1];
@@ -690,6 +712,7 @@ __device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepD
     RowStrip st[R::ITERS];
 #pragma unroll
     for (int it = 0; it < R::ITERS; ++it) st[it] = row_load<RX, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, it);
+    __syncthreads();  // the claim table is clear (the strips' loads are in flight)
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
     CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
 #pragma unroll
@@ -732,8 +755,8 @@ __device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepD
             const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
                                             kSiteEntity, site, 0, 0).w0;
             const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-            if (losing > 0) atomicMax(&claims[(uint32_t)losing], key);
-            if (gaining > 0) atomicMax(&claims[(uint32_t)gaining], key);
+            if (losing > 0) claim_local(s_ckey, s_cval, claims, losing, key);
+            if (gaining > 0) claim_local(s_ckey, s_cval, claims, gaining, key);
             seg[nact + e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
         }
         nact += na;
@@ -746,6 +769,8 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     __shared__ int32_t s_own[kWaves][2 * 64];
     __shared__ int32_t s_nb[kWaves][4][2 * 64];
     __shared__ uint32_t s_cnt[kWaves][3];
+    __shared__ uint32_t s_ckey[kClaimSlots];
+    __shared__ unsigned long long s_cval[kClaimSlots];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
@@ -755,6 +780,10 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
     }
+    for (int i = (int)threadIdx.x; i < kClaimSlots; i += kBlock) {
+        s_ckey[i] = 0u;
+        s_cval[i] = 0ull;
+    }
     const uint64_t mcs = y.mcs;
     const uint32_t colour = y.order[a.pass];
     const int res = d.col_res[d.col_first[colour]];
@@ -762,11 +791,11 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     const int wave = (int)(threadIdx.x >> 6);
     uint32_t nsites = 0, nsame = 0, nact = 0;
     if (rx == 0)
-        row_wave<0, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], nsites, nsame,
-                            nact);
+        row_wave<0, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], s_ckey, s_cval,
+                            nsites, nsame, nact);
     else
-        row_wave<1, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], nsites, nsame,
-                            nact);
+        row_wave<1, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], s_ckey, s_cval,
+                            nsites, nsame, nact);
     if (__lane_id() == 0) {
         a.seg_count[tile * kWaves + wave] = nact;
         s_cnt[wave][0] = nsites;
@@ -787,6 +816,11 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
         slot[1] = t1;   // same-owner no-ops
         slot[2] = 0;    // boundary no-ops: none on a periodic obstacle-free lattice
         slot[3] = t3;   // realized (actionable) attempts
+    }
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
+    for (int i = (int)threadIdx.x; i < kClaimSlots; i += kBlock) {
+        const uint32_t c = s_ckey[i];
+        if (c != 0u) atomicMax(&claims[c], s_cval[i]);
     }
 }
 
