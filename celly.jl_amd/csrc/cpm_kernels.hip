@@ -927,7 +927,8 @@ __device__ __forceinline__ void row_collect3(const RowStrip3 &st, uint32_t &ncan
 template <int RX, int TX, int TY, int TZ>
 __device__ __forceinline__ void row_wave3(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti,
                                           uint32_t tile, uint64_t mcs, int ry, int rz, int wave, uint32_t *s_site,
-                                          int32_t *s_own, int32_t (*s_nb)[2 * 64], uint32_t &nsites, uint32_t &nsame,
+                                          int32_t *s_own, int32_t (*s_nb)[2 * 64], uint32_t *s_ckey,
+                                          unsigned long long *s_cval, uint32_t &nsites, uint32_t &nsame,
                                           uint32_t &nact) {
     using R = RowGeom3<TX, TY, TZ>;
     const CPM_GLOBAL int32_t *lat = gp(d.lattice);
@@ -936,6 +937,7 @@ __device__ __forceinline__ void row_wave3(const PassArgs &a, const CPM_CONST Rep
 #pragma unroll
     for (int it = 0; it < R::ITERS; ++it)
         st[it] = row_load3<RX, TX, TY, TZ>(lat, d.dims[0], d.dims[1], d.dims[2], ti, ry, rz, wave, it);
+    __syncthreads();  // the claim table is clear (the strips' loads are in flight)
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
     CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
     const Bound dirs = const_bound(d.prop_bound);
@@ -977,8 +979,8 @@ __device__ __forceinline__ void row_wave3(const PassArgs &a, const CPM_CONST Rep
             const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
                                             kSiteEntity, site, 0, 0).w0;
             const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-            if (losing > 0) atomicMax(&claims[(uint32_t)losing], key);
-            if (gaining > 0) atomicMax(&claims[(uint32_t)gaining], key);
+            if (losing > 0) claim_local(s_ckey, s_cval, claims, losing, key);
+            if (gaining > 0) claim_local(s_ckey, s_cval, claims, gaining, key);
             seg[nact + e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
         }
         nact += na;
@@ -991,6 +993,8 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows3_kernel(PassArgs a) {
     __shared__ int32_t s_own[kWaves][2 * 64];
     __shared__ int32_t s_nb[kWaves][6][2 * 64];
     __shared__ uint32_t s_cnt[kWaves][3];
+    __shared__ uint32_t s_ckey[kClaimSlots];
+    __shared__ unsigned long long s_cval[kClaimSlots];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
@@ -1005,13 +1009,17 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows3_kernel(PassArgs a) {
     const int res = d.col_res[d.col_first[colour]];
     const int rx = d.pat_coord[res][0], ry = d.pat_coord[res][1], rz = d.pat_coord[res][2];
     const int wave = (int)(threadIdx.x >> 6);
+    for (int i = (int)threadIdx.x; i < kClaimSlots; i += kBlock) {
+        s_ckey[i] = 0u;
+        s_cval[i] = 0ull;
+    }
     uint32_t nsites = 0, nsame = 0, nact = 0;
     if (rx == 0)
-        row_wave3<0, TX, TY, TZ>(a, d, ti, tile, mcs, ry, rz, wave, s_site[wave], s_own[wave], s_nb[wave], nsites,
-                                 nsame, nact);
+        row_wave3<0, TX, TY, TZ>(a, d, ti, tile, mcs, ry, rz, wave, s_site[wave], s_own[wave], s_nb[wave], s_ckey,
+                                 s_cval, nsites, nsame, nact);
     else
-        row_wave3<1, TX, TY, TZ>(a, d, ti, tile, mcs, ry, rz, wave, s_site[wave], s_own[wave], s_nb[wave], nsites,
-                                 nsame, nact);
+        row_wave3<1, TX, TY, TZ>(a, d, ti, tile, mcs, ry, rz, wave, s_site[wave], s_own[wave], s_nb[wave], s_ckey,
+                                 s_cval, nsites, nsame, nact);
     if (__lane_id() == 0) {
         a.seg_count[tile * kWaves + wave] = nact;
         s_cnt[wave][0] = nsites;
@@ -1032,6 +1040,11 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows3_kernel(PassArgs a) {
         slot[1] = t1;   // same-owner no-ops
         slot[2] = 0;    // boundary no-ops: none on a periodic obstacle-free lattice
         slot[3] = t3;   // realized (actionable) attempts
+    }
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
+    for (int i = (int)threadIdx.x; i < kClaimSlots; i += kBlock) {
+        const uint32_t c = s_ckey[i];
+        if (c != 0u) atomicMax(&claims[c], s_cval[i]);
     }
 }
 
