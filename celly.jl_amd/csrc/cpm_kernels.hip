@@ -279,32 +279,29 @@ __device__ __forceinline__ double bnd_change_f64(double s, double t, double m, i
 // Propose side
 // ---------------------------------------------------------------------------
 
-// Publish the claim key of an actionable attempt on both cells (one 64-bit atomic max
-// per cell replaces the max-priority and min-identity claims, checkerboard.jl:281-335)
-// and append the attempt to the tile segment.
-__device__ __forceinline__ void stage_claim(bool act, const Entry &ent, const CPM_CONST RepDev &d,
-                                            unsigned long long *claims, uint32_t *s_nseg,
-                                            Entry *seg) {
-    if (act) {
-        const unsigned long long key = ((unsigned long long)ent.priority << 32) | (uint32_t)(~ent.site);
-        if (ent.losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)ent.losing], key);
-        if (ent.gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)ent.gaining], key);
+// Claims of one propose tile are first combined in a workgroup table (cell -> maximum key;
+// open addressing with linear probing, a full neighbourhood falls back to the global atomic),
+// then flushed with one global atomicMax per distinct cell: the maximum over a cell's keys is
+// the same, with fewer atomics at the memory side. One 64-bit key per cell replaces the
+// max-priority and min-identity claims (checkerboard.jl:281-335).
+constexpr int kClaimBits = 8;
+constexpr int kClaimSlots = 1 << kClaimBits;
+constexpr int kClaimProbes = 8;
+
+__device__ __forceinline__ void claim_local(uint32_t *s_ckey, unsigned long long *s_cval, unsigned long long *claims,
+                                            int32_t cell, unsigned long long key) {
+    uint32_t slot = ((uint32_t)cell * 0x9E3779B1u) >> (32 - kClaimBits);
+    for (int probe = 0; probe < kClaimProbes; ++probe) {
+        const uint32_t old = atomicCAS(&s_ckey[slot], 0u, (uint32_t)cell);
+        if (old == 0u || old == (uint32_t)cell) {
+            atomicMax(&s_cval[slot], key);
+            return;
+        }
+        slot = (slot + 1) & (kClaimSlots - 1);
     }
-    const uint64_t m = __ballot(act);
-    if (m == 0) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(s_nseg, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (act) {
-        uint4 v;
-        v.x = ent.site;
-        v.y = ent.priority;
-        v.z = (uint32_t)ent.losing;
-        v.w = (uint32_t)ent.gaining;
-        reinterpret_cast<uint4 *>(seg)[base + __popcll(m & __lanemask_lt())] = v;
-    }
+    atomicMax(&claims[(uint32_t)cell], key);
 }
+
 
 // Appends the lanes with `flag` set to a wave-private LDS list. `count` is the wave's
 // running length (uniform across the wave), so no atomics are needed.
@@ -384,6 +381,8 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
     __shared__ uint32_t s_cls[64];
     __shared__ uint32_t s_len[2][NW];
     __shared__ uint32_t s_sum[2];
+    __shared__ uint32_t s_ckey[kClaimSlots];
+    __shared__ unsigned long long s_cval[kClaimSlots];
 
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
@@ -423,6 +422,10 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
     }
 
     if (threadIdx.x < 2) s_sum[threadIdx.x] = 0;
+    for (int i = (int)threadIdx.x; i < kClaimSlots; i += NT) {
+        s_ckey[i] = 0u;
+        s_cval[i] = 0ull;
+    }
     if constexpr (KF == 0) {
         for (int k = threadIdx.x; k < K; k += NT)
             s_ploff[k] = d.prop.off[k][0] + G::LX * (d.prop.off[k][1] + G::LY * d.prop.off[k][2]);
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
     const uint32_t q1 = s_len[1][0], q2 = q1 + (NW > 1 ? s_len[1][NW > 1 ? 1 : 0] : 0u);
     const uint32_t q3 = q2 + (NW > 2 ? s_len[1][NW > 2 ? 2 : 0] : 0u);
     const uint32_t nact = q3 + (NW > 3 ? s_len[1][NW > 3 ? 3 : 0] : 0u);
-    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims;
+    unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
     CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
     for (uint32_t e = threadIdx.x; e < nact; e += NT) {
         const uint32_t at = list_slot<Q>(e, q1, q2, q3);
@@ -562,11 +565,10 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
                                         kSiteEntity, site, 0, 0).w0;
         const int32_t losing = s_own[pos];
         const int32_t gaining = s_own[pos + direction_offset<KF>(s_dir[at], po, s_ploff)];
-        // one 64-bit atomic max per cell replaces the max-priority and min-identity claims
-        // (checkerboard.jl:281-335); media are never claimed
+        // media are never claimed
         const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-        if (losing > 0) atomicMax(&claims[d.claim_base + (uint32_t)losing], key);
-        if (gaining > 0) atomicMax(&claims[d.claim_base + (uint32_t)gaining], key);
+        if (losing > 0) claim_local(s_ckey, s_cval, claims, losing, key);
+        if (gaining > 0) claim_local(s_ckey, s_cval, claims, gaining, key);
         seg[at] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
     }
 
@@ -579,6 +581,11 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
         slot[1] = lo >> 16;
         slot[2] = hi & 0xffffu;
         slot[3] = hi >> 16;
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < kClaimSlots; i += NT) {
+        const uint32_t c = s_ckey[i];
+        if (c != 0u) atomicMax(&claims[c], s_cval[i]);
     }
 }
 
@@ -673,28 +680,6 @@ __device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand,
         }
         ncand += (uint32_t)__popcll(m);
     }
-}
-
-// Claims of one row-kernel tile are first combined in a workgroup table (cell -> maximum key;
-// open addressing with linear probing, a full neighbourhood falls back to the global atomic),
-// then flushed with one global atomicMax per distinct cell: the maximum over a cell's keys is
-// the same, with fewer atomics at the memory side.
-constexpr int kClaimBits = 8;
-constexpr int kClaimSlots = 1 << kClaimBits;
-constexpr int kClaimProbes = 8;
-
-__device__ __forceinline__ void claim_local(uint32_t *s_ckey, unsigned long long *s_cval, unsigned long long *claims,
-                                            int32_t cell, unsigned long long key) {
-    uint32_t slot = ((uint32_t)cell * 0x9E3779B1u) >> (32 - kClaimBits);
-    for (int probe = 0; probe < kClaimProbes; ++probe) {
-        const uint32_t old = atomicCAS(&s_ckey[slot], 0u, (uint32_t)cell);
-        if (old == 0u || old == (uint32_t)cell) {
-            atomicMax(&s_cval[slot], key);
-            return;
-        }
-        slot = (slot + 1) & (kClaimSlots - 1);
-    }
-    atomicMax(&claims[(uint32_t)cell], key);
 }
 
 // The wave's part of one row-kernel tile: every strip's loads are issued first, then each
