@@ -284,9 +284,9 @@ __device__ __forceinline__ double bnd_change_f64(double s, double t, double m, i
 // then flushed with one global atomicMax per distinct cell: the maximum over a cell's keys is
 // the same, with fewer atomics at the memory side. One 64-bit key per cell replaces the
 // max-priority and min-identity claims (checkerboard.jl:281-335).
-constexpr int kClaimBits = 8;
+constexpr int kClaimBits = 7;
 constexpr int kClaimSlots = 1 << kClaimBits;
-constexpr int kClaimProbes = 8;
+constexpr int kClaimProbes = 2;
 
 __device__ __forceinline__ void claim_local(uint32_t *s_ckey, unsigned long long *s_cval, unsigned long long *claims,
                                             int32_t cell, unsigned long long key) {
