@@ -11,7 +11,8 @@ sides. Attempts per MCS = activated_attempts of the report (one per mutable site
 
 Roofline: the dominant kernel is the colour-pass propose kernel. Its algorithmic traffic is
 one read of the Int32 lattice per launch (4 bytes x sites); its average duration comes from
-HIP events recorded around every kernel on the engine stream (cpm_step_timed). `traffic` is
+HIP event pairs attached to every kernel's launch on the engine stream (hipExtLaunchKernelGGL
+in cpm_step_timed), which bracket the kernel's own execution. `traffic` is
 the measured HBM bytes per propose launch (FETCH_SIZE + WRITE_SIZE hardware counters, each
 scaled by what the counter reads for the library's plain copy kernel of known size), taken
 from profiles/cb_propose_4096_traffic.json when that file is present.

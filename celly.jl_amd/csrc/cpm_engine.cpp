@@ -32,6 +32,7 @@ extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, c
                                                       uint32_t interval, uint32_t staged, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
                                                           uint32_t capacity, hipStream_t stream);
+extern "C" void cpm_internal_time_next(hipEvent_t start, hipEvent_t stop);
 extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
                                                 int nslots, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
@@ -1470,17 +1471,21 @@ struct Engine {
         return a;
     }
 
-    // Optional per-kernel event timing: events[i] precede kernel i, kinds[i] its kind.
+    // Optional per-kernel timing: events[2i], events[2i + 1] bracket the execution of one
+    // kernel (attached to its launch), kinds[i] its kind (0 propose or tiled batch, 1 commit
+    // or tiled reconciliation, 2 begin).
     std::vector<hipEvent_t> *timing_events = nullptr;
     std::vector<int> *timing_kinds = nullptr;
 
     void mark(int kind) {
         if (!timing_events) return;
-        hipEvent_t ev;
-        hip_check(hipEventCreate(&ev), "hipEventCreate");
-        hip_check(hipEventRecord(ev, stream), "hipEventRecord");
-        timing_events->push_back(ev);
+        hipEvent_t ev[2];
+        for (hipEvent_t &e : ev) {
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            timing_events->push_back(e);
+        }
         timing_kinds->push_back(kind);
+        cpm_internal_time_next(ev[0], ev[1]);
     }
 
     // Enqueue one MCS for every active replica (begin + 2 launches per pass).
@@ -1499,8 +1504,8 @@ struct Engine {
                 // mechanics: half-interval OU steps before and after the pass
                 // (checkerboard.jl:522-557, lottery.jl:353-391); timed with the adjacent kernel
                 if (timing_events) {
-                    mark(0);
                     advance_mechanics(g, (uint32_t)p, 0);
+                    mark(0);
                     hip_check(cpm_internal_launch_propose(g.config, algorithm, &a, stream), "propose");
                     mark(1);
                     hip_check(cpm_internal_launch_commit(&a, stream), "commit");
@@ -1542,37 +1547,50 @@ struct Engine {
         for (Group &g : groups) g.dirty_arena_fresh = false;
         std::vector<hipEvent_t> events;
         std::vector<int> kinds;
+        hipEvent_t wall[2] = {nullptr, nullptr};
+        auto release = [&]() {
+            cpm_internal_time_next(nullptr, nullptr);
+            for (hipEvent_t e : events) (void)hipEventDestroy(e);
+            for (hipEvent_t e : wall)
+                if (e) (void)hipEventDestroy(e);
+        };
         timing_events = &events;
         timing_kinds = &kinds;
         try {
+            for (hipEvent_t &e : wall) hip_check(hipEventCreate(&e), "hipEventCreate");
+            hip_check(hipEventRecord(wall[0], stream), "hipEventRecord");
             for (uint64_t i = 0; i < n; ++i) enqueue_mcs();
-            mark(3);
+            hip_check(hipEventRecord(wall[1], stream), "hipEventRecord");
         } catch (...) {
             timing_events = nullptr;
             timing_kinds = nullptr;
-            for (hipEvent_t e : events) (void)hipEventDestroy(e);
+            (void)hipStreamSynchronize(stream);
+            release();
             throw;
         }
         timing_events = nullptr;
         timing_kinds = nullptr;
-        hip_check(hipStreamSynchronize(stream), "sync");
-        metrics[1]++;
-        for (int k = 0; k < 8; ++k) out[k] = 0.0;
-        for (size_t i = 0; i + 1 < events.size(); ++i) {
+        try {
+            hip_check(hipStreamSynchronize(stream), "sync");
+            metrics[1]++;
+            for (int k = 0; k < 8; ++k) out[k] = 0.0;
+            for (size_t i = 0; i < kinds.size(); ++i) {
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, events[2 * i], events[2 * i + 1]), "hipEventElapsedTime");
+                const int kind = kinds[i];
+                const int slot = kind == 0 ? 0 : kind == 1 ? 2 : 4;
+                out[slot] += ms;
+                out[slot + 1] += 1.0;
+            }
             float ms = 0.0f;
-            hip_check(hipEventElapsedTime(&ms, events[i], events[i + 1]), "hipEventElapsedTime");
-            const int kind = kinds[i];
-            const int slot = kind == 0 ? 0 : kind == 1 ? 2 : 4;
-            out[slot] += ms;
-            out[slot + 1] += 1.0;
-        }
-        if (events.size() > 1) {
-            float ms = 0.0f;
-            hip_check(hipEventElapsedTime(&ms, events.front(), events.back()), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&ms, wall[0], wall[1]), "hipEventElapsedTime");
             out[6] = ms;
+            out[7] = (double)n;
+        } catch (...) {
+            release();
+            throw;
         }
-        out[7] = (double)n;
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        release();
     }
 
     bool graph_parity_ok(uint32_t mcs) const {

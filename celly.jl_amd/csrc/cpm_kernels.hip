@@ -20,6 +20,28 @@
 
 #include "cpm_device.h"
 
+#include <hip/hip_ext.h>
+
+// Kernel-exact timing: an event pair set by cpm_internal_time_next is attached to the next
+// launch made here (hipExtLaunchKernelGGL), so the pair brackets that kernel's own execution
+// on its stream rather than the queue gaps around it.
+static thread_local hipEvent_t t_time_start = nullptr, t_time_stop = nullptr;
+
+#define CPM_LAUNCH(K, GRID, BLOCK, SHMEM, STREAM, ...)                                               \
+    do {                                                                                            \
+        if (t_time_start) {                                                                         \
+            hipExtLaunchKernelGGL(K, GRID, BLOCK, SHMEM, STREAM, t_time_start, t_time_stop, 0u, __VA_ARGS__); \
+            t_time_start = t_time_stop = nullptr;                                                   \
+        } else {                                                                                    \
+            hipLaunchKernelGGL(K, GRID, BLOCK, SHMEM, STREAM, __VA_ARGS__);                         \
+        }                                                                                           \
+    } while (0)
+
+extern "C" void cpm_internal_time_next(hipEvent_t start, hipEvent_t stop) {
+    t_time_start = start;
+    t_time_stop = stop;
+}
+
 namespace cpm {
 
 template <int DIM, int H, int TX, int TY, int TZ>
@@ -2266,23 +2288,23 @@ static hipError_t launch_propose_t(int algorithm, int generic, int plain, const 
     constexpr int KF = 2 * DIM;
     if (algorithm == CPM_ALG_LOTTERY) {
         if (generic && plain)
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, true>), dim3(a.ntiles), dim3(NT), 0,
+            CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, true>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else if (generic)
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, false>), dim3(a.ntiles), dim3(NT), 0,
+            CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, false>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else if (plain)
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, true>), dim3(a.ntiles), dim3(NT), 0,
+            CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, true>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else
-            hipLaunchKernelGGL((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, false>), dim3(a.ntiles), dim3(NT), 0,
+            CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, false>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
     } else {
         if (generic)
-            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, 0, NT>), dim3(a.ntiles), dim3(NT), 0,
+            CPM_LAUNCH((cb_propose_kernel<DIM, H, TX, TY, TZ, 0, NT>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else
-            hipLaunchKernelGGL((cb_propose_kernel<DIM, H, TX, TY, TZ, KF, NT>), dim3(a.ntiles), dim3(NT), 0,
+            CPM_LAUNCH((cb_propose_kernel<DIM, H, TX, TY, TZ, KF, NT>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
     }
     return hipGetLastError();
@@ -2292,10 +2314,10 @@ constexpr int kCommitWaves = 2;  // claim check on both waves, winners evaluated
 static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
     if (a.seg_cap >= 4096) {
         const uint32_t wgs = (a.ntiles + 1) / 2;
-        hipLaunchKernelGGL((commit_kernel<kCommitWaves, 2>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
+        CPM_LAUNCH((commit_kernel<kCommitWaves, 2>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
     } else {
         const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
-        hipLaunchKernelGGL((commit_kernel<kCommitWaves, kCommitTiles>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream,
+        CPM_LAUNCH((commit_kernel<kCommitWaves, kCommitTiles>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream,
                            a);
     }
     return hipGetLastError();
@@ -2338,17 +2360,17 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
         if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16)) return hipErrorInvalidValue;
         switch (config & 15) {
         case 0:
-        case 1: hipLaunchKernelGGL((cb_propose_rows_kernel<128, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 4: hipLaunchKernelGGL((cb_propose_rows_kernel<64, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 5: hipLaunchKernelGGL((cb_propose_rows_kernel<256, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 6: hipLaunchKernelGGL((cb_propose_rows_kernel<64, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 7: hipLaunchKernelGGL((cb_propose_rows_kernel<32, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 12: hipLaunchKernelGGL((cb_propose_rows_kernel<256, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 13: hipLaunchKernelGGL((cb_propose_rows_kernel<128, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 1: CPM_LAUNCH((cb_propose_rows_kernel<128, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 4: CPM_LAUNCH((cb_propose_rows_kernel<64, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 5: CPM_LAUNCH((cb_propose_rows_kernel<256, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 6: CPM_LAUNCH((cb_propose_rows_kernel<64, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 7: CPM_LAUNCH((cb_propose_rows_kernel<32, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 12: CPM_LAUNCH((cb_propose_rows_kernel<256, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 13: CPM_LAUNCH((cb_propose_rows_kernel<128, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         // 3D: 2 x 2 x 2 colour pattern, six von Neumann directions
-        case 2: hipLaunchKernelGGL((cb_propose_rows3_kernel<32, 8, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 8: hipLaunchKernelGGL((cb_propose_rows3_kernel<64, 8, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 9: hipLaunchKernelGGL((cb_propose_rows3_kernel<32, 16, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 2: CPM_LAUNCH((cb_propose_rows3_kernel<32, 8, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 8: CPM_LAUNCH((cb_propose_rows3_kernel<64, 8, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 9: CPM_LAUNCH((cb_propose_rows3_kernel<32, 16, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -2389,7 +2411,7 @@ extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const 
 extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
                                                 int nslots, hipStream_t stream) {
     const int blocks = (nslots + 63) / 64;
-    hipLaunchKernelGGL(cpm::begin_mcs_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, nslots);
+    CPM_LAUNCH(cpm::begin_mcs_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, nslots);
     return hipGetLastError();
 }
 
@@ -2402,11 +2424,11 @@ extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, c
     (void)interval;
     if (staged) {
         const size_t lds = cpm::kTiledStagedHead + (size_t)(staged + 2 * cpm::kListLds) * cpm::kTiledBlock * sizeof(int32_t);
-        hipLaunchKernelGGL(cpm::tiled_staged_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn,
+        CPM_LAUNCH(cpm::tiled_staged_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn,
                            slot, ordinal, batch);
     } else {
         const size_t lds = (size_t)2 * cpm::kListLds * cpm::kTiledBlock * sizeof(int32_t);
-        hipLaunchKernelGGL(cpm::tiled_batch_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn,
+        CPM_LAUNCH(cpm::tiled_batch_kernel, dim3(blocks), dim3(cpm::kTiledBlock), lds, stream, reps, dyn,
                            slot, ordinal, batch);
     }
     return hipGetLastError();
@@ -2416,7 +2438,7 @@ extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *rep
                                                           uint32_t capacity, hipStream_t stream) {
     const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(cpm::tiled_reconcile_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot);
+    CPM_LAUNCH(cpm::tiled_reconcile_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot);
     return hipGetLastError();
 }
 
@@ -2425,7 +2447,7 @@ extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, 
                                                        hipStream_t stream) {
     const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(cpm::mech_advance_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot, pass,
+    CPM_LAUNCH(cpm::mech_advance_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot, pass,
                        phase);
     return hipGetLastError();
 }
@@ -2434,14 +2456,14 @@ extern "C" hipError_t cpm_internal_launch_mech_init(const cpm::RepDev *reps, int
                                                     hipStream_t stream) {
     const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(cpm::mech_init_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, slot);
+    CPM_LAUNCH(cpm::mech_init_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, slot);
     return hipGetLastError();
 }
 
 extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
                                                   uint64_t target, uint32_t in_use, uint32_t mask,
                                                   hipStream_t stream) {
-    hipLaunchKernelGGL(cpm::set_dyn_kernel, dim3(1), dim3(1), 0, stream, dyn, slot, mcs, target,
+    CPM_LAUNCH(cpm::set_dyn_kernel, dim3(1), dim3(1), 0, stream, dyn, slot, mcs, target,
                        in_use, mask);
     return hipGetLastError();
 }
@@ -2461,7 +2483,7 @@ extern "C" hipError_t cpm_internal_launch_copy_probe(const void *src, void *dst,
                                                      hipStream_t stream) {
     const size_t want = (n4 + cpm::kBlock - 1) / cpm::kBlock;
     const unsigned blocks = (unsigned)(want < 8192 ? (want ? want : 1) : 8192);
-    hipLaunchKernelGGL(cpm::copy_probe_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream,
+    CPM_LAUNCH(cpm::copy_probe_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream,
                        (const cpm::v4i *)src, (cpm::v4i *)dst, n4);
     return hipGetLastError();
 }
@@ -2469,7 +2491,7 @@ extern "C" hipError_t cpm_internal_launch_copy_probe(const void *src, void *dst,
 extern "C" hipError_t cpm_internal_launch_report(const cpm::RepDev *reps, const uint32_t *slots,
                                                  uint32_t ntiles, int slot,
                                                  unsigned long long *out, hipStream_t stream) {
-    hipLaunchKernelGGL(cpm::report_kernel, dim3(1), dim3(cpm::kBlock), 0, stream, reps, slots,
+    CPM_LAUNCH(cpm::report_kernel, dim3(1), dim3(cpm::kBlock), 0, stream, reps, slots,
                        ntiles, slot, out);
     return hipGetLastError();
 }
