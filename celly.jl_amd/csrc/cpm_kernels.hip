@@ -1434,9 +1434,19 @@ __device__ __forceinline__ float copy_energy_from(const D &d, int32_t l, int32_t
 #pragma unroll
             for (int u = 0; u < kNb; ++u)
                 if (u < nc) contact_term(u, cw[u], ct[u]);
-            for (int k = kNb; k < nc; ++k) {
-                const int32_t w = more(1, k);
-                contact_term(k, w, gp(d.ctype)[w > 0 ? w : 0]);
+            // directions beyond the first kNb in groups of kTail: the group's neighbour words,
+            // then their types, are all in flight before the terms fold in canonical order
+            constexpr int kTail = 4;
+            for (int k0 = kNb; k0 < nc; k0 += kTail) {
+                int32_t gw[kTail];
+                uint32_t gt[kTail];
+#pragma unroll
+                for (int u = 0; u < kTail; ++u) gw[u] = k0 + u < nc ? more(1, k0 + u) : 0;
+#pragma unroll
+                for (int u = 0; u < kTail; ++u) gt[u] = gp(d.ctype)[gw[u] > 0 ? gw[u] : 0];
+#pragma unroll
+                for (int u = 0; u < kTail; ++u)
+                    if (k0 + u < nc) contact_term(k0 + u, gw[u], gt[u]);
             }
             v = v64 ? dd : (double)df;
         } else {
