@@ -225,11 +225,13 @@ int32_t cpm_kernel_config(const cpm_ctx *ctx, int32_t *config);
  * d2h_transfers, device_bytes_allocated, graph_launches]. */
 int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]);
 
-/* Like cpm_step, but records HIP events on the context stream around every kernel and
- * synchronizes at the end. out[0..7] = propose-kernel total ms, propose launches,
- * commit-kernel total ms, commit launches, begin-kernel total ms, begin launches,
- * wall ms from the first to the last event, MCS run. (No counterpart in the reference:
- * the measurement hook behind the per-kernel roofline.) */
+/* Like cpm_step, but attaches a HIP event pair to every kernel launch on the context stream
+ * (the pair brackets that kernel's execution) and synchronizes at the end. out[0..7] =
+ * propose-kernel total ms (TiledCheckerboardCPM: batch kernel), propose launches,
+ * commit-kernel total ms (tiled: reconciliation), commit launches, begin-kernel total ms,
+ * begin launches, wall ms of the whole run, MCS run. Mechanics kernels are not in any of the
+ * kernel totals. (No counterpart in the reference: the measurement hook behind the per-kernel
+ * roofline.) */
 int32_t cpm_step_timed(cpm_ctx *ctx, uint64_t n, double out[8]);
 
 /* Tile shapes of the propose kernels (the autotuner's search space): out = {dim, halo,
