@@ -141,3 +141,40 @@ def test_3d_row_kernels_partial_tiles(gpu_lib, shape):
         _parity(workloads.block_cells(68, 3, 6, 8, seed=518), 3, True)
     finally:
         set_tile_shape(ALG_CHECKERBOARD, 3, 1, -1)
+
+
+def _site_cells(dims, seed):
+    """Every site its own cell (volume target 2, Moore contact, boundary tracker) at a high
+    temperature: a tile's attempts touch more distinct cells than the claim table holds, so
+    most claims take the global-atomic path."""
+    from corepotts import (BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain, CheckerboardSweepCPM,
+                           ContactRole, LogicalPottsState, MediumTypeTable, ProposalRole,
+                           QuadraticVolumeHamiltonian, ScientificComponentSet, SurfaceRole,
+                           UnorderedContactHamiltonian, compile_scientific_state, first_shell_relation,
+                           moore_relation, required_properties)
+    ndim = len(dims)
+    n = int(np.prod(dims))
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    contact = UnorderedContactHamiltonian(np.array([[0, 4, 4], [4, 2, 6], [4, 6, 3]], np.float32),
+                                          MediumTypeTable({1: 1}), moore_relation(ContactRole, ndim))
+    owners = np.arange(1, n + 1, dtype=np.int64).reshape(dims, order="F")
+    state = LogicalPottsState(owners, n, cell_types={c: 2 + c % 2 for c in range(1, n + 1)}, medium_domains=(1,),
+                              property_schema=required_properties(volume))
+    state.property_values("target_volume")[:] = 2.0
+    state.property_values("volume_strength")[:] = 0.5
+    surface = first_shell_relation(SurfaceRole, ndim)
+    compiled = compile_scientific_state(state, CartesianDomain(dims), BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
+    return (compiled, first_shell_relation(ProposalRole, ndim), ScientificComponentSet(energies=(volume, contact)),
+            CheckerboardSweepCPM(np.float32(40.0)), seed)
+
+
+@pytest.mark.parametrize("rows", [True, False], ids=["rows", "tiles"])
+@pytest.mark.parametrize("dims", [(256, 32), (32, 16, 16)], ids=["2d", "3d"])
+def test_claim_table_overflow(gpu_lib, rows, dims):
+    """One cell per site: the propose kernels' per-tile claim tables overflow and fall back to
+    global atomics for most cells, with results bit-exact against the oracle."""
+    set_row_kernels(rows)
+    try:
+        _parity(_site_cells(dims, 4711), 3, rows)
+    finally:
+        set_row_kernels(True)
