@@ -323,6 +323,31 @@ __device__ __forceinline__ float uniform_open01(uint32_t w0) {
 __device__ inline void schedule_order(uint32_t key_lo, uint32_t key_hi, uint32_t stream,
                                       uint32_t operation, uint64_t mcs, int count,
                                       uint32_t *order) {
+    constexpr int kRegs = 16;
+    if (count <= kRegs) {
+        // the permutation in registers (swaps by selects), stored once: no store-to-load
+        // round trips through global memory between the swaps
+        uint32_t p[kRegs];
+#pragma unroll
+        for (int i = 0; i < kRegs; ++i) p[i] = (uint32_t)(i + 1);
+        for (int c = count; c >= 2; --c) {
+            const uint32_t draw = (uint32_t)(count - c);
+            const int sel = (int)bounded_uint(key_lo, key_hi, stream, mcs, 0, operation, kGlobalEntity, 0, draw,
+                                              (uint32_t)c);
+            uint32_t vc = 0, vs = 0;
+#pragma unroll
+            for (int i = 0; i < kRegs; ++i) {
+                vc = i == c - 1 ? p[i] : vc;
+                vs = i == sel ? p[i] : vs;
+            }
+#pragma unroll
+            for (int i = 0; i < kRegs; ++i) p[i] = i == c - 1 ? vs : i == sel ? vc : p[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kRegs; ++i)
+            if (i < count) order[i] = p[i];
+        return;
+    }
     for (int i = 0; i < count; ++i) order[i] = (uint32_t)(i + 1);
     for (int c = count; c >= 2; --c) {
         const uint32_t draw = (uint32_t)(count - c);
