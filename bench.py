@@ -9,13 +9,18 @@ Random numbers are the reference's Philox semantic addresses, so the run is synt
 fully deterministic. Timed region: `--steps` MCS after `--warmup` MCS, synchronized on both
 sides. Attempts per MCS = activated_attempts of the report (one per mutable site).
 
-Roofline: the dominant kernel is the colour-pass propose kernel. Its algorithmic traffic is
-one read of the Int32 lattice per launch (4 bytes x sites); its average duration comes from
-HIP event pairs attached to every kernel's launch on the engine stream (hipExtLaunchKernelGGL
-in cpm_step_timed), which bracket the kernel's own execution. `traffic` is
-the measured HBM bytes per propose launch (FETCH_SIZE + WRITE_SIZE hardware counters, each
-scaled by what the counter reads for the library's plain copy kernel of known size), taken
-from profiles/cb_propose_4096_traffic.json when that file is present.
+Roofline: the sweep is HBM-bound stencil work. Its algorithmic traffic per MCS is one read of
+the Int32 lattice per colour pass plus, per accepted copy, the lattice write and the volume and
+boundary tracker updates: B_MCS = 4 * C * N + A * (4 + 16 + 32) bytes (C colour passes, N sites,
+A accepted copies of the MCS). `roofline.achieved` = B_MCS x MCS/s of the timed run, so the
+fraction covers every kernel of the MCS (propose, commit and the colour-order kernel), not one
+of them. The per-kernel figures (propose: 4 * N bytes per launch over its average duration,
+from HIP event pairs attached to every kernel's launch on the engine stream by
+hipExtLaunchKernelGGL in cpm_step_timed) are under `extra.kernels`. `traffic` is the measured
+HBM bytes of one MCS: C x (propose + commit) + the colour-order kernel, each from the
+FETCH_SIZE and WRITE_SIZE hardware counters scaled by what the counters read for the
+library's plain copy kernel of known size, taken from profiles/cb_4096_traffic.json when that
+file is present.
 """
 import argparse
 import json
@@ -36,13 +41,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """Measured HBM bytes per propose launch from the committed counter summary, if present."""
-    path = os.path.join(ROOT, "profiles", "cb_propose_4096_traffic.json")
+def traffic_from_profiles(passes: int):
+    """Measured HBM bytes of one MCS (passes x (propose + commit) + colour-order kernel) from the
+    committed counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "cb_4096_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as handle:
-        return json.load(handle).get("propose", {}).get("bytes_per_launch")
+        t = json.load(handle)
+    try:
+        return (passes * (t["propose"]["bytes_per_launch"] + t["commit"]["bytes_per_launch"])
+                + t["begin_mcs"]["bytes_per_launch"])
+    except KeyError:
+        return None
 
 
 def cpu_baseline(sample_mcs: int):
@@ -126,15 +137,56 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
             "device_bytes": device_bytes}
 
 
-def cubes_leg(steps: int, graph_chunk: int):
+def lottery_leg(steps: int, shape: int = -1):
+    """4096^2 LotteryCPM on the headline workload (9 rounds per MCS: the Moore shell's conflict
+    degree 8 + 1), tuned tile shape, then event-timed kernels."""
+    from corepotts import init_scientific, set_tile_shape, tile_shapes, tune_tile_shape, workloads
+    from corepotts.spec import ALG_LOTTERY
+    log(f"lottery {steps} MCS")
+    compiled, proposal, components, lalg, lseed = workloads.config2_large(algorithm="lottery")
+    n = int(compiled.tags.size)
+    if shape >= 0:
+        set_tile_shape(ALG_LOTTERY, 2, 1, shape)
+        ltuned = {"shape": shape, "tile": dict(tile_shapes(2, 1))[shape], "pinned": True}
+    else:
+        ltuned = tune_tile_shape(compiled, proposal, components, lalg, seed=lseed, mcs=2, warmup=1)
+    lot = init_scientific(compiled, proposal, components, lalg, seed=lseed)
+    lot.step(5)
+    lot.synchronize()
+    t0 = time.perf_counter()
+    lot.step(steps)
+    lot.synchronize()
+    dt = time.perf_counter() - t0
+    lrep = lot.current_mcs_report()
+    timing = lot.step_timed(2)
+    trep = lot.current_mcs_report()
+    propose_us = 1e3 * timing["propose_ms"] / max(timing["propose_launches"], 1)
+    commit_us = 1e3 * timing["commit_ms"] / max(timing["commit_launches"], 1)
+    mcs_bytes = 4.0 * trep.internal_rounds * n + trep.accepted_copies * (4 + 16 + 32)
+    out = {"copy_attempts_per_s": lrep.activated_attempts * steps / dt,
+           "mcs_per_s": steps / dt, "rounds": lrep.internal_rounds, "tile_shape": ltuned,
+           "propose_kernel_avg_us": propose_us, "commit_kernel_avg_us": commit_us,
+           "algorithmic_bytes_per_mcs": mcs_bytes,
+           "whole_mcs_algorithmic_GBps": mcs_bytes * steps / dt / 1e9,
+           "last_report": lrep._asdict()}
+    lot.close()
+    return out
+
+
+def cubes_leg(steps: int, graph_chunk: int, shape: int = -1):
     """256^3 CheckerboardSweepCPM: 32,768 cubes of 8^3 tiling the lattice, 3 types, Float32
     contact matrix U[2, 20], volume (1.5, 512) + 26-neighbour contact + boundary (0.5, 384),
     T = 20."""
-    from corepotts import init_scientific, tune_tile_shape, workloads
+    from corepotts import init_scientific, set_tile_shape, tile_shapes, tune_tile_shape, workloads
+    from corepotts.spec import ALG_CHECKERBOARD
     log("3D: building 256^3 workload")
     model = workloads.cubes_3d(256)
     n = int(model[0].tags.size)
-    tuned = tune_tile_shape(*model, mcs=2, warmup=1)
+    if shape >= 0:
+        set_tile_shape(ALG_CHECKERBOARD, 3, 1, shape)
+        tuned = {"shape": shape, "tile": dict(tile_shapes(3, 1))[shape], "pinned": True}
+    else:
+        tuned = tune_tile_shape(*model, mcs=2, warmup=1)
     integ = init_scientific(*model)
     integ.set_graph_chunk(graph_chunk)
     integ.step(2 * graph_chunk)
@@ -146,7 +198,11 @@ def cubes_leg(steps: int, graph_chunk: int):
     dt = time.perf_counter() - t0
     rep = integ.current_mcs_report()
     timing = integ.step_timed(4)
+    trep = integ.current_mcs_report()
+    mcs_bytes = 4.0 * trep.internal_rounds * n + trep.accepted_copies * (4 + 16 + 32)
     out = {"lattice": [256, 256, 256], "cells": 32768, "colour_passes_per_mcs": rep.internal_rounds,
+           "algorithmic_bytes_per_mcs": mcs_bytes,
+           "whole_mcs_algorithmic_GBps": mcs_bytes * steps / dt / 1e9,
            "tile_shape": tuned,
            "copy_attempts_per_s": rep.activated_attempts * steps / dt, "mcs_per_s": steps / dt,
            "propose_kernel_avg_us": 1e3 * timing["propose_ms"] / max(timing["propose_launches"], 1),
@@ -212,38 +268,9 @@ def serving_leg(jobs: int, chunk: int, budget_gb: float):
     return summary
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--graph-chunk", type=int, default=50)
-    ap.add_argument("--timing-mcs", type=int, default=40)
-    ap.add_argument("--lottery-steps", type=int, default=50)
-    ap.add_argument("--cpu-sample-mcs", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-lottery", action="store_true")
-    ap.add_argument("--copy-probe-mib", type=int, default=256)
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-3d", action="store_true")
-    ap.add_argument("--cubes-steps", type=int, default=40)
-    ap.add_argument("--batch-replicas", type=int, default=512)
-    ap.add_argument("--batch-mcs", type=int, default=500)
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-tiled", action="store_true")
-    ap.add_argument("--tiled-steps", type=int, default=20)
-    ap.add_argument("--serve-jobs", type=int, default=200)
-    ap.add_argument("--serve-chunk", type=int, default=20)
-    ap.add_argument("--tile-shape", type=int, default=-1,
-                    help="2D checkerboard tile shape of the headline run (default: autotuned)")
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this engine runs one lattice (or one batch of "
-              "replicas) per GPU with no sharding or collectives; run with --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
-
+def headline_leg(args):
+    """The 4096^2 CheckerboardSweepCPM run: tuned tile shape, `--warmup` MCS, then `--steps`
+    timed MCS with HIP graphs over `--graph-chunk` MCS chunks; then event-timed kernels."""
     from corepotts import copy_probe, init_scientific, set_tile_shape, tile_shapes, tune_tile_shape, workloads
     from corepotts.spec import ALG_CHECKERBOARD
 
@@ -292,15 +319,20 @@ def main():
     timing = integ.step_timed(args.timing_mcs)
     propose_avg_ms = timing["propose_ms"] / max(timing["propose_launches"], 1)
     commit_avg_ms = timing["commit_ms"] / max(timing["commit_launches"], 1)
-    bytes_per_launch = 4.0 * n_sites
-    achieved = bytes_per_launch / (propose_avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    begin_avg_ms = timing["begin_ms"] / max(timing["begin_launches"], 1)
     rep_t = integ.current_mcs_report()
     passes = rep_t.internal_rounds
+    # algorithmic bytes of one MCS: a lattice read per colour pass + the accepted copies'
+    # lattice write and tracker updates (4 + 2 x 4 volume RMW + 2 x 8 boundary RMW, read and
+    # write: 4 + 16 + 32 bytes)
     mcs_bytes = 4.0 * passes * n_sites + rep_t.accepted_copies * (4 + 16 + 32)
+    mcs_per_s = args.steps / elapsed
+    achieved = mcs_bytes * mcs_per_s / 1e9
+    traffic = traffic_from_profiles(passes)
+    propose_gbps = 4.0 * n_sites / (propose_avg_ms * 1e-3) / 1e9
 
     extra = {
-        "mcs_per_s": args.steps / elapsed,
+        "mcs_per_s": mcs_per_s,
         "tile_shape": tuned,
         "pcie": {"upload_ms": upload_s * 1e3, "download_ms": download_s * 1e3,
                  "pcie_inclusive_attempts_per_s":
@@ -308,47 +340,18 @@ def main():
         "copy_probe": {"bytes": copy_bytes, "avg_ms": copy_ms,
                        "GBps": 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9},
         "colour_passes_per_mcs": passes,
-        "propose_kernel_avg_us": propose_avg_ms * 1e3,
-        "commit_kernel_avg_us": commit_avg_ms * 1e3,
-        "event_wall_ms_per_mcs": timing["wall_ms"] / max(timing["mcs"], 1),
         "algorithmic_bytes_per_mcs": mcs_bytes,
-        "whole_mcs_algorithmic_GBps": mcs_bytes * (args.steps / elapsed) / 1e9,
+        "kernels": {
+            "propose_avg_us": propose_avg_ms * 1e3,
+            "propose_algorithmic_GBps": propose_gbps,
+            "propose_frac_of_peak": propose_gbps / HBM_PEAK_GBS,
+            "commit_avg_us": commit_avg_ms * 1e3,
+            "begin_mcs_avg_us": begin_avg_ms * 1e3,
+            "event_wall_ms_per_mcs": timing["wall_ms"] / max(timing["mcs"], 1),
+        },
         "last_report": rep._asdict(),
     }
     del integ
-
-    if not args.no_lottery:
-        log(f"lottery {args.lottery_steps} MCS")
-        compiled, proposal, components, lalg, lseed = workloads.config2_large(algorithm="lottery")
-        ltuned = tune_tile_shape(compiled, proposal, components, lalg, seed=lseed, mcs=2, warmup=1)
-        lot = init_scientific(compiled, proposal, components, lalg, seed=lseed)
-        lot.step(5)
-        lot.synchronize()
-        t0 = time.perf_counter()
-        lot.step(args.lottery_steps)
-        lot.synchronize()
-        dt = time.perf_counter() - t0
-        lrep = lot.current_mcs_report()
-        extra["lottery"] = {"copy_attempts_per_s": lrep.activated_attempts * args.lottery_steps / dt,
-                            "mcs_per_s": args.lottery_steps / dt, "rounds": lrep.internal_rounds,
-                            "tile_shape": ltuned,
-                            "last_report": lrep._asdict()}
-        del lot
-
-    if not args.no_3d:
-        extra["cubes_3d"] = cubes_leg(args.cubes_steps, 10)
-    if not args.no_tiled:
-        extra["tiled"] = tiled_leg(args.tiled_steps)
-    if not args.no_batch:
-        extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
-    if not args.no_serve:
-        extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0)
-
-    baseline = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle)")
-        baseline = cpu_baseline(args.cpu_sample_mcs)
-
     line = {
         "metric": METRIC,
         "value": value,
@@ -366,10 +369,76 @@ def main():
                                "volume+Moore contact+boundary, T=10, seed 1234",
                    "lattice": [4096, 4096], "cells": 163840, "algorithm": "CheckerboardSweepCPM"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "cpu_baseline": baseline,
-        "extra": extra,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "scope": "whole MCS: B_MCS x MCS/s, B_MCS = 4*C*N + A*52 bytes"},
     }
+    return line, extra
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--graph-chunk", type=int, default=50)
+    ap.add_argument("--timing-mcs", type=int, default=40)
+    ap.add_argument("--lottery-steps", type=int, default=50)
+    ap.add_argument("--cpu-sample-mcs", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lottery", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--cubes-steps", type=int, default=40)
+    ap.add_argument("--batch-replicas", type=int, default=512)
+    ap.add_argument("--batch-mcs", type=int, default=500)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tiled", action="store_true")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="skip the 4096^2 checkerboard run (profiling the other legs alone)")
+    ap.add_argument("--tiled-steps", type=int, default=20)
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-chunk", type=int, default=20)
+    ap.add_argument("--tile-shape", type=int, default=-1,
+                    help="2D checkerboard tile shape of the headline run (default: autotuned)")
+    ap.add_argument("--lottery-tile-shape", type=int, default=-1,
+                    help="2D lottery tile shape (default: autotuned)")
+    ap.add_argument("--cubes-tile-shape", type=int, default=-1,
+                    help="3D checkerboard tile shape (default: autotuned)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this engine runs one lattice (or one batch of "
+              "replicas) per GPU with no sharding or collectives; run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from corepotts import init_scientific, tune_tile_shape, workloads
+
+    line = {"metric": METRIC, "n_gpus": 1, "vs_baseline": None}
+    extra = {}
+    if not args.no_headline:
+        line, extra = headline_leg(args)
+
+    if not args.no_lottery:
+        extra["lottery"] = lottery_leg(args.lottery_steps, args.lottery_tile_shape)
+
+    if not args.no_3d:
+        extra["cubes_3d"] = cubes_leg(args.cubes_steps, 10, args.cubes_tile_shape)
+    if not args.no_tiled:
+        extra["tiled"] = tiled_leg(args.tiled_steps)
+    if not args.no_batch:
+        extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
+    if not args.no_serve:
+        extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0)
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle)")
+        baseline = cpu_baseline(args.cpu_sample_mcs)
+
+    line["cpu_baseline"] = baseline
+    line["extra"] = extra
     print(json.dumps(line), flush=True)
 
 
