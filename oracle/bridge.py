@@ -110,6 +110,8 @@ def load() -> C.CDLL:
     lib.orc_sample_transitions.argtypes = [P, U64, U64, P, C.c_int32, P]
     lib.orc_copy_evaluation.restype = C.c_int32
     lib.orc_copy_evaluation.argtypes = [P, U32, C.c_int32, P]
+    lib.orc_copy_energy_fold.restype = C.c_int32
+    lib.orc_copy_energy_fold.argtypes = [P, U32, C.c_int32, C.POINTER(C.c_double)]
     lib.orc_set_tiled_reconcile.argtypes = [P, C.c_int32]
     lib.orc_tiled_layout.restype = C.c_int32
     lib.orc_tiled_layout.argtypes = [P, P, P, P, P]
@@ -252,6 +254,12 @@ class OracleSim:
         out = np.zeros(2, np.float32)
         rc = self.lib.orc_copy_evaluation(self._sim, int(recipient), int(direction), out.ctypes.data)
         return int(rc), out[0], out[1]
+
+    def copy_energy_fold(self, recipient: int, direction: int):
+        """(outcome, energy fold in the terms' own types, before the Float32 rounding)."""
+        out = C.c_double()
+        rc = self.lib.orc_copy_energy_fold(self._sim, int(recipient), int(direction), C.byref(out))
+        return int(rc), out.value
 
     def set_tiled_reconcile(self, recount: bool):
         """Reconcile tiled subrounds by recount from ownership (True) or snapshot plus deltas."""

@@ -829,9 +829,10 @@ static double boundary_term(const orc_sim *s, int64_t losing, int64_t gaining, i
     return (double)r;
 }
 
-/* scientific_inner_loop.jl:213-217, 257-269: ((0f0 + E1) + E2) + ... with promotion. */
-static float delta_h(const orc_sim *s, uint32_t recipient, int64_t losing, int64_t gaining,
-                     int64_t dl, int64_t dg) {
+/* scientific_inner_loop.jl:213-217: ((0 + E1) + E2) + ... with promotion, before evaluate_copy
+ * rounds the fold to the algorithm's number type (:257-259). */
+static double energy_fold(const orc_sim *s, uint32_t recipient, int64_t losing, int64_t gaining,
+                          int64_t dl, int64_t dg) {
     double acc = 0.0;
     int acc64 = 0;
     for (int t = 0; t < s->m.n_energies; ++t) {
@@ -855,7 +856,13 @@ static float delta_h(const orc_sim *s, uint32_t recipient, int64_t losing, int64
             acc64 = 1;
         }
     }
-    return (float)acc;
+    return acc;
+}
+
+/* evaluate_copy's delta_h: the fold rounded once to Float32 (scientific_inner_loop.jl:257-259). */
+static float delta_h(const orc_sim *s, uint32_t recipient, int64_t losing, int64_t gaining,
+                     int64_t dl, int64_t dg) {
+    return (float)energy_fold(s, recipient, losing, gaining, dl, dg);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1450,6 +1457,17 @@ int32_t orc_copy_evaluation(const orc_sim *s, uint32_t recipient, int32_t direct
     boundary_change(s, recipient, l, g, &dl, &dg);
     out[0] = delta_h(s, recipient, l, g, dl, dg);
     out[1] = mechanical_work(s, l, g, dl, dg);
+    return outcome;
+}
+
+int32_t orc_copy_energy_fold(const orc_sim *s, uint32_t recipient, int32_t direction, double *out) {
+    int64_t l, g, dl = 0, dg = 0;
+    uint32_t donor;
+    int outcome = construct_attempt(s, recipient, direction, &l, &g, &donor);
+    *out = 0.0;
+    if (outcome != OUT_ACTIONABLE) return outcome;
+    boundary_change(s, recipient, l, g, &dl, &dg);
+    *out = energy_fold(s, recipient, l, g, dl, dg);
     return outcome;
 }
 

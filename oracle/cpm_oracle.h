@@ -177,6 +177,9 @@ int32_t orc_tiled_layout(const orc_sim *s, int32_t grid[3], int32_t reach[3], in
 
 /* delta_h and mechanical work of one proposal (recipient site, 1-based direction) */
 int32_t orc_copy_evaluation(const orc_sim *s, uint32_t recipient, int32_t direction, float out[2]);
+/* the energy fold of one proposal in the terms' own types, before evaluate_copy rounds it to
+ * Float32 (the per-term energy_change sum the reference's local/global identity compares) */
+int32_t orc_copy_energy_fold(const orc_sim *s, uint32_t recipient, int32_t direction, double *out);
 orc_state *orc_state_of(orc_sim *s);     /* mutable view of the simulation state */
 void orc_set_seed(orc_sim *s, uint64_t seed);
 void orc_set_mcs(orc_sim *s, uint64_t mcs);
