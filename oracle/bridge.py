@@ -123,6 +123,7 @@ def load() -> C.CDLL:
     lib.orc_ou_transition.argtypes = [C.c_float] * 7
     lib.orc_total_energy.restype = C.c_double
     lib.orc_total_energy.argtypes = [P]
+    lib.orc_accepted_energy.argtypes = [P, P]
     _lib = lib
     return lib
 
@@ -323,6 +324,12 @@ class OracleSim:
 
     def total_energy(self) -> float:
         return float(self.lib.orc_total_energy(self._sim))
+
+    def accepted_energy(self):
+        """(sum of accepted delta_h over the last MCS, since the state was loaded), FP64."""
+        out = np.zeros(2, np.float64)
+        self.lib.orc_accepted_energy(self._sim, out.ctypes.data)
+        return float(out[0]), float(out[1])
 
     def sample_transitions(self, seed_base: int, replicas: int, labels) -> np.ndarray:
         labels = np.ascontiguousarray(labels, dtype=np.int64)

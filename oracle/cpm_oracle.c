@@ -246,6 +246,9 @@ struct orc_sim {
     uint32_t *color_tile_offsets; /* color_count + 1 */
     int32_t tiled_interval, tiled_batches, tiled_recount;
     int64_t *acc_dvol, *acc_dbnd;
+    /* FP64 sums of the accepted copies' Float32 delta_h: over the last MCS and since the
+       state was loaded (drift checks against the global energy) */
+    double dh_last_mcs, dh_total;
 };
 
 enum { OUT_ACTIONABLE = 1, OUT_SAME_OWNER = 2, OUT_BOUNDARY_NULL = 3, OUT_IMMUTABLE = 4 };
@@ -645,6 +648,7 @@ static void copy_state(const orc_model *m, int64_t n, const orc_state *src, orc_
 }
 
 int32_t orc_load_state(orc_sim *s, const orc_state *src) {
+    s->dh_last_mcs = s->dh_total = 0.0;
     copy_state(&s->m, s->n, src, &s->st);
     return 0;
 }
@@ -1107,6 +1111,7 @@ static void evaluate_selected(orc_sim *s, uint32_t a, uint64_t mcs, uint32_t sub
     orc_rng_words(s->seed, S_ACCEPTANCE, mcs, subround, 0, K_SITE, rec, 0, 0, 0, w);
     float u = orc_uniform_open01_f32(w[0]);
     s->att_disp[a] = u < p ? D_ACCEPTED : D_REJECTED;
+    if (u < p) s->dh_last_mcs += (double)dh;
 }
 
 /* checkerboard.jl:521-586 */
@@ -1348,6 +1353,7 @@ static void tiled_mcs(orc_sim *s) {
                     orc_rng_words(s->seed, S_TILED_PROPOSAL, mcs, subround, lp, K_SITE, tile, 0, 0, 2, w);
                     if (!(orc_uniform_open01_f32(w[0]) < p)) { rejected++; continue; }
                     accepted++;
+                    s->dh_last_mcs += (double)dh;
                     st->tags[rec - 1] = g > 0 ? 1 : 2;
                     st->ids[rec - 1] = (uint32_t)(g > 0 ? g : -g);
                     int64_t cells[2] = {l, g}, dv[2] = {-1, 1}, db[2] = {dl, dg};
@@ -1432,6 +1438,7 @@ static void sequential_mcs(orc_sim *s) {
         float u = orc_uniform_open01_f32(w[0]);
         if (u < p) {
             commit(s, rec, l, g, dl, dg);
+            s->dh_last_mcs += (double)dh;
             accepted++;
         } else {
             rejected++;
@@ -1473,12 +1480,19 @@ int32_t orc_copy_energy_fold(const orc_sim *s, uint32_t recipient, int32_t direc
 
 int32_t orc_step(orc_sim *s, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {
+        s->dh_last_mcs = 0.0;
         if (s->algorithm == ORC_ALG_CHECKERBOARD) checkerboard_mcs(s);
         else if (s->algorithm == ORC_ALG_LOTTERY) lottery_mcs(s);
         else if (s->algorithm == ORC_ALG_TILED) tiled_mcs(s);
         else sequential_mcs(s);
+        s->dh_total += s->dh_last_mcs;
     }
     return 0;
+}
+
+void orc_accepted_energy(const orc_sim *s, double out[2]) {
+    out[0] = s->dh_last_mcs;
+    out[1] = s->dh_total;
 }
 
 int32_t orc_report(const orc_sim *s, uint64_t out[12]) {

@@ -218,6 +218,9 @@ int32_t orc_sample_transitions(orc_sim *s, uint64_t seed_base, uint64_t replicas
 
 /* Global energies (FP64 accumulations) for statistical checks. */
 double orc_total_energy(const orc_sim *s);
+/* FP64 sums of the accepted copies' delta_h (the Float32 values the acceptance used):
+ * out[0] over the last MCS, out[1] since the state was loaded. */
+void orc_accepted_energy(const orc_sim *s, double out[2]);
 
 #ifdef __cplusplus
 }
