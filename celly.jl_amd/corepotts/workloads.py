@@ -256,3 +256,36 @@ def serving_model(job: "ServingJob", algorithm="checkerboard"):
     side = 10 if job.ndim == 2 else 8
     return block_cells(job.side, job.ndim, side, side, ntypes=job.ntypes, J=J,
                        temperature=np.float32(10.0), seed=job.seed, algorithm=algorithm)
+
+
+def single_cell_migration(algorithm="checkerboard", seed: int = 0, L: int = 64):
+    """The reference's realistic single-cell-migration workload
+    (integration/transition/RealisticScaleRunner.jl:126-148 with the manifest's model,
+    design/audits/phase-13-realistic-workload-manifest.toml): an L x L periodic lattice of
+    medium id 1 (type 1) holding cell 1 (type 2) as a centred 5 x 5 square (sites
+    fld(L - 5, 2) + 1 ... + 5 of each axis, 1-based); as PottsToolkit lowers it
+    (src/authoring/lowering.jl:468-505): first-shell von Neumann proposal, contact and surface
+    relations, energies (volume, contact) in Float32 with target volume 25 and strength 1,
+    contact J (medium-medium 0, medium-cell 12, cell-cell 4), a BoundaryEdgeCount tracker,
+    capacity 1, T = 5. `seed` is the replica's dynamics seed."""
+    side = 5
+    start = (L - side) // 2
+    owners = np.full((L, L), -1, dtype=np.int64)
+    owners[start:start + side, start:start + side] = 1
+    f32 = np.float32
+    spacing = (f32(1.0), f32(1.0))
+    proposal = first_shell_relation(ProposalRole, 2, spacing=spacing)
+    surface = first_shell_relation(SurfaceRole, 2, spacing=spacing)
+    contact_rel = first_shell_relation(ContactRole, 2, spacing=spacing)
+    volume = QuadraticVolumeHamiltonian(number_type=np.float32)
+    contact = UnorderedContactHamiltonian(np.array([[0, 12], [12, 4]], dtype=np.float32),
+                                          MediumTypeTable({1: 1}), contact_rel)
+    energies = (volume, contact)
+    state = LogicalPottsState(owners, 1, cell_types={1: 2}, medium_domains=(1,),
+                              property_schema=_schema(*energies))
+    state.property_values("target_volume")[:] = f32(25.0)
+    state.property_values("volume_strength")[:] = f32(1.0)
+    compiled = compile_scientific_state(state, CartesianDomain((L, L), spacing=spacing),
+                                        BoundaryMeasureTracker(BoundaryEdgeCount(), surface))
+    alg = make_algorithm(algorithm, f32(5.0))
+    return compiled, proposal, ScientificComponentSet(energies=energies), alg, seed

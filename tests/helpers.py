@@ -127,3 +127,101 @@ def oracle_for(compiled, proposal, components, algorithm, seed, alg_override=Non
     if spec.mechanics and initialize_mechanics:
         sim.initialize_mechanics()
     return sim, spec
+
+
+# ---------------------------------------------------------------------------
+# The reference's realistic single-cell-migration evidence
+# ---------------------------------------------------------------------------
+def load_migration():
+    with open(os.path.join(GOLDEN, "realistic_single_cell_migration.json")) as handle:
+        return json.load(handle)
+
+
+def _cell_center(owner_words, dims):
+    """_cell_center (integration/transition/RealisticScaleRunner.jl:230-241): per axis the
+    circular mean of the 1-based coordinates of the cell's sites."""
+    L0, L1 = dims
+    idx = np.nonzero(owner_words == 1)[0]
+    coords = (idx % L0 + 1, idx // L0 + 1)
+    out = []
+    for axis, d in enumerate(dims):
+        angles = 2 * np.pi * (coords[axis] - 1) / d
+        angle = np.arctan2(np.mean(np.sin(angles)), np.mean(np.cos(angles)))
+        out.append(1 + np.mod(angle, 2 * np.pi) * d / (2 * np.pi))
+    return tuple(out)
+
+
+def _wrapped(first, last, dims):
+    out = []
+    for axis, d in enumerate(dims):
+        delta = last[axis] - first[axis]
+        if delta > d / 2:
+            delta -= d
+        if delta < -d / 2:
+            delta += d
+        out.append(delta)
+    return out
+
+
+def _mixing_time(energies, cadence, window):
+    """_mixing_time (RealisticScaleRunner.jl:252-264)."""
+    if not energies:
+        return 0
+    window = min(window, len(energies))
+    rolling = [float(np.mean(energies[max(0, i - window + 1):i + 1])) for i in range(len(energies))]
+    target = rolling[-1]
+    tol = 0.05 * max(abs(target), np.finfo(np.float64).eps)
+    for i in range(len(rolling)):
+        if all(abs(v - target) <= tol for v in rolling[i:]):
+            return (i + 1) * cadence
+    return len(energies) * cadence
+
+
+def migration_observables(step, observe, dims=(64, 64), burn_in=500, measured=2000, cadence=5,
+                          window=20):
+    """run_realistic_replica (RealisticScaleRunner.jl:266-360) for the single-cell workload:
+    `step(n)` advances n MCS; `observe()` returns (global energy, boundary measure of cell 1,
+    owner words) after a step. Returns the observables the evidence records."""
+    step(burn_in)
+    energies, boundaries, centers = [], [], []
+    sites = dims[0] * dims[1]
+    for _ in range(measured // cadence):
+        step(cadence)
+        energy, boundary, owners = observe()
+        energies.append(energy / sites)
+        boundaries.append(float(boundary))
+        centers.append(_cell_center(owners, dims))
+    disp = _wrapped(centers[0], centers[-1], dims)
+    steps = [_wrapped(centers[i - 1], centers[i], dims) for i in range(1, len(centers))]
+    distances = [float(np.hypot(*s)) for s in steps]
+    net = float(np.hypot(*disp))
+    return {"energy_per_mutable_site_terminal": energies[-1],
+            "energy_per_mutable_site_mean": float(np.mean(energies)),
+            "boundary_measure_mean": float(np.mean(boundaries)),
+            "mixing_time_mcs": _mixing_time(energies, cadence, window),
+            "net_displacement": net,
+            "mean_squared_displacement": float(np.mean([s[0] ** 2 + s[1] ** 2 for s in steps])),
+            "speed": float(np.mean(distances)) / cadence,
+            "persistence": 0.0 if sum(distances) == 0 else net / sum(distances)}
+
+
+EXACT_MIGRATION_FIELDS = ("energy_per_mutable_site_terminal", "energy_per_mutable_site_mean",
+                          "boundary_measure_mean", "mixing_time_mcs")
+FLOAT_MIGRATION_FIELDS = ("net_displacement", "mean_squared_displacement", "speed", "persistence")
+
+
+def oracle_migration_replica(algorithm: str, dynamics_seed: int):
+    """One replica of the single-cell-migration workload on the oracle (algorithm
+    'sequential' or 'checkerboard')."""
+    from corepotts import workloads
+    from oracle.bridge import OracleSim
+    model = workloads.single_cell_migration("checkerboard", dynamics_seed)
+    spec = build_spec(*model)
+    sim = OracleSim(spec, state_arrays(model[0], spec), 0 if algorithm == "sequential" else None)
+
+    def observe():
+        st = sim.store()
+        words = np.where(st["tags"] == 1, st["ids"].astype(np.int64), -st["ids"].astype(np.int64))
+        return sim.total_energy(), int(st["boundary_measures"][0]), words
+
+    return migration_observables(sim.step, observe)
