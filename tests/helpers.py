@@ -137,7 +137,7 @@ def load_migration():
         return json.load(handle)
 
 
-def _cell_center(owner_words, dims):
+def cell_center(owner_words, dims):
     """_cell_center (integration/transition/RealisticScaleRunner.jl:230-241): per axis the
     circular mean of the 1-based coordinates of the cell's sites."""
     L0, L1 = dims
@@ -177,20 +177,10 @@ def _mixing_time(energies, cadence, window):
     return len(energies) * cadence
 
 
-def migration_observables(step, observe, dims=(64, 64), burn_in=500, measured=2000, cadence=5,
-                          window=20):
-    """run_realistic_replica (RealisticScaleRunner.jl:266-360) for the single-cell workload:
-    `step(n)` advances n MCS; `observe()` returns (global energy, boundary measure of cell 1,
-    owner words) after a step. Returns the observables the evidence records."""
-    step(burn_in)
-    energies, boundaries, centers = [], [], []
-    sites = dims[0] * dims[1]
-    for _ in range(measured // cadence):
-        step(cadence)
-        energy, boundary, owners = observe()
-        energies.append(energy / sites)
-        boundaries.append(float(boundary))
-        centers.append(_cell_center(owners, dims))
+def migration_summary(energies, boundaries, centers, dims=(64, 64), cadence=5, window=20):
+    """The observables run_realistic_replica (RealisticScaleRunner.jl:266-360) records for the
+    single-cell workload, from the per-observation energy per site, boundary measure and
+    cell centre."""
     disp = _wrapped(centers[0], centers[-1], dims)
     steps = [_wrapped(centers[i - 1], centers[i], dims) for i in range(1, len(centers))]
     distances = [float(np.hypot(*s)) for s in steps]
@@ -203,6 +193,23 @@ def migration_observables(step, observe, dims=(64, 64), burn_in=500, measured=20
             "mean_squared_displacement": float(np.mean([s[0] ** 2 + s[1] ** 2 for s in steps])),
             "speed": float(np.mean(distances)) / cadence,
             "persistence": 0.0 if sum(distances) == 0 else net / sum(distances)}
+
+
+def migration_observables(step, observe, dims=(64, 64), burn_in=500, measured=2000, cadence=5,
+                          window=20):
+    """run_realistic_replica (RealisticScaleRunner.jl:266-360) for the single-cell workload:
+    `step(n)` advances n MCS; `observe()` returns (global energy, boundary measure of cell 1,
+    owner words) after a step."""
+    step(burn_in)
+    energies, boundaries, centers = [], [], []
+    sites = dims[0] * dims[1]
+    for _ in range(measured // cadence):
+        step(cadence)
+        energy, boundary, owners = observe()
+        energies.append(energy / sites)
+        boundaries.append(float(boundary))
+        centers.append(cell_center(owners, dims))
+    return migration_summary(energies, boundaries, centers, dims, cadence, window)
 
 
 EXACT_MIGRATION_FIELDS = ("energy_per_mutable_site_terminal", "energy_per_mutable_site_mean",
