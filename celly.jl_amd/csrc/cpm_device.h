@@ -18,7 +18,9 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kCommitTiles = 4;    // tile segments per commit workgroup (claim check, evaluation)
 constexpr int kMaxRounds = 256;    // colour / round order capacity per replica
-constexpr int kSlotFields = 8;     // per-tile accounting fields
+constexpr int kSlotFields = 10;    // per-tile accounting fields (8-9: FP64 sum of the accepted
+                                   // copies' delta_h of the tile's pass)
+constexpr uint32_t kEnergyParts = 1024;  // partial sums of the global-energy reduction
 
 // Semantic RNG stream ids (lib/CorePotts/src/rng/semantic.jl:37-58).
 enum : uint32_t {

@@ -40,6 +40,8 @@ extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, ui
                                                   hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_copy_probe(const void *src, void *dst, size_t n4,
                                                      hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_energy(const cpm::RepDev *reps, int slot, double *buf,
+                                                 hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_report(const cpm::RepDev *reps, const uint32_t *slots,
                                                  uint32_t ntiles, int slot,
                                                  unsigned long long *out, hipStream_t stream);
@@ -163,6 +165,7 @@ struct Replica {
     bool report_cached = false;  // cached_report valid while mcs == cached_mcs
     uint64_t cached_mcs = 0;
     uint64_t cached_report[12] = {};
+    double cached_dh = 0.0;      // FP64 sum of the accepted copies' delta_h of that MCS (NaN: tiled)
     int group = -1;
 
     ~Replica() {
@@ -1678,7 +1681,7 @@ struct Engine {
         } else {
             hip_check(cpm_internal_launch_report(d_reps, g.d_slots, g.ntiles, slot, d_report, stream), "report");
             metrics[0]++;
-            hip_check(hipMemcpyAsync(h_report, d_report, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+            hip_check(hipMemcpyAsync(h_report, d_report, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                      stream), "copy report");
         }
         hip_check(hipStreamSynchronize(stream), "sync");
@@ -1723,8 +1726,37 @@ struct Engine {
         out[10] = s[5];
         out[11] = s[6];
         std::memcpy(r->cached_report, out, sizeof(r->cached_report));
+        double dh;
+        std::memcpy(&dh, &s[7], sizeof(dh));
+        r->cached_dh = dh;
         r->cached_mcs = mcs;
         r->report_cached = true;
+    }
+
+    // FP64 sum of the accepted copies' delta_h over the replica's last MCS (report counters).
+    double report_energy(int slot) {
+        uint64_t out[12];
+        report(slot, out);
+        Replica *r = reps[slot];
+        if (groups[r->group].tiled())
+            fail(CPM_E_UNSUPPORTED, "TiledCheckerboardCPM evaluates against tile-local views; no energy report");
+        return r->cached_dh;
+    }
+
+    // Global FP64 energy of the replica's current state; buf holds kEnergyParts + 1 doubles of
+    // device memory (partial sums, then the total). One launch pair, one copy, one sync.
+    double total_energy(int slot, double *buf) {
+        check_slot(slot);
+        hip_check(cpm_internal_launch_energy(d_reps, slot, buf, stream), "energy");
+        metrics[0] += 2;
+        hip_check(hipMemcpyAsync(h_report, buf + cpm::kEnergyParts, sizeof(double), hipMemcpyDeviceToHost, stream),
+                  "copy energy");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        metrics[3]++;
+        double e;
+        std::memcpy(&e, h_report, sizeof(e));
+        return e;
     }
 
     void upload_state(int slot, const uint8_t *tags, const uint32_t *ids, const int32_t *fv,
@@ -1895,6 +1927,10 @@ struct cpm_ctx {
     Engine engine;
     int slot = 0;
     uint64_t host_mcs = 0;
+    double *d_energy = nullptr;  // cpm_total_energy's partial sums, allocated by its first call
+    ~cpm_ctx() {
+        if (d_energy) (void)hipFree(d_energy);
+    }
 };
 
 struct cpm_batch {
@@ -2015,6 +2051,20 @@ int32_t cpm_step(cpm_ctx *ctx, uint64_t n) {
 int32_t cpm_report(cpm_ctx *ctx, uint64_t out[12]) {
     if (!ctx || !out) return CPM_E_INVALID;
     return guarded(&ctx->engine.err, [&] { ctx->engine.report(ctx->slot, out); });
+}
+
+int32_t cpm_report_energy(cpm_ctx *ctx, double *accepted_delta_h) {
+    if (!ctx || !accepted_delta_h) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] { *accepted_delta_h = ctx->engine.report_energy(ctx->slot); });
+}
+
+int32_t cpm_total_energy(cpm_ctx *ctx, double *energy) {
+    if (!ctx || !energy) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        if (!ctx->d_energy)
+            hip_check(hipMalloc(&ctx->d_energy, (cpm::kEnergyParts + 1) * sizeof(double)), "hipMalloc energy");
+        *energy = ctx->engine.total_energy(ctx->slot, ctx->d_energy);
+    });
 }
 
 int32_t cpm_sync(cpm_ctx *ctx) {

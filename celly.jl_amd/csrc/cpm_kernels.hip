@@ -1361,7 +1361,7 @@ __device__ __forceinline__ float copy_energy_from(const D &d, int32_t l, int32_t
                                                   const CellHot &cl, const CellHot &cg, float qpl, float qtl,
                                                   float qpg, float qtg, const int32_t (&sw)[kNb],
                                                   const int32_t (&cw)[kNb], const uint32_t (&ct)[kNb], More &&more,
-                                                  JF &&jf, JD &&jd, Staged *st) {
+                                                  JF &&jf, JD &&jd, Staged *st, float *dh_out = nullptr) {
     const int32_t vl0 = cl.vol + view.dvl, vg0 = cg.vol + view.dvg;
     const double vs_l = cl.vs, tv_l = cl.tv, vs_g = cg.vs, tv_g = cg.tv;
     const double bs_l = cl.bs, bs_g = cg.bs;
@@ -1472,6 +1472,7 @@ __device__ __forceinline__ float copy_energy_from(const D &d, int32_t l, int32_t
         }
     }
     const float dh = (float)acc;
+    if (dh_out) *dh_out = dh;
     // mechanical work folded in tuple order (scientific_inner_loop.jl:238-242;
     // scientific_components.jl:220-231, 646-657), then added to delta_h
     float mech = 0.0f;
@@ -1493,7 +1494,7 @@ __device__ __forceinline__ float copy_energy_from(const D &d, int32_t l, int32_t
 // against the trackers in the cell records plus `view`, reading the lattice in global memory.
 template <class D>
 __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t l, int32_t g, const View &view,
-                                             Staged *st) {
+                                             Staged *st, float *dh_out = nullptr) {
     const uint32_t idx0 = site - 1;
     const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
     const uint32_t zq = fast_div(idx0, div_xy);
@@ -1551,7 +1552,7 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
     return copy_energy_from(
         d, l, g, view, cl, cg, qpl, qtl, qpg, qtg, sw, cw, ct, more,
         [&](uint32_t i) __attribute__((always_inline)) { return d.Jf[i]; },
-        [&](uint32_t i) __attribute__((always_inline)) { return d.Jd[i]; }, st);
+        [&](uint32_t i) __attribute__((always_inline)) { return d.Jd[i]; }, st, dh_out);
 }
 
 // ConventionalMetropolis acceptance probability of an energy change at temperature T
@@ -1566,10 +1567,13 @@ __device__ __forceinline__ float metropolis_p(float T, float dd) {
 
 // evaluate_copy + ConventionalMetropolis + acceptance draw of one winning attempt against
 // the pass snapshot (checkerboard.jl:371-408).
+// *dh receives the energy change (delta_h, without the mechanical work).
 template <class D>
-__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub, Staged *st) {
+__device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub, Staged *st,
+                                       float *dh) {
     const View none{0, 0, 0, 0};
-    const float p = metropolis_p(d.temperature, copy_energy(d, e.site, e.losing, e.gaining, none, st));
+    const float total = copy_energy(d, e.site, e.losing, e.gaining, none, st, dh);
+    const float p = metropolis_p(d.temperature, total);
     const Words w = rng_words(d.key_lo, d.key_hi, kStreamAcceptance, mcs, sub, 0, kSiteEntity, e.site, 0, 0);
     return uniform_open01(w.w0) < p;
 }
@@ -1629,6 +1633,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     __shared__ uint32_t s_base[NS + 1];
     __shared__ uint32_t s_cbase[CT];
     __shared__ uint32_t s_cnt[CT][3];  // conflicts, rejections, acceptances
+    __shared__ double s_dh[CT];        // accepted copies' delta_h
     __shared__ v4i s_win[WCAP];
     __shared__ uint32_t s_nw;
     const uint32_t nwg = gridDim.x;
@@ -1666,6 +1671,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     const uint64_t live_tiles = __ballot(live && lane % kWaves == 0);  // wave 0: stretch 0 carries the flag
     if (tid < nt) s_cbase[tid] = a.tiles[t0 + tid].claim_base;
     if (tid < (uint32_t)CT * 3) s_cnt[tid / 3][tid % 3] = 0;
+    if (tid < (uint32_t)CT) s_dh[tid] = 0.0;
     __syncthreads();
     const uint32_t total = s_base[NS];
     const uint32_t Q = a.seg_cap / kWaves;
@@ -1731,8 +1737,12 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
                     const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
                     const Entry ent{(uint32_t)rec.x, 0u, rec.z, rec.w};
                     Staged st;
-                    const bool accept = evaluate_winner(d, ent, y.mcs, sub, &st);
-                    if (accept) commit_attempt(d, ent, st);
+                    float dh;
+                    const bool accept = evaluate_winner(d, ent, y.mcs, sub, &st, &dh);
+                    if (accept) {
+                        commit_attempt(d, ent, st);
+                        atomicAdd(&s_dh[kw], (double)dh);
+                    }
                     atomicAdd(&s_cnt[kw][accept ? 2 : 1], 1u);
                 }
                 if (one) break;
@@ -1747,6 +1757,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
         slot[4] = s_cnt[lane][0];
         slot[5] = s_cnt[lane][1];
         slot[6] = s_cnt[lane][2];
+        *(double *)(slot + 8) = s_dh[lane];
     }
 }
 
@@ -2270,21 +2281,110 @@ __global__ __launch_bounds__(kBlock) void report_kernel(const RepDev *reps, cons
                                                         uint32_t ntiles, int slot,
                                                         unsigned long long *out) {
     __shared__ unsigned long long s_acc[7][kBlock];
+    __shared__ double s_dh[kBlock];
     const RepDev &d = reps[slot];
     unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    double dh = 0.0;
     const uint32_t per_pass = d.tile_count;
     const uint32_t total = per_pass * (uint32_t)d.rounds;
     for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
         const uint32_t p = i / per_pass, t = d.tile_base + i % per_pass;
         const uint32_t *s = slots + ((size_t)p * ntiles + t) * kSlotFields;
         for (int k = 0; k < 7; ++k) acc[k] += s[k];
+        dh += *(const double *)(s + 8);
     }
     for (int k = 0; k < 7; ++k) s_acc[k][threadIdx.x] = acc[k];
+    s_dh[threadIdx.x] = dh;
     __syncthreads();
     if (threadIdx.x < 7) {
         unsigned long long t = 0;
         for (int j = 0; j < kBlock; ++j) t += s_acc[threadIdx.x][j];
         out[threadIdx.x] = t;
+    } else if (threadIdx.x == 7) {
+        double t = 0.0;  // fixed order: the result does not depend on scheduling
+        for (int j = 0; j < kBlock; ++j) t += s_dh[j];
+        out[7] = (unsigned long long)__double_as_longlong(t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Global energy in FP64 (scientific_components.jl:97-107, 315-340, 525-538): over active
+// cells lambda_V (V - V*)^2 + lambda_S (S - S*)^2 from the trackers, and over mutable sites
+// and realized contact directions w J[type(owner), type(neighbour)] where the owners differ,
+// a pair of mutable sites once (the direction not after its opposite). A fixed grid of
+// kEnergyParts workgroups writes one partial each; energy_sum_kernel adds them in a fixed
+// order, so the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void energy_partials_kernel(const RepDev *reps, int slot, double *partials) {
+    __shared__ double s_part[kBlock];
+    const RepDev &d = reps[slot];
+    bool has_vol = false, has_con = false, has_bnd = false;
+    for (int t = 0; t < d.n_energies; ++t) {
+        has_vol |= d.energy_order[t] == CPM_TERM_VOLUME;
+        has_con |= d.energy_order[t] == CPM_TERM_CONTACT;
+        has_bnd |= d.energy_order[t] == CPM_TERM_BOUNDARY;
+    }
+    const uint32_t first = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+    double acc = 0.0;
+    if (has_vol || has_bnd) {
+        for (uint32_t c = 1 + first; c <= d.capacity; c += stride) {
+            const CellRec &r = d.cells[c];
+            if (!r.active) continue;
+            if (has_vol) {
+                const double dv = (double)r.vol - r.tv;
+                acc += r.vs * (dv * dv);
+            }
+            if (has_bnd) {
+                const double db = (double)r.bnd - (double)r.tb;
+                acc += r.bs * (db * db);
+            }
+        }
+    }
+    if (has_con) {
+        const int nc = d.cont.n;
+        const uint32_t nt = (uint32_t)d.n_types;
+        for (uint32_t i = first; i < d.nsites; i += stride) {
+            if (d.mask && !d.mask[i]) continue;
+            const int x = (int)(i % (uint32_t)d.dims[0]);
+            const int y = (int)((i / (uint32_t)d.dims[0]) % (uint32_t)d.dims[1]);
+            const int z = (int)(i / ((uint32_t)d.dims[0] * (uint32_t)d.dims[1]));
+            const int32_t o = d.lattice[i];
+            const uint32_t ot = owner_type(d, o);
+            for (int k = 0; k < nc; ++k) {
+                const Realized nb = realize_position(d, x + d.cont.off[k][0], y + d.cont.off[k][1],
+                                                     z + d.cont.off[k][2]);
+                if (nb.word == 0 || nb.word == o) continue;
+                if (nb.site && nb.mut) {
+                    int opp = k;  // index of the opposite offset
+                    for (int e = 0; e < nc; ++e)
+                        if (d.cont.off[e][0] == -d.cont.off[k][0] && d.cont.off[e][1] == -d.cont.off[k][1] &&
+                            d.cont.off[e][2] == -d.cont.off[k][2])
+                            opp = e;
+                    if (k > opp) continue;
+                }
+                acc += d.cont.wd[k] * d.Jd[(ot - 1) * nt + (owner_type(d, nb.word) - 1)];
+            }
+        }
+    }
+    s_part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int j = 0; j < kBlock; ++j) t += s_part[j];
+        partials[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void energy_sum_kernel(const double *partials, uint32_t n, double *out) {
+    __shared__ double s_part[kBlock];
+    double t = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) t += partials[i];
+    s_part[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int j = 0; j < kBlock; ++j) sum += s_part[j];
+        out[0] = sum;
     }
 }
 
@@ -2495,6 +2595,19 @@ extern "C" hipError_t cpm_internal_launch_copy_probe(const void *src, void *dst,
     const unsigned blocks = (unsigned)(want < 8192 ? (want ? want : 1) : 8192);
     CPM_LAUNCH(cpm::copy_probe_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream,
                        (const cpm::v4i *)src, (cpm::v4i *)dst, n4);
+    return hipGetLastError();
+}
+
+// Global FP64 energy of replica `slot` into buf[kEnergyParts] (buf[0 .. kEnergyParts) hold the
+// partial sums, buf[kEnergyParts] the total).
+extern "C" hipError_t cpm_internal_launch_energy(const cpm::RepDev *reps, int slot, double *buf,
+                                                 hipStream_t stream) {
+    hipLaunchKernelGGL(cpm::energy_partials_kernel, dim3(cpm::kEnergyParts), dim3(cpm::kBlock), 0, stream, reps,
+                       slot, buf);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cpm::energy_sum_kernel, dim3(1), dim3(cpm::kBlock), 0, stream, (const double *)buf,
+                       cpm::kEnergyParts, buf + cpm::kEnergyParts);
     return hipGetLastError();
 }
 

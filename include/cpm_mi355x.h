@@ -189,6 +189,21 @@ int32_t cpm_step(cpm_ctx *ctx, uint64_t n);
 int32_t cpm_report(cpm_ctx *ctx, uint64_t out[12]);
 int32_t cpm_sync(cpm_ctx *ctx);
 
+/* FP64 sum of the accepted copies' delta_h (the Float32 energy changes the acceptance used)
+ * over the last completed MCS, from the same counters as cpm_report (one launch + one
+ * synchronization when not cached). Each accepted copy changes the global energy by its
+ * delta_h (lib/CorePotts/test/test_scientific_hamiltonians.jl:47-80), so summed over MCS this
+ * tracks the energy change for drift checks against cpm_total_energy. CPM_E_UNSUPPORTED for
+ * TiledCheckerboardCPM (tile-local views). (No single counterpart in the reference: the
+ * per-MCS energy bookkeeping of its evaluation, scientific_inner_loop.jl:257-269.) */
+int32_t cpm_report_energy(cpm_ctx *ctx, double *accepted_delta_h);
+/* Global energy of the current state in FP64: the model's terms summed as the reference's
+ * global_energy methods define them (scientific_components.jl:97-107 volume, 315-340 contact,
+ * 525-538 boundary; volume and boundary measures from the trackers), as
+ * RealisticScaleRunner's _global_energy adds them. Synchronizes. The first call allocates
+ * the context's partial-sum buffer (8 KiB), kept until cpm_destroy. */
+int32_t cpm_total_energy(cpm_ctx *ctx, double *energy);
+
 /* Colour (checkerboard), round (lottery) or tile-colour (tiled) count, and the order used
  * by the last MCS. order may be NULL; it needs round_count entries. Synchronizes. */
 int32_t cpm_schedule(cpm_ctx *ctx, int32_t *round_count, uint32_t *order);
