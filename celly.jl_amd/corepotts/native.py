@@ -73,7 +73,7 @@ EXPORTS = (
     "cpm_replica_bytes", "cpm_batch_add", "cpm_batch_remove", "cpm_batch_upload_state",
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
     "cpm_batch_capacity", "cpm_batch_report", "cpm_batch_sync", "cpm_batch_set_graph_chunk",
-    "cpm_batch_bytes", "cpm_report_energy", "cpm_total_energy",
+    "cpm_batch_bytes", "cpm_report_energy", "cpm_total_energy", "cpm_total_energy_noalloc",
 )
 
 _lib = None
@@ -106,6 +106,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_sync.argtypes = [P]
     lib.cpm_report_energy.argtypes = [P, C.POINTER(C.c_double)]
     lib.cpm_total_energy.argtypes = [P, C.POINTER(C.c_double)]
+    lib.cpm_total_energy_noalloc.argtypes = [P, C.POINTER(C.c_double)]
     lib.cpm_schedule.argtypes = [P, C.POINTER(C.c_int32), P]
     lib.cpm_site_colors.argtypes = [P, P]
     lib.cpm_metrics.argtypes = [P, C.POINTER(C.c_uint64)]
@@ -404,6 +405,12 @@ class ScientificPottsIntegrator:
         """Global energy of the current state in FP64 (cpm_total_energy)."""
         out = C.c_double()
         self._check(self.lib.cpm_total_energy(self._ctx, C.byref(out)))
+        return float(out.value)
+
+    def total_energy_noalloc(self) -> float:
+        """cpm_total_energy through the library's static scratch (no device allocation)."""
+        out = C.c_double()
+        self._check(self.lib.cpm_total_energy_noalloc(self._ctx, C.byref(out)))
         return float(out.value)
 
     def schedule(self):

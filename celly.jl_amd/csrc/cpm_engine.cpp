@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -1923,9 +1924,38 @@ int32_t guarded(std::string *err, F &&fn) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Static scratch of cpm_total_energy_noalloc: the partial sums and the total of one global
+// energy reduction (kEnergyParts + 1 doubles = 8 KiB on every device the library runs on,
+// held until the process ends). Host code never passes `scratch` itself to a kernel (on the
+// host it only names the symbol): cpm_create resolves its device address once per device
+// with hipGetSymbolAddress, with that device current, and every use goes through that address.
+// Every call on one device shares the array; each reduction overwrites all of it, so nothing
+// needs clearing, and calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[cpm::kEnergyParts + 1];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static double *g_scratch_address[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, resolved on the first call per device.
+static double *resolve_scratch(int *device) {
+    hip_check(hipGetDevice(device), "hipGetDevice");
+    if (*device < 0 || *device >= kMaxScratchDevices) fail(CPM_E_UNSUPPORTED, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[*device]) {
+        void *p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_address[*device] = static_cast<double *>(p);
+    }
+    return g_scratch_address[*device];
+}
+
 struct cpm_ctx {
     Engine engine;
     int slot = 0;
+    int device = -1;             // device current at cpm_create
+    double *scratch_ptr = nullptr;  // device address of `scratch` on that device
     uint64_t host_mcs = 0;
     double *d_energy = nullptr;  // cpm_total_energy's partial sums, allocated by its first call
     ~cpm_ctx() {
@@ -1953,6 +1983,7 @@ int32_t cpm_create(const cpm_model_desc *desc, cpm_ctx **out) {
     cpm_ctx *ctx = nullptr;
     const int32_t rc = guarded(&g_create_error, [&] {
         ctx = new cpm_ctx();
+        ctx->scratch_ptr = resolve_scratch(&ctx->device);
         ctx->engine.init(desc->algorithm);
         ctx->slot = ctx->engine.add(*desc);
         ctx->engine.rebuild_dirty();
@@ -2064,6 +2095,20 @@ int32_t cpm_total_energy(cpm_ctx *ctx, double *energy) {
         if (!ctx->d_energy)
             hip_check(hipMalloc(&ctx->d_energy, (cpm::kEnergyParts + 1) * sizeof(double)), "hipMalloc energy");
         *energy = ctx->engine.total_energy(ctx->slot, ctx->d_energy);
+    });
+}
+
+int32_t cpm_total_energy_noalloc(cpm_ctx *ctx, double *energy) {
+    if (!ctx || !energy) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        static_assert(sizeof(scratch) == (cpm::kEnergyParts + 1) * sizeof(double),
+                      "the reduction's fixed grid writes kEnergyParts partial sums and one total");
+        int current = -1;
+        hip_check(hipGetDevice(&current), "hipGetDevice");
+        if (current != ctx->device)
+            fail(CPM_E_STATE, "the context's device is not current (its scratch address belongs to its device)");
+        if (!ctx->scratch_ptr) fail(CPM_E_STATE, "scratch address was not resolved at cpm_create");
+        *energy = ctx->engine.total_energy(ctx->slot, ctx->scratch_ptr);
     });
 }
 

@@ -203,6 +203,15 @@ int32_t cpm_report_energy(cpm_ctx *ctx, double *accepted_delta_h);
  * RealisticScaleRunner's _global_energy adds them. Synchronizes. The first call allocates
  * the context's partial-sum buffer (8 KiB), kept until cpm_destroy. */
 int32_t cpm_total_energy(cpm_ctx *ctx, double *energy);
+/* cpm_total_energy without any allocation. No counterpart in the reference: this entry point
+ * is for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). Same kernels and result as cpm_total_energy; its
+ * partial sums live in the library's static device array of kEnergyParts + 1 = 1025 doubles
+ * (8 KiB per device, held until the process ends), whose address cpm_create resolved. The
+ * reduction's grid is fixed, so no lattice exceeds that maximum. Every call on one device
+ * shares the array: calls that overlap on one device (other threads, contexts or streams)
+ * race. CPM_E_STATE when the context's device is not the current device. Synchronizes. */
+int32_t cpm_total_energy_noalloc(cpm_ctx *ctx, double *energy);
 
 /* Colour (checkerboard), round (lottery) or tile-colour (tiled) count, and the order used
  * by the last MCS. order may be NULL; it needs round_count entries. Synchronizes. */
