@@ -118,6 +118,17 @@ def _mechanics_entry(component, compiled, algorithm, columns):
             noise_is_temperature, noise)
 
 
+def julia_float_promotion(*dtypes) -> np.dtype:
+    """float(promote_type(T...)) under Julia's rules, the arithmetic type of an energy term
+    (scientific_components.jl:101, 112, 529, 549): an integer column promotes with a float one
+    to that float type (Int64 with Float32 gives Float32, where numpy would give float64),
+    Float32 with Float64 gives Float64, and integers alone give Float64."""
+    floats = [np.dtype(d) for d in dtypes if np.issubdtype(np.dtype(d), np.floating)]
+    if not floats:
+        return np.dtype(np.float64)
+    return np.dtype(np.float64) if any(d.itemsize > 4 for d in floats) else np.dtype(np.float32)
+
+
 def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCartesianRelation,
                components, algorithm, seed: int = 0) -> ModelSpec:
     if not 0 <= int(seed) <= 0xFFFFFFFFFFFFFFFF:
@@ -152,7 +163,7 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
         if isinstance(component, QuadraticVolumeHamiltonian):
             order.append(TERM_VOLUME)
             tv, vs = props[component.target], props[component.strength]
-            volume_f64 = np.result_type(tv.dtype, vs.dtype) == np.float64
+            volume_f64 = julia_float_promotion(tv.dtype, vs.dtype) == np.float64
             columns["target_volume"] = component.target
             columns["volume_strength"] = component.strength
         elif isinstance(component, UnorderedContactHamiltonian):
@@ -170,7 +181,7 @@ def build_spec(compiled: CompiledScientificState, proposal_relation: StaticCarte
                 raise ValueError("boundary component does not match the compiled boundary tracker descriptor")
             order.append(TERM_BOUNDARY)
             tb, bs = props[component.target], props[component.strength]
-            boundary_f64 = np.result_type(tb.dtype, bs.dtype, component.relation.weight_dtype) == np.float64
+            boundary_f64 = julia_float_promotion(tb.dtype, bs.dtype, component.relation.weight_dtype) == np.float64
             columns["target_boundary"] = component.target
             columns["boundary_strength"] = component.strength
         else:

@@ -23,6 +23,10 @@ from .model import (BoundaryEdgeCount, BoundaryMeasureTracker, CartesianDomain,
                     compile_scientific_state, first_shell_relation, moore_relation,
                     required_properties)
 
+# Float32 spacing, as PottsToolkit lowers a Float32 model (src/authoring/lowering.jl:473-484):
+# relation weights are Float32, so no energy term is promoted to Float64 by them.
+F32_SPACING = {2: (np.float32(1.0),) * 2, 3: (np.float32(1.0),) * 3}
+
 J_TWO_TYPES = np.array([[0, 16, 16], [16, 2, 11], [16, 11, 14]], dtype=np.float32)
 
 
@@ -59,10 +63,12 @@ def plain_2d(L: int, corners, side: int = 10, *, temperature=np.float32(10.0), s
              algorithm="checkerboard", J=J_TWO_TYPES, ntypes: int = 2, contact="moore"):
     owners, types = square_cells_2d(L, side, corners, ntypes)
     ncells = len(corners)
-    domain = CartesianDomain((L, L))
-    proposal = first_shell_relation(ProposalRole, 2)
-    surface = first_shell_relation(SurfaceRole, 2)
-    contact_rel = moore_relation(ContactRole, 2) if contact == "moore" else first_shell_relation(ContactRole, 2)
+    sp = F32_SPACING[2]
+    domain = CartesianDomain((L, L), spacing=sp)
+    proposal = first_shell_relation(ProposalRole, 2, spacing=sp)
+    surface = first_shell_relation(SurfaceRole, 2, spacing=sp)
+    contact_rel = (moore_relation(ContactRole, 2, spacing=sp) if contact == "moore"
+                   else first_shell_relation(ContactRole, 2, spacing=sp))
     volume = QuadraticVolumeHamiltonian(number_type=np.float32)
     contact_h = UnorderedContactHamiltonian(np.asarray(J, dtype=np.float32),
                                             MediumTypeTable({1: 1}), contact_rel)
@@ -133,10 +139,11 @@ def cubes_3d(L: int = 256, side: int = 8, *, temperature=np.float32(20.0), seed:
     owners = cell.reshape(-1, order="F").astype(np.int64)
     ncells = n_per ** 3
     types = {k: 2 + (k - 1) % 3 for k in range(1, ncells + 1)}
-    domain = CartesianDomain((L, L, L))
-    proposal = first_shell_relation(ProposalRole, 3)
-    surface = first_shell_relation(SurfaceRole, 3)
-    contact_rel = moore_relation(ContactRole, 3)
+    sp = F32_SPACING[3]
+    domain = CartesianDomain((L, L, L), spacing=sp)
+    proposal = first_shell_relation(ProposalRole, 3, spacing=sp)
+    surface = first_shell_relation(SurfaceRole, 3, spacing=sp)
+    contact_rel = moore_relation(ContactRole, 3, spacing=sp)
     volume = QuadraticVolumeHamiltonian(number_type=np.float32)
     J = contact_matrix_3d() if J is None else J
     contact_h = UnorderedContactHamiltonian(np.asarray(J, np.float32), MediumTypeTable({1: 1}),
@@ -179,11 +186,12 @@ def block_cells(L: int, ndim: int, side: int, pitch: int, *, ntypes: int = 2, J=
     ncells = per ** ndim
     types = {k: 2 + (k - 1) % ntypes for k in range(1, ncells + 1)}
     dims = (L,) * ndim
-    domain = CartesianDomain(dims)
-    proposal = first_shell_relation(ProposalRole, ndim)
-    surface = first_shell_relation(SurfaceRole, ndim)
+    sp = F32_SPACING[ndim]
+    domain = CartesianDomain(dims, spacing=sp)
+    proposal = first_shell_relation(ProposalRole, ndim, spacing=sp)
+    surface = first_shell_relation(SurfaceRole, ndim, spacing=sp)
     contact_h = UnorderedContactHamiltonian(np.asarray(J, np.float32), MediumTypeTable({1: 1}),
-                                            moore_relation(ContactRole, ndim))
+                                            moore_relation(ContactRole, ndim, spacing=sp))
     volume = QuadraticVolumeHamiltonian(number_type=np.float32)
     energies = (volume, contact_h)
     state = LogicalPottsState(owners, max(ncells, 1), cell_types=types, medium_domains=(1,),

@@ -1,8 +1,10 @@
 // Device-side data structures and primitives of the MI355X CPM engine.
 //
-// Owner encoding on the device: one int32 word per site, +id for a cell, -id for a
-// medium; 0 never names an owner and marks an absent neighbour inside LDS tiles.
-// OwnerRef equality (lib/CorePotts/src/state/logical.jl:9-21) is word equality.
+// Owner encoding on the device: one int32 word per site, id | type << kTypeShift for a cell
+// (positive), -id for a medium; 0 never names an owner and marks an absent neighbour inside
+// LDS tiles. A cell's type is fixed while it owns sites, so OwnerRef equality
+// (lib/CorePotts/src/state/logical.jl:9-21) is still word equality, and the contact energy reads
+// a neighbour cell's type from its word instead of from a per-cell table.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,6 +15,26 @@
 #include "../../include/cpm_mi355x.h"
 
 namespace cpm {
+
+constexpr int kTypeShift = 26;                       // cell words: id in bits 0-25, type in 26-30
+constexpr uint32_t kIdMask = (1u << kTypeShift) - 1u;  // also the largest cell capacity
+__host__ __device__ __forceinline__ uint32_t word_cell(int32_t w) { return (uint32_t)w & kIdMask; }
+__host__ __device__ __forceinline__ uint32_t word_type(int32_t w) { return (uint32_t)w >> kTypeShift; }
+// per-cell slot of an owner word: the cell id, 0 (the spare slot) for a medium
+__host__ __device__ __forceinline__ uint32_t cell_slot(int32_t w) { return w > 0 ? word_cell(w) : 0u; }
+
+// Canonical direction order of the Moore shells (cartesian.jl:324-328: squared length, then
+// the offset tuple); their first 2 * dim directions are the first (von Neumann) shell. The
+// commit kernel's specialised evaluations use these neighbourhoods with compile-time offsets.
+constexpr int8_t kMoore2[8][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+constexpr int8_t kMoore3[26][3] = {
+    {-1, 0, 0},  {0, -1, 0},  {0, 0, -1}, {0, 0, 1},  {0, 1, 0},   {1, 0, 0},  {-1, -1, 0}, {-1, 0, -1}, {-1, 0, 1},
+    {-1, 1, 0},  {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1},   {1, -1, 0}, {1, 0, -1},  {1, 0, 1},   {1, 1, 0},
+    {-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1}, {1, -1, -1}, {1, -1, 1}, {1, 1, -1}, {1, 1, 1}};
+template <int DIM>
+__host__ __device__ constexpr int moore_offset(int k, int axis) {
+    return DIM == 2 ? (axis < 2 ? kMoore2[k][axis] : 0) : kMoore3[k][axis];
+}
 
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
@@ -120,6 +142,11 @@ struct RepDev {
     int32_t energy_order[3];
     int32_t volume_f64, contact_f64, boundary_f64;
     int32_t has_contact;
+    // Specialised evaluation of the commit kernel (0: the general one): 1-3 2D with the von
+    // Neumann surface relation and no / von Neumann / Moore contact relation, 4-6 the same in
+    // 3D (relations in canonical order, kMoore2 / kMoore3); periodic obstacle-free lattices,
+    // Float32 terms, no mechanics.
+    int32_t fast_eval;
     int32_t n_types;
     float Jf[CPM_MAX_TYPES * CPM_MAX_TYPES];
     double Jd[CPM_MAX_TYPES * CPM_MAX_TYPES];
@@ -150,7 +177,6 @@ struct RepDev {
     // size / (2 * mutable sites)) or entry 0 (lottery, 1 / (2 * rounds)), Float32
     float mech_half[kMaxRounds + 1];
     CellRec *cells;              // cap + 1, index = cell id (entry 0: spare read for media)
-    uint32_t *ctype;             // cap + 1 cell types (neighbour type lookups)
     int32_t *medium_vol;         // n_media
     // TiledCheckerboardCPM: halo-coloured tiles (1-based ids, first axis fastest), the
     // mutable sites of each tile (x fastest inside a tile), the tiles of each colour, the
@@ -215,6 +241,7 @@ struct PassArgs {
     uint32_t pass;              // ordinal inside the MCS (0-based)
     uint32_t parity;            // claim buffer used by this pass
     uint32_t total_claims;
+    uint32_t fast_eval;         // RepDev::fast_eval shared by every replica of the group (0: mixed)
 };
 
 // Device buffers are reached through pointers stored in RepDev; casting them to the

@@ -156,7 +156,6 @@ struct Replica {
     uint32_t *d_coff = nullptr;
     int32_t *d_mvol = nullptr;
     cpm::CellRec *d_cells = nullptr;
-    uint32_t *d_ctype = nullptr;
     uint32_t *d_tile_sites = nullptr, *d_tile_offsets = nullptr, *d_color_tiles = nullptr, *d_color_offsets = nullptr;
     int32_t *d_tile_dvol = nullptr, *d_tile_dbnd = nullptr, *d_tile_spill = nullptr;
     unsigned long long *d_tiled_counts = nullptr;
@@ -172,7 +171,7 @@ struct Replica {
     ~Replica() {
         dev_free(d_lattice); dev_free(d_mask); dev_free(d_color); dev_free(d_degree);
         dev_free(d_clist); dev_free(d_coff);
-        dev_free(d_cells); dev_free(d_ctype); dev_free(d_mvol);
+        dev_free(d_cells); dev_free(d_mvol);
         dev_free(d_tile_sites); dev_free(d_tile_offsets); dev_free(d_color_tiles); dev_free(d_color_offsets);
         dev_free(d_tile_dvol); dev_free(d_tile_dbnd); dev_free(d_tile_spill); dev_free(d_tiled_counts);
     }
@@ -504,6 +503,8 @@ void compile_replica(Replica &r) {
     if (!(d.temperature >= 0.0f) || !std::isfinite(d.temperature))
         fail(CPM_E_INVALID, "temperature must be finite and non-negative");
     if (d.capacity < 1 || d.capacity > 0x7ffffffeu) fail(CPM_E_INVALID, "capacity must be 1..2^31-2");
+    if (d.capacity > cpm::kIdMask)
+        fail(CPM_E_UNSUPPORTED, "cell capacity beyond 2^26 - 1 does not fit the owner word (id and type bits)");
     if (d.algorithm == CPM_ALG_TILED) {
         // tiled_checkerboard.jl:27-60 and the tiled admission of init_scientific
         for (int a = 0; a < 3; ++a)
@@ -1011,6 +1012,28 @@ void compile_replica(Replica &r) {
     v.contact_f64 = d.contact_f64 ? 1 : 0;
     v.boundary_f64 = d.boundary_f64 ? 1 : 0;
     v.has_contact = contact ? 1 : 0;
+    v.fast_eval = 0;
+    {
+        const int ns = d.surface.count, nc = contact ? d.contact.count : 0;
+        const bool f32 = !(has_term(d, CPM_TERM_VOLUME) && d.volume_f64) && !(contact && d.contact_f64) &&
+                         !(has_term(d, CPM_TERM_BOUNDARY) && d.boundary_f64);
+        // relations that are the canonical first shell / Moore shell in canonical order
+        auto canonical = [&](const cpm_relation_desc &rel) {
+            for (int k = 0; k < rel.count; ++k)
+                for (int a = 0; a < 3; ++a) {
+                    const int want = d.ndim == 2 ? cpm::moore_offset<2>(k, a) : cpm::moore_offset<3>(k, a);
+                    if (rel.offsets[k][a] != want) return false;
+                }
+            return true;
+        };
+        const int shell = 2 * d.ndim, moore = d.ndim == 2 ? 8 : 26;
+        const bool rels = ns == shell && canonical(d.surface) &&
+                          (nc == 0 || ((nc == shell || nc == moore) && canonical(d.contact)));
+        if (v.simple && d.n_mechanics == 0 && f32 && rels) {
+            if (d.ndim == 2 && ns == 4) v.fast_eval = nc == 0 ? 1 : nc == 4 ? 2 : nc == 8 ? 3 : 0;
+            if (d.ndim == 3 && ns == 6) v.fast_eval = nc == 0 ? 4 : nc == 6 ? 5 : nc == 26 ? 6 : 0;
+        }
+    }
     v.n_types = d.n_types;
     for (int i = 0; i < CPM_MAX_TYPES * CPM_MAX_TYPES; ++i) {
         v.Jd[i] = d.J[i];
@@ -1064,7 +1087,6 @@ void allocate_replica(Replica &r, hipStream_t stream) {
         hip_check(hipMemcpyAsync(r.d_degree, r.degrees.data(), (size_t)r.n, hipMemcpyHostToDevice, stream), "copy degrees");
     }
     r.d_cells = dev_alloc<cpm::CellRec>(cap1, r.bytes);
-    r.d_ctype = dev_alloc<uint32_t>(cap1, r.bytes);
     r.d_mvol = dev_alloc<int32_t>(CPM_MAX_MEDIA, r.bytes);
     if (r.desc.algorithm == CPM_ALG_TILED) {
         auto upload = [&](uint32_t *&dst, const std::vector<uint32_t> &src) {
@@ -1090,7 +1112,6 @@ void allocate_replica(Replica &r, hipStream_t stream) {
     }
     hip_check(hipMemsetAsync(r.d_lattice, 0, (size_t)r.n * 4, stream), "memset");
     hip_check(hipMemsetAsync(r.d_cells, 0, cap1 * sizeof(cpm::CellRec), stream), "memset");
-    hip_check(hipMemsetAsync(r.d_ctype, 0, cap1 * 4, stream), "memset");
     hip_check(hipMemsetAsync(r.d_mvol, 0, CPM_MAX_MEDIA * 4, stream), "memset");
     cpm::RepDev &v = r.dev;
     v.lattice = r.d_lattice;
@@ -1099,7 +1120,6 @@ void allocate_replica(Replica &r, hipStream_t stream) {
     v.colour_off = r.d_coff;
     v.site_degree = r.d_degree;
     v.cells = r.d_cells;
-    v.ctype = r.d_ctype;
     v.medium_vol = r.d_mvol;
     v.tile_sites = r.d_tile_sites;
     v.tile_offsets = r.d_tile_offsets;
@@ -1132,7 +1152,7 @@ uint64_t estimate_replica_bytes(const cpm_model_desc &d) {
     const uint64_t cap1 = (uint64_t)d.capacity + 1;
     const uint64_t own = n * 4 + (d.mutable_mask ? n : 0) + 2 * n /* colour lists or degree table */ +
                          tiles * (rounds + 1) * 4 /* colour ranges */ +
-                         cap1 * (sizeof(cpm::CellRec) + 4) + CPM_MAX_MEDIA * 4 +
+                         cap1 * sizeof(cpm::CellRec) + CPM_MAX_MEDIA * 4 +
                          (d.algorithm == CPM_ALG_TILED ? 3 * n * 4 + 16 + cap1 * 8 + 64 : 0) /* tile lists */;
     const uint64_t arenas = tiles * (tile_sites * sizeof(cpm::Entry) + 4 * cpm::kWaves + sizeof(cpm::TileInfo) +
                                      rounds * cpm::kSlotFields * 4) +
@@ -1150,6 +1170,7 @@ struct Group {
     std::vector<int> slots;
     uint32_t ntiles = 0, total_claims = 0, seg_cap = 0;
     int max_rounds = 0;
+    int fast_eval = 0;  // the commit evaluation every replica of the group takes (0: general)
     cpm::TileInfo *d_tiles = nullptr;
     unsigned long long *d_claims = nullptr;
     cpm::Entry *d_seg = nullptr;
@@ -1397,6 +1418,9 @@ struct Engine {
         }
         cache_reports(g);
         std::sort(g.slots.begin(), g.slots.end());
+        g.fast_eval = g.slots.empty() ? 0 : reps[g.slots[0]]->dev.fast_eval;
+        for (int s : g.slots)
+            if (reps[s]->dev.fast_eval != g.fast_eval) g.fast_eval = 0;
         uint32_t tiles = 0, claims = 0;
         int maxr = 1;
         std::vector<cpm::TileInfo> tile_info;
@@ -1472,6 +1496,7 @@ struct Engine {
         a.pass = pass;
         a.parity = parity;
         a.total_claims = g.total_claims;
+        a.fast_eval = (uint32_t)g.fast_eval;
         return a;
     }
 
@@ -1773,7 +1798,10 @@ struct Engine {
             if (id == 0 || id > 0x7fffffffu) fail(CPM_E_INVALID, "owner identity must be positive and fit Int32");
             if (tags[i] == 1) {
                 if (id > cap) fail(CPM_E_INVALID, "cell owner exceeds the capacity");
-                words[i] = (int32_t)id;
+                // the cell's word carries its type (cpm_device.h: id | type << kTypeShift)
+                if (ctype[id - 1] > 31u)
+                    fail(CPM_E_UNSUPPORTED, "a cell type beyond 31 does not fit the owner word");
+                words[i] = (int32_t)(id | (ctype[id - 1] << cpm::kTypeShift));
             } else if (tags[i] == 2) {
                 words[i] = -(int32_t)id;
             } else {
@@ -1783,7 +1811,6 @@ struct Engine {
         // per-cell arrays -> 64-byte cell records (entry 0 is the spare read for media)
         std::vector<cpm::CellRec> cells(cap1);
         std::memset(cells.data(), 0, cap1 * sizeof(cpm::CellRec));
-        std::vector<uint32_t> ct(cap1, 0);
         for (size_t c = 0; c < cap; ++c) {
             cpm::CellRec &q = cells[c + 1];
             q.vol = fv[c];
@@ -1794,7 +1821,6 @@ struct Engine {
             q.vs = vs ? vs[c] : 0.0;
             q.tb = tb ? tb[c] : 0;
             q.bs = bs ? bs[c] : 0.0;
-            ct[c + 1] = ctype[c];
         }
         if (has_term(r->desc, CPM_TERM_CONTACT))
             for (size_t c = 0; c < cap; ++c)
@@ -1805,7 +1831,6 @@ struct Engine {
         hip_check(hipStreamSynchronize(stream), "sync");
         hip_check(hipMemcpy(r->d_lattice, words.data(), n * 4, hipMemcpyHostToDevice), "upload lattice");
         hip_check(hipMemcpy(r->d_cells, cells.data(), cap1 * sizeof(cpm::CellRec), hipMemcpyHostToDevice), "upload");
-        hip_check(hipMemcpy(r->d_ctype, ct.data(), cap1 * 4, hipMemcpyHostToDevice), "upload");
         hip_check(hipMemcpy(r->d_mvol, media.data(), CPM_MAX_MEDIA * 4, hipMemcpyHostToDevice), "upload");
         metrics[2]++;
         r->base_mcs = read_mcs(slot);
@@ -1825,7 +1850,7 @@ struct Engine {
             for (size_t i = 0; i < n; ++i) {
                 const int32_t w = words[i];
                 if (tags) tags[i] = w > 0 ? 1 : 2;
-                if (ids) ids[i] = (uint32_t)(w > 0 ? w : -w);
+                if (ids) ids[i] = w > 0 ? cpm::word_cell(w) : (uint32_t)-w;
             }
         }
         if (fv || bm || active || ctype || tv || vs || tb || bs) {
@@ -2184,7 +2209,8 @@ int32_t cpm_tiled_layout(const cpm_model_desc *desc, int32_t grid[3], int32_t re
 
 int32_t cpm_kernel_config(const cpm_ctx *ctx, int32_t *config) {
     if (!ctx || !config) return CPM_E_INVALID;
-    *config = ctx->engine.reps[ctx->slot]->config;
+    const Replica *r = ctx->engine.reps[ctx->slot];
+    *config = r->config | (r->dev.fast_eval << 8);
     return CPM_OK;
 }
 

@@ -247,7 +247,7 @@ template <class D>
 __device__ __forceinline__ uint32_t owner_type(const D &d, int32_t w) {
     uint32_t t = 1;
     if (w > 0) {
-        t = gp(d.ctype)[w];
+        t = word_type(w);
     } else {
         for (int i = 0; i < d.n_table; ++i)
             if (d.table_word[i] == w) t = d.table_type[i];
@@ -589,8 +589,8 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
         const int32_t gaining = s_own[pos + direction_offset<KF>(s_dir[at], po, s_ploff)];
         // media are never claimed
         const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-        if (losing > 0) claim_local(s_ckey, s_cval, claims, losing, key);
-        if (gaining > 0) claim_local(s_ckey, s_cval, claims, gaining, key);
+        if (losing > 0) claim_local(s_ckey, s_cval, claims, (int32_t)word_cell(losing), key);
+        if (gaining > 0) claim_local(s_ckey, s_cval, claims, (int32_t)word_cell(gaining), key);
         seg[at] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
     }
 
@@ -762,8 +762,8 @@ __device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepD
             const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
                                             kSiteEntity, site, 0, 0).w0;
             const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-            if (losing > 0) claim_local(s_ckey, s_cval, claims, losing, key);
-            if (gaining > 0) claim_local(s_ckey, s_cval, claims, gaining, key);
+            if (losing > 0) claim_local(s_ckey, s_cval, claims, (int32_t)word_cell(losing), key);
+            if (gaining > 0) claim_local(s_ckey, s_cval, claims, (int32_t)word_cell(gaining), key);
             seg[nact + e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
         }
         nact += na;
@@ -986,8 +986,8 @@ __device__ __forceinline__ void row_wave3(const PassArgs &a, const CPM_CONST Rep
             const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
                                             kSiteEntity, site, 0, 0).w0;
             const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-            if (losing > 0) claim_local(s_ckey, s_cval, claims, losing, key);
-            if (gaining > 0) claim_local(s_ckey, s_cval, claims, gaining, key);
+            if (losing > 0) claim_local(s_ckey, s_cval, claims, (int32_t)word_cell(losing), key);
+            if (gaining > 0) claim_local(s_ckey, s_cval, claims, (int32_t)word_cell(gaining), key);
             seg[nact + e] = v4i{(int32_t)site, (int32_t)prio, losing, gaining};
         }
         nact += na;
@@ -1252,8 +1252,8 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
                     c23 += 1u << 16;
                     const uint32_t prio = s_tk[pos];
                     const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
-                    if (own > 0) atomicMax(&claims[d.claim_base + (uint32_t)own], key);
-                    if (gain > 0) atomicMax(&claims[d.claim_base + (uint32_t)gain], key);
+                    if (own > 0) atomicMax(&claims[d.claim_base + word_cell(own)], key);
+                    if (gain > 0) atomicMax(&claims[d.claim_base + word_cell(gain)], key);
                     v = v4i{(int32_t)site, (int32_t)prio, own, gain};
                     act = true;
                 }
@@ -1443,7 +1443,7 @@ __device__ __forceinline__ float copy_energy_from(const D &d, int32_t l, int32_t
 #pragma unroll
                 for (int u = 0; u < kTail; ++u) gw[u] = k0 + u < nc ? more(1, k0 + u) : 0;
 #pragma unroll
-                for (int u = 0; u < kTail; ++u) gt[u] = gp(d.ctype)[gw[u] > 0 ? gw[u] : 0];
+                for (int u = 0; u < kTail; ++u) gt[u] = gw[u] > 0 ? word_type(gw[u]) : 0u;
 #pragma unroll
                 for (int u = 0; u < kTail; ++u)
                     if (k0 + u < nc) contact_term(k0 + u, gw[u], gt[u]);
@@ -1504,7 +1504,7 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
     const int y = (int)yq;
     const int z = (int)zq;
     // slot 0 of every per-cell array is a spare, read in place of a medium owner
-    const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
+    const uint32_t li = cell_slot(l), gi = cell_slot(g);
     const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
     // mechanical state of both cells (volume_pressure, surface_tension)
     float qpl = 0.0f, qtl = 0.0f, qpg = 0.0f, qtg = 0.0f;
@@ -1544,7 +1544,7 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
     }
     uint32_t ct[kNb];
 #pragma unroll
-    for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
+    for (int u = 0; u < kNb; ++u) ct[u] = cw[u] > 0 ? word_type(cw[u]) : 0u;
     auto more = [&](int role, int k) __attribute__((always_inline)) {
         if (role == 0) return k < 32 ? nb(d.surf.off[k], d.surf_lin[k]) : neighbour_word(d, x, y, z, d.surf.off[k]);
         return k < 32 ? nb(d.cont.off[k], d.cont_lin[k]) : neighbour_word(d, x, y, z, d.cont.off[k]);
@@ -1553,6 +1553,108 @@ __device__ __forceinline__ float copy_energy(const D &d, uint32_t site, int32_t 
         d, l, g, view, cl, cg, qpl, qtl, qpg, qtg, sw, cw, ct, more,
         [&](uint32_t i) __attribute__((always_inline)) { return d.Jf[i]; },
         [&](uint32_t i) __attribute__((always_inline)) { return d.Jd[i]; }, st, dh_out);
+}
+
+// evaluate_copy of one actionable attempt on the common model shape (RepDev::fast_eval): a
+// periodic obstacle-free lattice, Float32 energy terms in tuple order, no mechanics, NS surface
+// directions that are the first of the NC contact directions (NC = 0: no contact term), all
+// within one site. The same arithmetic as copy_energy_from, with every loop fixed at compile
+// time: both cell records and all NW neighbour words are loaded at once, then all NC
+// neighbour types, so an evaluation waits on two memory levels whatever NC is.
+// sJ: the replica's Float32 contact matrix staged in LDS, or nullptr to read it from the
+// descriptor.
+typedef const __attribute__((address_space(3))) float LdsFloat;
+template <int DIM, int NS, int NC, class D>
+__device__ __forceinline__ float fast_copy_energy(const D &d, uint32_t site, int32_t l, int32_t g, Staged *st,
+                                                  LdsFloat *sJ) {
+    constexpr int NW = NC > 0 ? NC : NS;
+    const uint32_t idx0 = site - 1;
+    const FastDiv div_xy{d.div_xy.d, d.div_xy.m}, div_x{d.div_x.d, d.div_x.m};
+    const uint32_t zq = DIM == 3 ? fast_div(idx0, div_xy) : 0u;
+    const uint32_t rem = idx0 - zq * d.div_xy.d;
+    const uint32_t yq = fast_div(rem, div_x);
+    const int x = (int)(rem - yq * d.div_x.d), y = (int)yq, z = (int)zq;
+    const uint32_t li = cell_slot(l), gi = cell_slot(g);
+    const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
+    const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
+    const bool inner = x >= 1 && x < X - 1 && y >= 1 && y < Y - 1 && (DIM == 2 || (z >= 1 && z < Z - 1));
+    const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+    const int32_t sy = X, sz = X * Y;
+    int32_t w[NW];
+    if (inner) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+            w[k] = lat[(int32_t)idx0 + moore_offset<DIM>(k, 0) + sy * moore_offset<DIM>(k, 1) +
+                       sz * moore_offset<DIM>(k, 2)];
+    } else {
+        // periodic wrap of each axis (the relations reach one site)
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            int nx = x + moore_offset<DIM>(k, 0), ny = y + moore_offset<DIM>(k, 1), nz = z + moore_offset<DIM>(k, 2);
+            nx = nx < 0 ? X - 1 : nx == X ? 0 : nx;
+            ny = ny < 0 ? Y - 1 : ny == Y ? 0 : ny;
+            nz = nz < 0 ? Z - 1 : nz == Z ? 0 : nz;
+            w[k] = lat[(uint32_t)nx + (uint32_t)sy * (uint32_t)ny + (uint32_t)sz * (uint32_t)nz];
+        }
+    }
+    uint32_t ct[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) ct[k] = w[k] > 0 ? word_type(w[k]) : 0u;
+
+    const int32_t vl = l > 0 ? cl.vol : 0, vg = g > 0 ? cg.vol : 0;
+    // boundary tracker deltas (BoundaryEdgeCount, scientific_components.jl:466-481)
+    int32_t dl = 0, dg = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        dl += (w[k] == l) ? 1 : -1;
+        dg += (w[k] == g) ? -1 : 1;
+    }
+    st->dl = dl;
+    st->dg = dg;
+    st->vl = vl;
+    st->vg = vg;
+    st->bl = cl.bnd;
+    st->bg = cg.bnd;
+    float acc = 0.0f;
+    for (int t = 0; t < d.n_energies; ++t) {
+        const int term = d.energy_order[t];
+        float v;
+        if (term == CPM_TERM_VOLUME) {
+            float delta = 0.0f;
+            if (l > 0) delta = __fadd_rn(delta, vol_change_f32((float)cl.vs, (float)cl.tv, vl, -1));
+            if (g > 0) delta = __fadd_rn(delta, vol_change_f32((float)cg.vs, (float)cg.tv, vg, 1));
+            v = delta;
+        } else if (term == CPM_TERM_CONTACT) {
+            // scientific_components.jl:289-308, directions in canonical order
+            const uint32_t nt = (uint32_t)d.n_types;
+            const uint32_t lt = l > 0 ? min(max(cl.ctype, 1u), nt) : owner_type(d, l);
+            const uint32_t gt = g > 0 ? min(max(cg.ctype, 1u), nt) : owner_type(d, g);
+            float df = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const int32_t o = w[k];
+                uint32_t wt;
+                if (o == l) wt = lt;
+                else if (o == g) wt = gt;
+                else if (o > 0) wt = min(max(ct[k], 1u), nt);
+                else wt = owner_type(d, o);
+                const float wk = d.cont.wf[k];
+                const uint32_t il = (lt - 1) * nt + (wt - 1), ig = (gt - 1) * nt + (wt - 1);
+                const float jl = sJ ? sJ[il] : d.Jf[il], jg = sJ ? sJ[ig] : d.Jf[ig];
+                if (l != o) df = __fsub_rn(df, __fmul_rn(wk, jl));
+                if (g != o) df = __fadd_rn(df, __fmul_rn(wk, jg));
+            }
+            v = df;
+        } else {
+            // scientific_trackers.jl:144-172
+            float r = 0.0f;
+            if (l > 0) r = __fadd_rn(r, bnd_change_f32((float)cl.bs, (float)cl.tb, (float)cl.bnd, vl, (float)dl, -1));
+            if (g > 0) r = __fadd_rn(r, bnd_change_f32((float)cg.bs, (float)cg.tb, (float)cg.bnd, vg, (float)dg, 1));
+            v = r;
+        }
+        acc = __fadd_rn(acc, v);
+    }
+    return acc;
 }
 
 // ConventionalMetropolis acceptance probability of an energy change at temperature T
@@ -1568,11 +1670,23 @@ __device__ __forceinline__ float metropolis_p(float T, float dd) {
 // evaluate_copy + ConventionalMetropolis + acceptance draw of one winning attempt against
 // the pass snapshot (checkerboard.jl:371-408).
 // *dh receives the energy change (delta_h, without the mechanical work).
-template <class D>
+// FAST: the group's RepDev::fast_eval (every replica of the launch shares it), 0 for the
+// general evaluation. Each kind is its own kernel instantiation, so the registers of the
+// largest evaluation do not limit the occupancy of the others.
+template <int FAST, class D>
 __device__ inline bool evaluate_winner(const D &d, const Entry &e, uint64_t mcs, uint32_t sub, Staged *st,
-                                       float *dh) {
-    const View none{0, 0, 0, 0};
-    const float total = copy_energy(d, e.site, e.losing, e.gaining, none, st, dh);
+                                       float *dh, LdsFloat *sJ) {
+    float total;
+    if constexpr (FAST == 0) {
+        const View none{0, 0, 0, 0};
+        total = copy_energy(d, e.site, e.losing, e.gaining, none, st, dh);
+    } else {
+        constexpr int DIM = FAST <= 3 ? 2 : 3;
+        constexpr int NS = 2 * DIM;
+        constexpr int NC = FAST == 1 || FAST == 4 ? 0 : FAST == 2 ? 4 : FAST == 5 ? 6 : FAST == 3 ? 8 : 26;
+        total = fast_copy_energy<DIM, NS, NC>(d, e.site, e.losing, e.gaining, st, sJ);
+        *dh = total;
+    }
     const float p = metropolis_p(d.temperature, total);
     const Words w = rng_words(d.key_lo, d.key_hi, kStreamAcceptance, mcs, sub, 0, kSiteEntity, e.site, 0, 0);
     return uniform_open01(w.w0) < p;
@@ -1586,26 +1700,28 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &
     gp(d.lattice)[e.site - 1] = g;
     CPM_GLOBAL CellRec *cells = gp(d.cells);
     if (l > 0) {
+        CPM_GLOBAL CellRec &c = cells[word_cell(l)];
         const int32_t nv = st.vl - 1;
-        cells[l].vol = nv;
-        cells[l].bnd = st.bl + st.dl;
+        c.vol = nv;
+        c.bnd = st.bl + st.dl;
         if (nv == 0) {
-            cells[l].active = 0;
-            cells[l].ctype = 0;
-            gp(d.ctype)[l] = 0;
-            cells[l].tv = d.retire_tv;
-            cells[l].vs = d.retire_vs;
-            cells[l].tb = d.retire_tb;
-            cells[l].bs = d.retire_bs;
-            cells[l].qp = 0.0f;
-            cells[l].qt = 0.0f;
+            // retirement; the cell owns no site, so no owner word holds its type any more
+            c.active = 0;
+            c.ctype = 0;
+            c.tv = d.retire_tv;
+            c.vs = d.retire_vs;
+            c.tb = d.retire_tb;
+            c.bs = d.retire_bs;
+            c.qp = 0.0f;
+            c.qt = 0.0f;
         }
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, l)], -1);
     }
     if (g > 0) {
-        cells[g].vol = st.vg + 1;
-        cells[g].bnd = st.bg + st.dg;
+        CPM_GLOBAL CellRec &c = cells[word_cell(g)];
+        c.vol = st.vg + 1;
+        c.bnd = st.bg + st.dg;
     } else {
         atomicAdd(&d.medium_vol[medium_slot(d, g)], 1);
     }
@@ -1624,7 +1740,7 @@ __device__ inline void commit_attempt(const D &d, const Entry &e, const Staged &
 // otherwise one round per tile. The replica index is uniform in every round, so its
 // descriptor is read through the scalar cache.
 constexpr int kCheckBatch = 4;
-template <int NWV, int CT>
+template <int NWV, int CT, int FAST>
 __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     constexpr int NT = 64 * NWV;
     constexpr int NS = CT * kWaves;
@@ -1634,6 +1750,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     __shared__ uint32_t s_cbase[CT];
     __shared__ uint32_t s_cnt[CT][3];  // conflicts, rejections, acceptances
     __shared__ double s_dh[CT];        // accepted copies' delta_h
+    __shared__ float s_J[FAST ? CPM_MAX_TYPES * CPM_MAX_TYPES : 1];  // contact matrix of a one-replica region
     __shared__ v4i s_win[WCAP];
     __shared__ uint32_t s_nw;
     const uint32_t nwg = gridDim.x;
@@ -1679,6 +1796,13 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
     const CPM_GLOBAL v4i *region = (const CPM_GLOBAL v4i *)(a.seg + (size_t)t0 * a.seg_cap);
     const uint32_t r_first = a.tiles[t0].rep;
     const bool one = a.tiles[t0 + nt - 1].rep == r_first;
+    if (FAST && one) {
+        // staged while the claim check runs; read after its barrier
+        const CPM_CONST RepDev &d0 = ((const CPM_CONST RepDev *)a.reps)[r_first];
+        const uint32_t nj = (uint32_t)(d0.n_types * d0.n_types);
+        for (uint32_t i = tid; i < nj; i += NT) s_J[i] = d0.Jf[i];
+    }
+    LdsFloat *sJ = FAST && one ? (LdsFloat *)s_J : nullptr;
     for (uint32_t blk = 0; blk < total; blk += WCAP) {
         // entries of the batch, then their claims
         v4i v[kCheckBatch];
@@ -1699,8 +1823,8 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
 #pragma unroll
         for (int j = 0; j < kCheckBatch; ++j) {
             const uint32_t cb = s_cbase[kt[j]];
-            cl[j] = claims[cb + (uint32_t)max(v[j].z, 0)];
-            cg[j] = claims[cb + (uint32_t)max(v[j].w, 0)];
+            cl[j] = claims[cb + cell_slot(v[j].z)];
+            cg[j] = claims[cb + cell_slot(v[j].w)];
         }
 #pragma unroll
         for (int j = 0; j < kCheckBatch; ++j) {
@@ -1738,7 +1862,7 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
                     const Entry ent{(uint32_t)rec.x, 0u, rec.z, rec.w};
                     Staged st;
                     float dh;
-                    const bool accept = evaluate_winner(d, ent, y.mcs, sub, &st, &dh);
+                    const bool accept = evaluate_winner<FAST>(d, ent, y.mcs, sub, &st, &dh, sJ);
                     if (accept) {
                         commit_attempt(d, ent, st);
                         atomicAdd(&s_dh[kw], (double)dh);
@@ -1950,8 +2074,8 @@ struct TileList {
             const int32_t c = j < kListLds ? cell[j * kTiledBlock] : spill[2 * (j - kListLds)];
             const int32_t q = change((int)j);
             const int32_t dv = tiled_volume_part(q), db = tiled_boundary_part(q);
-            if (dv != 0) atomicAdd(&d.tile_dvol[c], dv);
-            if (db != 0) atomicAdd(&d.tile_dbnd[c], db);
+            if (dv != 0) atomicAdd(&d.tile_dvol[word_cell(c)], dv);
+            if (db != 0) atomicAdd(&d.tile_dbnd[word_cell(c)], db);
         }
     }
 };
@@ -2174,7 +2298,7 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev 
         if (__ballot(pending) == 0) break;
         if (pending) {
             ++v[0];
-            const uint32_t li = l > 0 ? (uint32_t)l : 0u, gi = g > 0 ? (uint32_t)g : 0u;
+            const uint32_t li = cell_slot(l), gi = cell_slot(g);
             const CellHot cl = load_cell(gp(d.cells), li), cg = load_cell(gp(d.cells), gi);
             int32_t sw[kNb], cw[kNb];
             uint32_t ct[kNb];
@@ -2183,7 +2307,7 @@ __global__ __launch_bounds__(kTiledBlock) void tiled_staged_kernel(const RepDev 
 #pragma unroll
             for (int u = 0; u < kNb; ++u) cw[u] = u < nc ? s_w[(pr + delta(d.cont.off[u])) * kTiledBlock] : 0;
 #pragma unroll
-            for (int u = 0; u < kNb; ++u) ct[u] = gp(d.ctype)[cw[u] > 0 ? cw[u] : 0];
+            for (int u = 0; u < kNb; ++u) ct[u] = cw[u] > 0 ? word_type(cw[u]) : 0u;
             auto more = [&](int role, int q) __attribute__((always_inline)) {
                 return s_w[(pr + delta(role == 0 ? d.surf.off[q] : d.cont.off[q])) * kTiledBlock];
             };
@@ -2231,7 +2355,6 @@ __global__ __launch_bounds__(kBlock) void tiled_reconcile_kernel(const RepDev *r
     if (vol == 0 && c->active) {
         c->active = 0;
         c->ctype = 0;
-        gp(d.ctype)[cell] = 0;
         c->tv = d.retire_tv;
         c->vs = d.retire_vs;
         c->tb = d.retire_tb;
@@ -2421,15 +2544,30 @@ static hipError_t launch_propose_t(int algorithm, int generic, int plain, const 
 }
 
 constexpr int kCommitWaves = 2;  // claim check on both waves, winners evaluated in full chunks
-static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
-    if (a.seg_cap >= 4096) {
-        const uint32_t wgs = (a.ntiles + 1) / 2;
-        CPM_LAUNCH((commit_kernel<kCommitWaves, 2>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
-    } else {
-        const uint32_t wgs = (a.ntiles + kCommitTiles - 1) / kCommitTiles;
-        CPM_LAUNCH((commit_kernel<kCommitWaves, kCommitTiles>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream,
-                           a);
+template <int CT>
+static void launch_commit_ct(const PassArgs &a, hipStream_t stream) {
+    const uint32_t wgs = (a.ntiles + CT - 1) / CT;
+    switch (a.fast_eval) {
+#define CPM_COMMIT_CASE(F)                                                                              \
+    case F:                                                                                            \
+        CPM_LAUNCH((commit_kernel<kCommitWaves, CT, F>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a); \
+        break;
+        CPM_COMMIT_CASE(1)
+        CPM_COMMIT_CASE(2)
+        CPM_COMMIT_CASE(3)
+        CPM_COMMIT_CASE(4)
+        CPM_COMMIT_CASE(5)
+        CPM_COMMIT_CASE(6)
+#undef CPM_COMMIT_CASE
+    default:
+        CPM_LAUNCH((commit_kernel<kCommitWaves, CT, 0>), dim3(wgs), dim3(64 * kCommitWaves), 0, stream, a);
     }
+}
+
+static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
+    if (a.seg_cap >= 4096) launch_commit_ct<2>(a, stream);
+    else if (a.fast_eval == 6) launch_commit_ct<8>(a, stream);  // 26 neighbours: two waves per SIMD fit
+    else launch_commit_ct<kCommitTiles>(a, stream);
     return hipGetLastError();
 }
 

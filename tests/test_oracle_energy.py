@@ -265,3 +265,21 @@ def test_oracle_total_energy_matches_restatement(seed, geometry):
             fx = random_fixture(seed, dims, periodic, T, ("volume", "contact", "boundary"), contact_kind)
             sim = fx.oracle()
             assert sim.total_energy() == pytest.approx(fx.energy(fx.owners), rel=1e-12)
+
+
+def test_energy_term_types_follow_julia_promotion():
+    """The arithmetic type of a term is float(promote_type(...)) of its columns and relation
+    weights under Julia's rules (scientific_components.jl:101, 529): the Float32 fixture of
+    test_scientific_hamiltonians.jl keeps every term Float32 (`global_energy(...) isa T`), even
+    though target_boundary is Int64; Float64 relation weights promote the boundary term."""
+    from corepotts.spec import julia_float_promotion
+    assert julia_float_promotion(np.int64, np.float32, np.float32) == np.float32
+    assert julia_float_promotion(np.int64, np.float32, np.float64) == np.float64
+    assert julia_float_promotion(np.float32, np.float32) == np.float32
+    assert julia_float_promotion(np.float32, np.float64) == np.float64
+    assert julia_float_promotion(np.int64) == np.float64
+    for T, f64 in ((np.float32, False), (np.float64, True)):
+        for dims in ((4, 4), (4, 4, 4)):
+            fx = reference_fixture(dims, T, ("volume", "contact", "boundary"))
+            spec = fx.oracle().spec
+            assert (spec.volume_f64, spec.contact_f64, spec.boundary_f64) == (f64, f64, f64)
