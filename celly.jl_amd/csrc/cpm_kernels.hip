@@ -704,61 +704,90 @@ __device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand,
     }
 }
 
-// The wave's part of one row-kernel tile: every strip's loads are issued first, then each
-// strip runs its direction draws (ProposalDirectionStream) over its compacted boundary
-// sites and its priority draws and claims over its compacted actionable attempts.
+// LDS of one row-kernel tile: each wave's boundary sites of the current strip (site, owner,
+// the four von Neumann neighbour words), then the workgroup's actionable attempts of that
+// strip, pooled over the four waves.
+struct RowLds {
+    uint32_t site[kWaves][2 * 64];
+    int32_t own[kWaves][2 * 64];
+    int32_t nb[kWaves][4][2 * 64];
+    uint32_t cnt[kWaves];
+    uint32_t act_site[kWaves * 2 * 64];
+    int32_t act_own[kWaves * 2 * 64];
+    int32_t act_gain[kWaves * 2 * 64];
+    uint32_t nact[2];  // double-buffered by strip parity
+};
+
+// One row-kernel tile. Every wave issues all its strips' loads first. Then strip by strip:
+// each wave compacts its boundary sites (interior sites are same-owner no-ops without a draw);
+// after a barrier the workgroup draws the directions (ProposalDirectionStream) over the four
+// waves' pooled boundary sites in full 64-lane chunks and compacts the actionable attempts;
+// after a second barrier it draws their priorities (CheckerboardPriorityStream), claims both
+// cells in the LDS table and writes the tile's segment (stretch 0). Pooling over the waves
+// fills the Philox chunks: a strip holds about 40 boundary sites and 13 actionable attempts
+// per wave, so four waves need 3 + 1 wave-wide Philox calls instead of 4 + 4.
 template <int RX, int TX, int TY>
-__device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti, uint32_t tile,
-                                         uint64_t mcs, int ry, int wave, uint32_t *s_site, int32_t *s_own,
-                                         int32_t (*s_nb)[2 * 64], uint32_t *s_ckey, unsigned long long *s_cval,
-                                         uint32_t &nsites, uint32_t &nsame, uint32_t &nact) {
+__device__ __forceinline__ void row_tile(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti,
+                                         uint32_t tile, uint64_t mcs, int ry, int wave, RowLds &L, uint32_t *s_ckey,
+                                         unsigned long long *s_cval, uint32_t &nsites, uint32_t &nsame,
+                                         uint32_t &nact) {
     using R = RowGeom<TX, TY>;
     const CPM_GLOBAL int32_t *lat = gp(d.lattice);
     const int X = d.dims[0], Y = d.dims[1];
     const int lane = (int)__lane_id();
+    const uint32_t tid = threadIdx.x;
     RowStrip st[R::ITERS];
 #pragma unroll
     for (int it = 0; it < R::ITERS; ++it) st[it] = row_load<RX, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, it);
-    __syncthreads();  // the claim table is clear (the strips' loads are in flight)
+    __syncthreads();  // the claim table and the first strip's counter are clear (loads in flight)
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
-    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap) + wave * R::Q;
+    CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
 #pragma unroll
     for (int it = 0; it < R::ITERS; ++it) {
         uint32_t ncand = 0;
-        row_collect<RX>(st[it], ncand, nsites, nsame, s_site, s_own, s_nb);
-        // direction draws; actionable attempts compacted in place (entry e is read before
-        // any write at or below it)
-        uint32_t na = 0;
-        for (uint32_t e0 = 0; e0 < ncand; e0 += 64) {
-            const uint32_t e = e0 + (uint32_t)lane;
+        row_collect<RX>(st[it], ncand, nsites, nsame, L.site[wave], L.own[wave], L.nb[wave]);
+        if (lane == 0) L.cnt[wave] = ncand;
+        __syncthreads();
+        // the next strip's counter: its last readers (the previous strip) passed the barrier above
+        if (tid == 0) L.nact[(it + 1) & 1] = 0u;
+        const uint32_t p1 = L.cnt[0], p2 = p1 + L.cnt[1], p3 = p2 + L.cnt[2], total = p3 + L.cnt[3];
+        // direction draws over the pooled boundary sites; four directions: bounded_uint(4)
+        // never rejects ((2^32 - 4) mod 4 == 0)
+        for (uint32_t i0 = (uint32_t)wave * 64u; i0 < total; i0 += kBlock) {
+            const uint32_t i = i0 + (uint32_t)lane;
             bool act = false;
             uint32_t site = 0;
             int32_t own = 0, gain = 0;
-            if (e < ncand) {
-                site = s_site[e];
-                own = s_own[e];
-                // four directions: bounded_uint(4) never rejects ((2^32 - 4) mod 4 == 0)
-                const uint32_t dir = rng_words(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity,
-                                               site, 0, 0).w0 & 3u;
-                gain = s_nb[dir][e];
+            if (i < total) {
+                const uint32_t sg = (uint32_t)(i >= p1) + (uint32_t)(i >= p2) + (uint32_t)(i >= p3);
+                const uint32_t e = i - (sg == 0 ? 0u : sg == 1 ? p1 : sg == 2 ? p2 : p3);
+                site = L.site[sg][e];
+                own = L.own[sg][e];
+                const uint32_t dir = rng_words(d.key_lo, d.key_hi, kStreamDirection, mcs, 0, 0, kSiteEntity, site,
+                                               0, 0).w0 & 3u;
+                gain = L.nb[sg][dir][e];
                 act = gain != own;
             }
-            const uint64_t valid = __ballot(e < ncand);
+            const uint64_t valid = __ballot(i < total);
             const uint64_t m = __ballot(act);
             nsame += (uint32_t)__popcll(valid & ~m);
+            uint32_t base = 0;
+            if (lane == 0 && m) base = atomicAdd(&L.nact[it & 1], (uint32_t)__popcll(m));
+            base = __shfl(base, 0);
             if (act) {
-                const uint32_t at = na + __popcll(m & __lanemask_lt());
-                s_site[at] = site;
-                s_own[at] = own;
-                s_nb[0][at] = gain;
+                const uint32_t at = base + __popcll(m & __lanemask_lt());
+                L.act_site[at] = site;
+                L.act_own[at] = own;
+                L.act_gain[at] = gain;
             }
-            na += (uint32_t)__popcll(m);
         }
-        // priority draws and claims (CheckerboardPriorityStream); stretch `wave` of the segment
-        for (uint32_t e = (uint32_t)lane; e < na; e += 64) {
-            const uint32_t site = s_site[e];
-            const int32_t losing = s_own[e];
-            const int32_t gaining = s_nb[0][e];
+        __syncthreads();
+        const uint32_t na = L.nact[it & 1];
+        // priority draws, claims and the segment entries
+        for (uint32_t e = tid; e < na; e += kBlock) {
+            const uint32_t site = L.act_site[e];
+            const int32_t losing = L.act_own[e];
+            const int32_t gaining = L.act_gain[e];
             const uint32_t prio = rng_words(d.key_lo, d.key_hi, kStreamCheckerboardPriority, mcs, 0, 0,
                                             kSiteEntity, site, 0, 0).w0;
             const unsigned long long key = ((unsigned long long)prio << 32) | (uint32_t)(~site);
@@ -772,10 +801,8 @@ __device__ __forceinline__ void row_wave(const PassArgs &a, const CPM_CONST RepD
 
 template <int TX, int TY>
 __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
-    __shared__ uint32_t s_site[kWaves][2 * 64];
-    __shared__ int32_t s_own[kWaves][2 * 64];
-    __shared__ int32_t s_nb[kWaves][4][2 * 64];
-    __shared__ uint32_t s_cnt[kWaves][3];
+    __shared__ RowLds L;
+    __shared__ uint32_t s_cnt[kWaves][2];
     __shared__ uint32_t s_ckey[kClaimSlots];
     __shared__ unsigned long long s_cval[kClaimSlots];
 
@@ -791,6 +818,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
         s_ckey[i] = 0u;
         s_cval[i] = 0ull;
     }
+    if (threadIdx.x == 0) L.nact[0] = 0u;
     const uint64_t mcs = y.mcs;
     const uint32_t colour = y.order[a.pass];
     const int res = d.col_res[d.col_first[colour]];
@@ -798,31 +826,28 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     const int wave = (int)(threadIdx.x >> 6);
     uint32_t nsites = 0, nsame = 0, nact = 0;
     if (rx == 0)
-        row_wave<0, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], s_ckey, s_cval,
-                            nsites, nsame, nact);
+        row_tile<0, TX, TY>(a, d, ti, tile, mcs, ry, wave, L, s_ckey, s_cval, nsites, nsame, nact);
     else
-        row_wave<1, TX, TY>(a, d, ti, tile, mcs, ry, wave, s_site[wave], s_own[wave], s_nb[wave], s_ckey, s_cval,
-                            nsites, nsame, nact);
+        row_tile<1, TX, TY>(a, d, ti, tile, mcs, ry, wave, L, s_ckey, s_cval, nsites, nsame, nact);
     if (__lane_id() == 0) {
-        a.seg_count[tile * kWaves + wave] = nact;
         s_cnt[wave][0] = nsites;
         s_cnt[wave][1] = nsame;
-        s_cnt[wave][2] = nact;
     }
     __syncthreads();
+    if (threadIdx.x < kWaves)  // the whole segment is stretch 0
+        a.seg_count[tile * kWaves + threadIdx.x] = threadIdx.x == 0 ? nact : 0u;
     if (threadIdx.x == 0) {
         uint32_t *slot = a.slots + ((size_t)a.pass * a.ntiles + tile) * kSlotFields;
-        uint32_t t0 = 0, t1 = 0, t3 = 0;
+        uint32_t t0 = 0, t1 = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
             t0 += s_cnt[w][0];
             t1 += s_cnt[w][1];
-            t3 += s_cnt[w][2];
         }
-        slot[0] = t0;   // candidates
-        slot[1] = t1;   // same-owner no-ops
-        slot[2] = 0;    // boundary no-ops: none on a periodic obstacle-free lattice
-        slot[3] = t3;   // realized (actionable) attempts
+        slot[0] = t0;    // candidates
+        slot[1] = t1;    // same-owner no-ops
+        slot[2] = 0;     // boundary no-ops: none on a periodic obstacle-free lattice
+        slot[3] = nact;  // realized (actionable) attempts
     }
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
     for (int i = (int)threadIdx.x; i < kClaimSlots; i += kBlock) {
