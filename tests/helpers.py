@@ -232,3 +232,37 @@ def oracle_migration_replica(algorithm: str, dynamics_seed: int):
         return sim.total_energy(), int(st["boundary_measures"][0]), words
 
     return migration_observables(sim.step, observe)
+
+
+# ---------------------------------------------------------------------------
+# Deep-run digests (tests/golden/make_deep_digests.py writes them from the oracle)
+# ---------------------------------------------------------------------------
+DIGEST_FIELDS = (("tags", np.uint8), ("ids", np.uint32), ("finite_volumes", np.int32),
+                 ("medium_volumes", np.int32), ("boundary_measures", np.int64))
+
+
+def state_digest(state: dict) -> str:
+    """BLAKE2b-128 of the ownership and tracker arrays of a downloaded or oracle state."""
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for key, dtype in DIGEST_FIELDS:
+        h.update(np.ascontiguousarray(state[key], dtype=dtype).tobytes())
+    return h.hexdigest()
+
+
+def load_deep_digests():
+    with open(os.path.join(GOLDEN, "deep_digests.json")) as handle:
+        return json.load(handle)
+
+
+def deep_serving_jobs(n2d: int = 6, n3d: int = 2, mcs: int = 20):
+    """Jobs of the served stream at their configured sizes: the first n2d 2D jobs of side at
+    least 1024 and n3d 3D jobs of side at least 96, each run for `mcs` MCS."""
+    from corepotts import workloads
+    picked2, picked3 = [], []
+    for job in workloads.serving_jobs(400):
+        if job.ndim == 2 and job.side >= 1024 and len(picked2) < n2d:
+            picked2.append(job._replace(mcs=mcs))
+        if job.ndim == 3 and job.side >= 96 and len(picked3) < n3d:
+            picked3.append(job._replace(mcs=mcs))
+    return picked2 + picked3
