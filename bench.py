@@ -95,16 +95,22 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
     """Batched ensemble of `replicas` 2D replicas (sides log-uniform in [64, 1024], T, J and
     seed per replica) for `mcs` MCS in one ReplicaBatch, against the same replicas run one
     after another through the plain single-lattice path on the same GPU."""
-    from corepotts import ReplicaBatch, init_scientific, workloads
+    from corepotts import BatchShapes, ReplicaBatch, init_scientific, workloads
     from corepotts.spec import ALG_CHECKERBOARD
-    default_tile_shapes()
     log(f"batched: building {replicas} replicas")
     t0 = time.perf_counter()
     models = [workloads.ensemble_replica(r) for r in range(replicas)]
     sites = [int(m[0].tags.size) for m in models]
     log(f"batched: built in {time.perf_counter() - t0:.1f} s, {sum(sites) / 1e6:.1f} M sites")
 
-    batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=graph_chunk)
+    # tile shape per size class, tuned on other draws of the same ensemble distribution
+    t0 = time.perf_counter()
+    shapes = BatchShapes()
+    shapes.tune([workloads.ensemble_replica(replicas + i) for i in range(128)], per_class=16, mcs=10)
+    tune_s = time.perf_counter() - t0
+    log(f"batched: shapes tuned in {tune_s:.1f} s: {shapes.describe()}")
+
+    batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=graph_chunk, shapes=shapes)
     slots = [batch.add_model(*m, target_mcs=mcs) for m in models]
     batch.synchronize()
     log(f"batched: {mcs} MCS")
@@ -117,6 +123,7 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
     batch.close()
 
     log("plain: replicas one after another")
+    default_tile_shapes()
     plain_s = 0.0
     same = 0
     for m, d in zip(models, digests):
@@ -134,7 +141,7 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
             "batched_attempts_per_s": attempts / batched_s, "batched_s": batched_s,
             "plain_sequential_attempts_per_s": attempts / plain_s, "plain_sequential_s": plain_s,
             "speedup": plain_s / batched_s, "replicas_identical_to_plain": same,
-            "device_bytes": device_bytes}
+            "device_bytes": device_bytes, "tile_shapes": shapes.describe(), "tune_s": tune_s}
 
 
 def lottery_leg(steps: int, shape: int = -1):
@@ -244,26 +251,36 @@ def tiled_leg(steps: int):
     return out
 
 
-def serving_leg(jobs: int, chunk: int, budget_gb: float):
-    """Served stream: the first `jobs` Poisson arrivals at 200 jobs/s (80 % 2D side
+def serving_leg(jobs: int, chunk: int, budget_gb: float, graph_chunk: int = -1, rate: float = 200.0):
+    """Served stream: the first `jobs` Poisson arrivals at `rate` jobs/s (80 % 2D side
     log-uniform [128, 2048], 20 % 3D side log-uniform [32, 128], 100-2000 MCS), admitted under
-    an HBM budget and advanced by continuous batching; requests are built before the clock
-    starts, admission (compile, allocation, upload) and result download count."""
-    from corepotts import JobServer, latency_summary, make_request, workloads
-    default_tile_shapes()
+    an HBM budget and advanced by continuous batching (HIP graphs of `graph_chunk` MCS between
+    membership changes; -1: one server step); requests are built before the clock starts,
+    admission (compile, allocation, upload) and result download count."""
+    from corepotts import BatchShapes, JobServer, latency_summary, make_request, workloads
     log(f"serving: building {jobs} requests")
     t0 = time.perf_counter()
-    stream = workloads.serving_jobs(jobs)
+    stream = workloads.serving_jobs(jobs, rate_per_s=rate)
     reqs = [make_request(j.index, j.arrival_s, *workloads.serving_model(j), mcs=j.mcs) for j in stream]
     log(f"serving: built in {time.perf_counter() - t0:.1f} s")
-    server = JobServer(int(budget_gb * 1e9), chunk_mcs=chunk)
+    # tile shape per size class, tuned before serving on another draw of the stream
+    t0 = time.perf_counter()
+    shapes = BatchShapes()
+    sample = workloads.serving_jobs(64, seed=workloads.SERVING_SEED + 1)
+    shapes.tune([workloads.serving_model(j) for j in sample], per_class=6, mcs=10)
+    tune_s = time.perf_counter() - t0
+    log(f"serving: shapes tuned in {tune_s:.1f} s: {shapes.describe()}")
+    server = JobServer(int(budget_gb * 1e9), chunk_mcs=chunk, shapes=shapes,
+                       graph_chunk=None if graph_chunk < 0 else graph_chunk)
     t0 = time.perf_counter()
     done = server.serve(reqs)
     wall = time.perf_counter() - t0
     summary = latency_summary(done)
     summary.update({"wall_s": wall, "chunk_mcs": chunk, "budget_bytes": int(budget_gb * 1e9),
                     "peak_jobs_resident": server.peak_jobs, "peak_bytes": server.peak_used,
-                    "arrival_rate_per_s": 200.0, "jobs_3d": sum(j.ndim == 3 for j in stream)})
+                    "arrival_rate_per_s": rate, "jobs_3d": sum(j.ndim == 3 for j in stream),
+                    "graph_chunk": server.graph_chunk, "tile_shapes": shapes.describe(),
+                    "tune_s": tune_s})
     server.batch.close()
     return summary
 
@@ -399,6 +416,9 @@ def main():
     ap.add_argument("--tiled-steps", type=int, default=20)
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-chunk", type=int, default=20)
+    ap.add_argument("--serve-graph-chunk", type=int, default=-1,
+                    help="MCS per HIP graph in the served batch (-1: one server step, 0: no graphs)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="job arrivals per second")
     ap.add_argument("--tile-shape", type=int, default=-1,
                     help="2D checkerboard tile shape of the headline run (default: autotuned)")
     ap.add_argument("--lottery-tile-shape", type=int, default=-1,
@@ -430,7 +450,8 @@ def main():
     if not args.no_batch:
         extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
     if not args.no_serve:
-        extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0)
+        extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0, args.serve_graph_chunk,
+                                       args.serve_rate)
 
     baseline = None
     if not args.no_cpu_baseline:

@@ -21,7 +21,7 @@ from .native import (EngineError, ScientificMCSReport, ScientificPottsIntegrator
                      ensemble_seed, init_scientific, load_library)
 from .batch import ReplicaBatch, replica_bytes
 from .serve import Completion, JobServer, Request, latency_summary, make_request
-from .autotune import set_row_kernels, set_tiled_staging, set_tile_shape, tile_shapes, tune_tile_shape
+from .autotune import BatchShapes, set_row_kernels, set_tiled_staging, set_tile_shape, tile_shapes, tune_tile_shape
 from .checkpoint import (CanonicalCheckpoint, CheckpointCompatibilityError, CheckpointIntegrityError,
                          IncompleteCheckpointError, capture_checkpoint, import_checkpoint,
                          load_checkpoint, restore_checkpoint, save_checkpoint, validate_checkpoint)

@@ -79,3 +79,87 @@ def tune_tile_shape(state, proposal_relation, components, algorithm, seed: int =
     best = min(timings, key=timings.get)
     set_tile_shape(spec.algorithm, ndim, halo, best)
     return {"shape": best, "tile": list(tiles[best]), "ms_per_mcs": timings, "row_kernel": rows}
+
+
+# Size classes of batched replicas, by the longest lattice side: one tile shape is tuned per
+# (algorithm, dimension, halo, class). Replicas of different shapes form different launch groups.
+SIZE_CLASS_BOUNDS = (128, 256, 512)
+
+
+def size_class(spec) -> int:
+    side = max(int(d) for d in spec.dims[:spec.ndim])
+    return sum(side >= b for b in SIZE_CLASS_BOUNDS)
+
+
+class BatchShapes:
+    """Tile shape per (algorithm, dimension, halo, size class) for batched replicas.
+
+    Replicas of different tile shapes run in different launch groups, so a class's best shape
+    depends on the others (one group for everything saves launches; a shape fitted to a class
+    saves halo work). tune() therefore times the whole sample batch: starting from the engine
+    defaults it goes class by class, from the largest lattices down, tries every compiled
+    shape for that class with the other classes fixed, and keeps the fastest (HIP graphs of
+    `mcs` MCS, timed after a warm-up step of the same length). apply(spec) sets the
+    process-wide preference for the next replica of that class; ReplicaBatch.reserve calls it.
+    Shapes change speed only, never results."""
+
+    def __init__(self):
+        self.shapes: Dict[tuple, int] = {}
+        self.timings: Dict[tuple, Dict[int, float]] = {}
+
+    @staticmethod
+    def key(spec) -> tuple:
+        return (spec.algorithm, spec.ndim, model_halo(spec), size_class(spec))
+
+    def apply(self, spec):
+        k = self.key(spec)
+        set_tile_shape(spec.algorithm, spec.ndim, k[2], self.shapes.get(k, -1))
+
+    def _time_sample(self, sample, mcs: int) -> float:
+        import time
+        from .batch import ReplicaBatch
+        batch = ReplicaBatch(sample[0][1].algorithm, graph_chunk=mcs, shapes=self)
+        try:
+            for m, spec in sample:
+                batch.add(m[0], spec, target_mcs=2 * mcs)
+            batch.step(mcs)  # warm-up: first launches and graph capture
+            batch.synchronize()
+            t0 = time.perf_counter()
+            batch.step(mcs)
+            batch.synchronize()
+            return time.perf_counter() - t0
+        finally:
+            batch.close()
+
+    def tune(self, models, per_class: int = 16, mcs: int = 10) -> Dict[tuple, int]:
+        """models: init_scientific argument tuples of one algorithm; the first per_class
+        replicas of each class form the timed sample."""
+        classes: Dict[tuple, list] = {}
+        for m in models:
+            spec = build_spec(*m)
+            members = classes.setdefault(self.key(spec), [])
+            if len(members) < per_class:
+                members.append((m, spec))
+        sample = [x for members in classes.values() for x in members]
+        if not sample:
+            return self.shapes
+        for k in sorted(classes, key=lambda k: -k[3]):
+            times = {}
+            for shape, _tile in tile_shapes(k[1], k[2]):
+                self.shapes[k] = shape
+                times[shape] = self._time_sample(sample, mcs)
+            self.shapes[k] = min(times, key=times.get)
+            self.timings[k] = times
+        for k in classes:  # the process-wide preferences back to the defaults
+            set_tile_shape(k[0], k[1], k[2], -1)
+        return self.shapes
+
+    def describe(self) -> Dict[str, dict]:
+        """{"alg/ndim/halo/class": {"shape", "tile", "ms": {shape: ms}}} for reports."""
+        out = {}
+        for k, shape in self.shapes.items():
+            tiles = dict(tile_shapes(k[1], k[2]))
+            out["%d/%dd/h%d/c%d" % k] = {
+                "shape": shape, "tile": list(tiles[shape]),
+                "ms": {int(s): round(t * 1e3, 3) for s, t in self.timings.get(k, {}).items()}}
+        return out

@@ -38,7 +38,7 @@ class ReplicaBatch:
     TiledCheckerboardCPM)."""
 
     def __init__(self, algorithm: int = ALG_CHECKERBOARD, hbm_budget_bytes: int = 0,
-                 graph_chunk: int = 0):
+                 graph_chunk: int = 0, shapes=None):
         if algorithm not in (ALG_CHECKERBOARD, ALG_LOTTERY, ALG_TILED):
             raise NotImplementedError("the MI355X engine batches CheckerboardSweepCPM, LotteryCPM and "
                                       "TiledCheckerboardCPM")
@@ -49,6 +49,7 @@ class ReplicaBatch:
         if rc != 0:
             raise EngineError(rc, self.lib.cpm_batch_last_error(None).decode())
         self._b = handle
+        self.shapes = shapes  # optional autotune.BatchShapes: tile shape per replica size class
         self.specs: Dict[int, ModelSpec] = {}
         self.compiled: Dict[int, CompiledScientificState] = {}
         if graph_chunk:
@@ -78,6 +79,8 @@ class ReplicaBatch:
         replica stays idle until load() gives it a state and a target."""
         if spec.algorithm != self.algorithm:
             raise ValueError("replica algorithm differs from the batch")
+        if self.shapes is not None:
+            self.shapes.apply(spec)
         keep: list = []
         desc = make_desc(spec, keep)
         slot = C.c_int32()

@@ -65,13 +65,17 @@ class JobServer:
 
     def __init__(self, hbm_budget_bytes: int, chunk_mcs: int = 50, algorithm: int = ALG_CHECKERBOARD,
                  batch=None, bytes_of: Callable[[ModelSpec], int] = replica_bytes,
-                 graph_chunk: int = 0, clock: Callable[[], float] = time.perf_counter,
-                 sleep: Callable[[float], None] = time.sleep):
+                 graph_chunk: Optional[int] = None, clock: Callable[[], float] = time.perf_counter,
+                 sleep: Callable[[float], None] = time.sleep, shapes=None):
         if chunk_mcs < 1:
             raise ValueError("chunk_mcs must be positive")
         self.budget = int(hbm_budget_bytes)
         self.chunk = int(chunk_mcs)
-        self.batch = batch if batch is not None else ReplicaBatch(algorithm, self.budget, graph_chunk)
+        # steps between membership changes replay HIP graphs of one server step (chunk_mcs MCS)
+        # unless graph_chunk says otherwise (0: plain launches)
+        gc = self.chunk if graph_chunk is None else int(graph_chunk)
+        self.graph_chunk = gc
+        self.batch = batch if batch is not None else ReplicaBatch(algorithm, self.budget, gc, shapes)
         self.bytes_of = bytes_of
         self.clock = clock
         self.sleep = sleep

@@ -115,7 +115,9 @@ def test_batch_equals_single_integrator(gpu_lib):
     gpu.close()
 
 
-def test_served_jobs_equal_solo_runs(gpu_lib):
+@pytest.mark.parametrize("graph_chunk", [None, 0])
+def test_served_jobs_equal_solo_runs(gpu_lib, graph_chunk):
+    # None: the server replays HIP graphs of one server step between membership changes
     jobs = workloads.serving_jobs(8, rate_per_s=400.0, min_2d=20, max_2d=70, min_3d=8, max_3d=16)
     reqs, models = [], {}
     for j in jobs:
@@ -125,7 +127,7 @@ def test_served_jobs_equal_solo_runs(gpu_lib):
         reqs.append(make_request(j.index, j.arrival_s, *model, mcs=mcs))
     sizes = sorted(replica_bytes(r.spec) for r in reqs)
     # a budget of the three largest forces queueing and staggered admissions
-    server = JobServer(sum(sizes[-3:]) + (8 << 20), chunk_mcs=3)
+    server = JobServer(sum(sizes[-3:]) + (8 << 20), chunk_mcs=3, graph_chunk=graph_chunk)
     done = server.serve(reqs)
     assert sorted(c.key for c in done) == [j.index for j in jobs]
     assert server.peak_jobs >= 2
@@ -133,3 +135,31 @@ def test_served_jobs_equal_solo_runs(gpu_lib):
         _check(c.state, c.report, _solo(models[c.key], c.mcs), f"job {c.key}")
     s = latency_summary(done)
     assert s["jobs"] == len(jobs) and s["p99_s"] >= s["p50_s"] > 0
+
+
+def test_batch_shapes_per_size_class_keep_results(gpu_lib):
+    """Tile shapes tuned per size class (replicas of different shapes run in different launch
+    groups) change speed only: the batch still equals the oracle's solo runs."""
+    from corepotts import BatchShapes
+    from corepotts.autotune import size_class, tile_shapes
+    rng = np.random.default_rng(5)
+    models = []
+    for k, L in enumerate([40, 60, 130, 140, 260]):
+        J = workloads.integer_contact_matrix(rng, 2)
+        models.append(workloads.block_cells(L, 2, 10, 10, ntypes=2, J=J, temperature=np.float32(8.0),
+                                            seed=ensemble_seed(7, k + 1)))
+    assert sorted({size_class(build_spec(*m)) for m in models}) == [0, 1, 2]
+    shapes = BatchShapes()
+    tuned = shapes.tune(models, per_class=2, mcs=2)
+    assert len(tuned) == 3
+    for k, shape in tuned.items():
+        assert shape in dict(tile_shapes(k[1], k[2]))
+        assert set(shapes.timings[k]) == set(dict(tile_shapes(k[1], k[2])))
+    # force three different shapes so the batch has one launch group per class
+    shapes.shapes = {k: s for k, s in zip(sorted(tuned), [7, 0, 12])}
+    batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=3, shapes=shapes)
+    slots = [batch.add_model(*m, target_mcs=6) for m in models]
+    batch.step(6)
+    for s, m in zip(slots, models):
+        _check(batch.download(s), batch.report(s), _solo(m, 6), f"slot {s}")
+    batch.close()
