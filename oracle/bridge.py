@@ -124,6 +124,8 @@ def load() -> C.CDLL:
     lib.orc_total_energy.restype = C.c_double
     lib.orc_total_energy.argtypes = [P]
     lib.orc_accepted_energy.argtypes = [P, P]
+    lib.orc_set_threads.argtypes = [C.c_int32]
+    lib.orc_threads.restype = C.c_int32
     _lib = lib
     return lib
 
@@ -340,6 +342,13 @@ class OracleSim:
         if rc != 0:
             raise RuntimeError("a destination owner is not among the catalog labels")
         return counts
+
+
+def set_threads(threads: int) -> int:
+    """Threads of the oracle's colour-pass loops (0: every processor); returns the count in use."""
+    lib = load()
+    lib.orc_set_threads(int(threads))
+    return int(lib.orc_threads())
 
 
 def coloring(spec) -> np.ndarray:

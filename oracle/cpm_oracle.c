@@ -7,6 +7,9 @@
 #include "cpm_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -232,6 +235,7 @@ struct orc_sim {
     uint32_t *att_recipient, *att_priority;
     uint8_t *att_outcome, *att_selected, *att_disp;
     int64_t *att_dl, *att_dg;
+    float *att_dh;           /* delta_h of evaluated attempts */
     uint32_t *cell_max_priority, *cell_min_identity;
     uint32_t (*tickets)[4];
     uint32_t *activations;   /* lottery: activated opportunities of each mutable position in the
@@ -526,6 +530,7 @@ orc_sim *orc_create(const orc_model *model, int32_t algorithm, uint64_t seed) {
     s->att_disp = (uint8_t *)xcalloc((size_t)nm, 1);
     s->att_dl = (int64_t *)xcalloc((size_t)nm, 8);
     s->att_dg = (int64_t *)xcalloc((size_t)nm, 8);
+    s->att_dh = (float *)xcalloc((size_t)nm, 4);
     s->cell_max_priority = (uint32_t *)xcalloc((size_t)cap + 1, 4);
     s->cell_min_identity = (uint32_t *)xcalloc((size_t)cap + 1, 4);
     s->order = (uint32_t *)xcalloc(256, 4);
@@ -601,7 +606,7 @@ void orc_destroy(orc_sim *s) {
     free(s->st.boundary_strength);
     free(s->st.volume_pressure); free(s->st.surface_tension);
     free(s->att_losing); free(s->att_gaining); free(s->att_recipient); free(s->att_priority);
-    free(s->att_outcome); free(s->att_selected); free(s->att_disp); free(s->att_dl);
+    free(s->att_outcome); free(s->att_selected); free(s->att_disp); free(s->att_dl); free(s->att_dh);
     free(s->att_dg); free(s->cell_max_priority); free(s->cell_min_identity); free(s->order);
     free(s->color_sites); free(s->color_offsets); free(s->csr_offsets); free(s->csr_nbrs);
     free(s->tickets);
@@ -1028,25 +1033,59 @@ static void commit(orc_sim *s, uint32_t recipient, int64_t losing, int64_t gaini
 /* Claim protocol: checkerboard.jl:273-369                                    */
 /* ------------------------------------------------------------------------- */
 
+/* Loops over the sites or attempts of one colour run on several threads when they are long
+ * enough. Everything they do is independent per element or an order-free maximum / minimum
+ * (claims), and the sums are formed afterwards in element order, so the result is the same for
+ * any thread count. */
+#define ORC_PAR_MIN 16384
+
+static void atomic_max_u32(uint32_t *p, uint32_t v) {
+    uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+static void atomic_min_u32(uint32_t *p, uint32_t v) {
+    uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+
+void orc_set_threads(int32_t threads) {
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : omp_get_num_procs());
+#else
+    (void)threads;
+#endif
+}
+
+int32_t orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static void claims_run(int32_t n, const uint32_t *idx, const uint8_t *use,
                        const uint32_t *recipient, const int64_t *losing, const int64_t *gaining,
                        const uint32_t *priority, uint32_t *maxp, uint32_t *minid,
                        uint8_t *selected, uint8_t *disp) {
     /* idx maps 0..n-1 to attempt slots; use[] marks attempts that take part */
+#pragma omp parallel for if (n >= ORC_PAR_MIN) schedule(static)
     for (int32_t k = 0; k < n; ++k) {
         uint32_t a = idx ? idx[k] : (uint32_t)k;
         if (!use[a]) continue;
-        if (losing[a] > 0 && priority[a] > maxp[losing[a]]) maxp[losing[a]] = priority[a];
-        if (gaining[a] > 0 && priority[a] > maxp[gaining[a]]) maxp[gaining[a]] = priority[a];
+        if (losing[a] > 0) atomic_max_u32(&maxp[losing[a]], priority[a]);
+        if (gaining[a] > 0) atomic_max_u32(&maxp[gaining[a]], priority[a]);
     }
+#pragma omp parallel for if (n >= ORC_PAR_MIN) schedule(static)
     for (int32_t k = 0; k < n; ++k) {
         uint32_t a = idx ? idx[k] : (uint32_t)k;
         if (!use[a]) continue;
-        if (losing[a] > 0 && maxp[losing[a]] == priority[a] && recipient[a] < minid[losing[a]])
-            minid[losing[a]] = recipient[a];
-        if (gaining[a] > 0 && maxp[gaining[a]] == priority[a] && recipient[a] < minid[gaining[a]])
-            minid[gaining[a]] = recipient[a];
+        if (losing[a] > 0 && maxp[losing[a]] == priority[a]) atomic_min_u32(&minid[losing[a]], recipient[a]);
+        if (gaining[a] > 0 && maxp[gaining[a]] == priority[a]) atomic_min_u32(&minid[gaining[a]], recipient[a]);
     }
+#pragma omp parallel for if (n >= ORC_PAR_MIN) schedule(static)
     for (int32_t k = 0; k < n; ++k) {
         uint32_t a = idx ? idx[k] : (uint32_t)k;
         if (!use[a]) continue;
@@ -1111,7 +1150,7 @@ static void evaluate_selected(orc_sim *s, uint32_t a, uint64_t mcs, uint32_t sub
     orc_rng_words(s->seed, S_ACCEPTANCE, mcs, subround, 0, K_SITE, rec, 0, 0, 0, w);
     float u = orc_uniform_open01_f32(w[0]);
     s->att_disp[a] = u < p ? D_ACCEPTED : D_REJECTED;
-    if (u < p) s->dh_last_mcs += (double)dh;
+    s->att_dh[a] = dh;
 }
 
 /* checkerboard.jl:521-586 */
@@ -1132,6 +1171,7 @@ static void checkerboard_mcs(orc_sim *s) {
             s->cell_min_identity[c] = 0xffffffffu;
         }
         /* candidates: checkerboard.jl:244-271 */
+#pragma omp parallel for if (stop - first >= ORC_PAR_MIN) schedule(static)
         for (uint32_t k = first; k < stop; ++k) {
             uint32_t site = s->color_sites[k];
             uint32_t dir = orc_bounded_uint(s->seed, S_DIRECTION, mcs, 0, 0, K_SITE, site, 0,
@@ -1158,12 +1198,15 @@ static void checkerboard_mcs(orc_sim *s) {
             free(use);
         }
         /* evaluate every selected attempt against the common snapshot, then commit */
+#pragma omp parallel for if (stop - first >= ORC_PAR_MIN) schedule(dynamic, 1024)
         for (uint32_t k = first; k < stop; ++k)
             if (s->att_selected[k]) evaluate_selected(s, k, mcs, 0);
         for (uint32_t k = first; k < stop; ++k)
-            if (s->att_disp[k] == D_ACCEPTED)
+            if (s->att_disp[k] == D_ACCEPTED) {
                 commit(s, s->att_recipient[k], s->att_losing[k], s->att_gaining[k], s->att_dl[k],
                        s->att_dg[k]);
+                s->dh_last_mcs += (double)s->att_dh[k];
+            }
         orc_advance_mechanics(s, mcs, (uint32_t)ordinal, 1, half);
     }
     /* checkerboard.jl:459-505 */
@@ -1214,9 +1257,12 @@ static void lottery_mcs(orc_sim *s) {
             s->cell_max_priority[c] = 0;
             s->cell_min_identity[c] = 0xffffffffu;
         }
+#pragma omp parallel for if (nm >= ORC_PAR_MIN) schedule(static)
         for (int64_t i = 0; i < nm; ++i)
             orc_rng_words(s->seed, S_LOTTERY_PRIORITY, mcs, sub, 0, K_SITE, s->mutable_sites[i], 0,
                           0, 0, s->tickets[i]);
+        uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+#pragma omp parallel for if (nm >= ORC_PAR_MIN) schedule(static) reduction(+ : a0, a1, a2, a3, a4)
         for (int64_t i = 0; i < nm; ++i) {
             uint32_t site = s->mutable_sites[i];
             uint32_t f = s->csr_offsets[i], e = s->csr_offsets[i + 1];
@@ -1232,7 +1278,7 @@ static void lottery_mcs(orc_sim *s) {
             s->att_disp[i] = D_PENDING;
             use[i] = 0;
             if (eligible && wins) {
-                acc[0]++;
+                a0++;
                 s->activations[i]++;
                 uint32_t dir = orc_bounded_uint(s->seed, S_DIRECTION, mcs, sub, 0, K_SITE, site, 0,
                                                 (uint32_t)K) + 1;
@@ -1242,27 +1288,32 @@ static void lottery_mcs(orc_sim *s) {
                 s->att_recipient[i] = site;
                 s->att_outcome[i] = (uint8_t)out;
                 s->att_priority[i] = s->tickets[i][0];
-                if (out == OUT_ACTIONABLE) { acc[1]++; s->att_selected[i] = 1; use[i] = 1; }
-                else if (out == OUT_SAME_OWNER) acc[2]++;
-                else if (out == OUT_BOUNDARY_NULL) acc[3]++;
-                else acc[4]++;
+                if (out == OUT_ACTIONABLE) { a1++; s->att_selected[i] = 1; use[i] = 1; }
+                else if (out == OUT_SAME_OWNER) a2++;
+                else if (out == OUT_BOUNDARY_NULL) a3++;
+                else a4++;
             }
         }
+        acc[0] += a0; acc[1] += a1; acc[2] += a2; acc[3] += a3; acc[4] += a4;
         claims_run((int32_t)nm, NULL, use, s->att_recipient, s->att_losing, s->att_gaining,
                    s->att_priority, s->cell_max_priority, s->cell_min_identity, s->att_selected,
                    s->att_disp);
         for (int64_t i = 0; i < nm; ++i)
             if (use[i] && !s->att_selected[i]) acc[5]++;
+#pragma omp parallel for if (nm >= ORC_PAR_MIN) schedule(dynamic, 1024)
+        for (int64_t i = 0; i < nm; ++i)
+            if (s->att_selected[i]) evaluate_selected(s, (uint32_t)i, mcs, sub);
         for (int64_t i = 0; i < nm; ++i)
             if (s->att_selected[i]) {
-                evaluate_selected(s, (uint32_t)i, mcs, sub);
-                if (s->att_disp[i] == D_ACCEPTED) acc[8]++;
-                else acc[7]++;
+                if (s->att_disp[i] == D_ACCEPTED) {
+                    acc[8]++;
+                    commit(s, s->att_recipient[i], s->att_losing[i], s->att_gaining[i], s->att_dl[i],
+                           s->att_dg[i]);
+                    s->dh_last_mcs += (double)s->att_dh[i];
+                } else {
+                    acc[7]++;
+                }
             }
-        for (int64_t i = 0; i < nm; ++i)
-            if (s->att_selected[i] && s->att_disp[i] == D_ACCEPTED)
-                commit(s, s->att_recipient[i], s->att_losing[i], s->att_gaining[i], s->att_dl[i],
-                       s->att_dg[i]);
         orc_advance_mechanics(s, mcs, (uint32_t)ordinal, 1, half);
     }
     free(use);

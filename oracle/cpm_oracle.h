@@ -153,6 +153,11 @@ float orc_normal_box_muller(uint64_t seed, uint32_t stream, uint64_t mcs, uint32
 float orc_ou_transition(float q, float displacement, float strength, float eta, float noise,
                         float interval, float standard_normal);
 
+/* Threads of the colour-pass loops of the checkerboard and lottery sweeps (0: every
+ * processor); results do not depend on it. */
+void orc_set_threads(int32_t threads);
+int32_t orc_threads(void);
+
 /* ---- schedules ---- */
 /* Greedy realized colouring. colors[N] receives 1-based colours (0 for immutable
  * sites); returns the colour count or a negative error. */
