@@ -56,21 +56,44 @@ def traffic_from_profiles(passes: int):
         return None
 
 
-def cpu_baseline(sample_mcs: int):
-    """Single-threaded C oracle on the same workload (bounded sample)."""
+def host_threads() -> int:
+    """Host threads this process may use: OMP_NUM_THREADS when set, else the CPUs it may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "").strip()
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(sample_mcs: int, warmup_mcs: int):
+    """The C restatement of the sweep (the oracle: colour passes on OpenMP threads, results
+    independent of the thread count) on the same 4096^2 workload: `warmup_mcs` MCS from the
+    initial state, then `sample_mcs` steady-state MCS on every host thread this process may use,
+    then `sample_mcs` more on one thread. Returns (cpu_baseline, one-thread figure)."""
     from corepotts import build_spec, state_arrays, workloads
-    from oracle.bridge import OracleSim
+    from oracle.bridge import OracleSim, set_threads
     compiled, proposal, components, alg, seed = workloads.config2_large()
     spec = build_spec(compiled, proposal, components, alg, seed)
     sim = OracleSim(spec, state_arrays(compiled, spec))
+    threads = set_threads(host_threads())
+    sim.step(warmup_mcs)
+    attempts_per_mcs = sim.report()[3]  # every mutable site is attempted once per MCS
     t0 = time.perf_counter()
     sim.step(sample_mcs)
-    dt = time.perf_counter() - t0
-    rep = sim.report()
-    attempts = rep[3] * sample_mcs
-    return {"value": attempts / dt, "unit": "copy attempts/s", "cores": 1, "kind": "port",
-            "sample": f"{sample_mcs} MCS of the 4096x4096 checkerboard workload from its initial "
-                      f"state, C oracle (single thread), {dt:.1f} s"}
+    dt_all = time.perf_counter() - t0
+    set_threads(1)
+    t0 = time.perf_counter()
+    sim.step(sample_mcs)
+    dt_one = time.perf_counter() - t0
+    set_threads(host_threads())
+    base = {"value": attempts_per_mcs * sample_mcs / dt_all, "unit": "copy attempts/s",
+            "cores": threads, "kind": "port",
+            "sample": f"C restatement (oracle), {threads} OpenMP threads: MCS {warmup_mcs + 1}-"
+                      f"{warmup_mcs + sample_mcs} of the 4096x4096 checkerboard workload after "
+                      f"{warmup_mcs} warm-up MCS, {dt_all:.1f} s"}
+    one = {"value": attempts_per_mcs * sample_mcs / dt_one, "unit": "copy attempts/s", "cores": 1,
+           "sample": f"MCS {warmup_mcs + sample_mcs + 1}-{warmup_mcs + 2 * sample_mcs}, one thread, "
+                     f"{dt_one:.1f} s"}
+    return base, one
 
 
 def state_digest(state: dict) -> str:
@@ -400,7 +423,8 @@ def main():
     ap.add_argument("--graph-chunk", type=int, default=50)
     ap.add_argument("--timing-mcs", type=int, default=40)
     ap.add_argument("--lottery-steps", type=int, default=50)
-    ap.add_argument("--cpu-sample-mcs", type=int, default=3)
+    ap.add_argument("--cpu-sample-mcs", type=int, default=20)
+    ap.add_argument("--cpu-warmup-mcs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lottery", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=256)
@@ -455,8 +479,12 @@ def main():
 
     baseline = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle)")
-        baseline = cpu_baseline(args.cpu_sample_mcs)
+        log("cpu baseline (C restatement, every host thread, then one thread)")
+        baseline, one_thread = cpu_baseline(args.cpu_sample_mcs, args.cpu_warmup_mcs)
+        extra["cpu_baseline_one_thread"] = one_thread
+        if line.get("value"):
+            extra["gpu_over_cpu"] = {"all_threads": line["value"] / baseline["value"],
+                                     "one_thread": line["value"] / one_thread["value"]}
 
     line["cpu_baseline"] = baseline
     line["extra"] = extra
