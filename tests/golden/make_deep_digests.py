@@ -11,11 +11,14 @@ Parts (each a list of {mcs, digest, report}):
                         side >= 96): each at MCS 20
 
 Run from the repository root: python tests/golden/make_deep_digests.py [--parts a,b] [--workers 8]
-The oracle runs single-threaded per part; parts and ensemble replicas run in parallel
-processes. The full set takes about 20 minutes on 8 cores.
+The three large parts run one after another with the oracle's colour-pass loops on every
+processor (the oracle's results do not depend on its thread count); ensemble replicas and jobs
+run single-threaded in parallel processes. The file is rewritten after every part, and parts
+already in it are kept unless named again. The full set takes about 20 minutes on 8 cores.
 """
 import argparse
 import json
+import multiprocessing
 import os
 import sys
 import time
@@ -65,6 +68,31 @@ def part(name):
     return name, res
 
 
+LARGE = ("config2_checkerboard", "config2_lottery", "config3_checkerboard")
+
+
+def _single_threaded(name):
+    from oracle.bridge import set_threads
+    set_threads(1)
+    return part(name)
+
+
+def _store(data, name, res):
+    if name.startswith("config4_replica_"):
+        data.setdefault("config4_ensemble", {})[name.rsplit("_", 1)[1]] = res
+    elif name.startswith("config5_job_"):
+        data.setdefault("config5_jobs", {})[name.rsplit("_", 1)[1]] = res
+    else:
+        data[name] = res
+
+
+def _write(data):
+    tmp = OUT + ".tmp"
+    with open(tmp, "w") as handle:
+        json.dump(data, handle, indent=0, sort_keys=True)
+    os.replace(tmp, OUT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--parts", default="config2_checkerboard,config2_lottery,config3_checkerboard,config4,config5")
@@ -82,16 +110,19 @@ def main():
     if os.path.exists(OUT):
         with open(OUT) as handle:
             data = json.load(handle)
-    with ProcessPoolExecutor(max_workers=args.workers) as pool:
-        for name, res in pool.map(part, names):
-            if name.startswith("config4_replica_"):
-                data.setdefault("config4_ensemble", {})[name.rsplit("_", 1)[1]] = res
-            elif name.startswith("config5_job_"):
-                data.setdefault("config5_jobs", {})[name.rsplit("_", 1)[1]] = res
-            else:
-                data[name] = res
-    with open(OUT, "w") as handle:
-        json.dump(data, handle, indent=0, sort_keys=True)
+    from oracle.bridge import set_threads
+    set_threads(0)
+    for name in [n for n in names if n in LARGE]:
+        _store(data, *part(name))
+        _write(data)
+    small = [n for n in names if n not in LARGE]
+    if small:
+        # spawned, not forked: the parent has run OpenMP regions
+        ctx = multiprocessing.get_context("spawn")
+        with ProcessPoolExecutor(max_workers=args.workers, mp_context=ctx) as pool:
+            for name, res in pool.map(_single_threaded, small):
+                _store(data, name, res)
+        _write(data)
     print("wrote", OUT)
 
 
