@@ -909,6 +909,18 @@ void compile_replica(Replica &r) {
     // lottery on a periodic obstacle-free lattice with at least 2 halo + 1 sites per axis:
     // winners follow from the tickets alone
     if (d.algorithm == CPM_ALG_LOTTERY && all_periodic && wide) r.config |= 64;
+    // plain lottery whose conflict shell is the whole radius-1 Moore shell (8 positions in 2D,
+    // 26 in 3D): the winner test reads the tickets at compile-time offsets
+    if ((r.config & 64) && (r.config & 16) == 0) {
+        const int want = d.ndim == 2 ? 8 : 26;
+        bool moore = (int)(r.conf_off.size() / 3) == want;
+        for (size_t q = 0; moore && q < r.conf_off.size(); q += 3) {
+            const int64_t ox = r.conf_off[q], oy = r.conf_off[q + 1], oz = r.conf_off[q + 2];
+            moore = ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1 &&
+                    (d.ndim == 3 || oz == 0) && (ox | oy | oz) != 0;
+        }
+        if (moore) r.config |= 128;  // offsets are distinct, so the count fixes the set
+    }
     v.capacity = d.capacity;
     v.tiled_interval = r.tiled_interval;
     v.tiled_batches = r.tiled_batches;
@@ -1325,7 +1337,8 @@ struct Engine {
             Group g;
             g.config = r->config;
             int dim, halo;
-            cpm_internal_tile_config(g.config, &dim, &halo, &g.tx, &g.ty, &g.tz);
+            if (!g.tiled() && !cpm_internal_tile_config(g.config, &dim, &halo, &g.tx, &g.ty, &g.tz))
+                fail(CPM_E_STATE, "unknown kernel configuration");
             groups.push_back(g);
             gi = (int)groups.size() - 1;
         }

@@ -1085,8 +1085,11 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows3_kernel(PassArgs a) {
 // ---------------------------------------------------------------------------
 // PLAIN: periodic, obstacle-free and at least 2 halo + 1 sites along every axis, so every
 // conflict neighbour position of a tile site is a mutable site other than itself.
-template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT, bool PLAIN>
+// MOORE (with PLAIN, halo 1): the conflict shell is the radius-1 Moore shell, so the winner
+// test is the maximum of the 8 (26) neighbour tickets at compile-time LDS offsets.
+template <int DIM, int H, int TX, int TY, int TZ, int KF, int NT, bool PLAIN, bool MOORE = false>
 __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
+    static_assert(!MOORE || (PLAIN && H == 1), "Moore winner test needs a plain lattice and halo 1");
     using G = Geom<DIM, H, TX, TY, TZ>;
     constexpr int NW = NT / 64;
     constexpr int Q = G::TN / NW;
@@ -1163,10 +1166,12 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
             const int i2 = (G::PX - H) + k2 % WX + G::LX * (k2 / WX);
             const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
                                           sid(i), 0, 0).w0;
-            const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
-                                          k2 < WN ? sid(i2) : 0u, 0, 0).w0;
             s_tk[i] = t1;
-            if (k2 < WN) s_tk[i2] = t2;
+            if (k2 < WN) {  // skipped by whole waves in the last iteration
+                const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0,
+                                              kSiteEntity, sid(i2), 0, 0).w0;
+                s_tk[i2] = t2;
+            }
         }
     } else {
         for (int i = threadIdx.x; i < G::LN; i += 2 * NT) {
@@ -1194,19 +1199,35 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
             if (gx < X && gy < Y && gz < Z) {
                 const int pos = G::pos(ix, iy, iz);
                 if (PLAIN || s_mut[pos]) {
-                    const uint32_t site = PLAIN ? sid(pos) : s_sid[pos];
+                    // a tile site lies inside the lattice: its id needs no wrap
+                    const uint32_t site = PLAIN ? 1u + (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)
+                                                : s_sid[pos];
                     const uint32_t mine = s_tk[pos];
                     // strict lexicographic ticket maximum over the conflict neighbours
                     // (lottery.jl:153-185): word 1 decides unless it ties, which needs the
                     // full tickets (rare)
                     bool beaten = false, tied = false;
+                    if constexpr (MOORE) {
+                        uint32_t mx = 0u;
+#pragma unroll
+                        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
+                            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                                for (int dx = -1; dx <= 1; ++dx)
+                                    if (dx != 0 || dy != 0 || dz != 0)
+                                        mx = max(mx, s_tk[pos + dx + G::LX * (dy + G::LY * dz)]);
+                        beaten = mx > mine;
+                        tied = mx == mine;  // a neighbour ties and none is higher
+                    } else {
 #pragma unroll 8
-                    for (int k = 0; k < KC; ++k) {
-                        const int q = pos + s_cloff[k];
-                        const bool other_site = PLAIN || (s_mut[q] && s_sid[q] != site);
-                        const uint32_t other = s_tk[q];
-                        beaten |= other_site && other > mine;
-                        tied |= other_site && other == mine;
+                        for (int k = 0; k < KC; ++k) {
+                            const int q = pos + s_cloff[k];
+                            const bool other_site = PLAIN || (s_mut[q] && s_sid[q] != site);
+                            const uint32_t other = s_tk[q];
+                            beaten |= other_site && other > mine;
+                            tied |= other_site && other == mine;
+                        }
                     }
                     bool wins = !beaten;
                     if (wins && tied) {
@@ -1263,7 +1284,9 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
             const int i = s_work[list_slot<Q>(e, p1, p2, p3)];
             const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
             const int pos = G::pos(ix, iy, iz);
-            const uint32_t site = PLAIN ? sid(pos) : s_sid[pos];
+            const uint32_t site = PLAIN ? 1u + (uint32_t)(x0 + ix) +
+                                              (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz))
+                                        : s_sid[pos];
             const uint32_t dir = bounded_draw(d.key_lo, d.key_hi, kStreamDirection, mcs, sub, 0,
                                               kSiteEntity, site, 0, const_bound(d.prop_bound));
             const int q = pos + direction_offset<KF>(dir, po, s_ploff);
@@ -2540,7 +2563,8 @@ __global__ __launch_bounds__(kBlock) void energy_sum_kernel(const double *partia
 // Launchers
 // ---------------------------------------------------------------------------
 // generic == 0: the proposal relation has 2 * DIM directions (offsets in registers);
-// plain: lottery winners are decided from the tickets alone (see lot_propose_kernel)
+// plain: lottery winners are decided from the tickets alone (see lot_propose_kernel); 2: and
+// the conflict shell is the radius-1 Moore shell
 template <int DIM, int H, int TX, int TY, int TZ, int NT = kBlock>
 static hipError_t launch_propose_t(int algorithm, int generic, int plain, const PassArgs &a, hipStream_t stream) {
     constexpr int KF = 2 * DIM;
@@ -2550,6 +2574,9 @@ static hipError_t launch_propose_t(int algorithm, int generic, int plain, const 
                                stream, a);
         else if (generic)
             CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, 0, NT, false>), dim3(a.ntiles), dim3(NT), 0,
+                               stream, a);
+        else if (plain == 2 && H == 1)
+            CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, true, H == 1>), dim3(a.ntiles), dim3(NT), 0,
                                stream, a);
         else if (plain)
             CPM_LAUNCH((lot_propose_kernel<DIM, H, TX, TY, TZ, KF, NT, true>), dim3(a.ntiles), dim3(NT), 0,
@@ -2601,9 +2628,10 @@ static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
 // Tile configurations: id & 15 -> (dim, halo, TX, TY, TZ); bit 4 of id selects the kernels
 // for proposal relations whose direction count differs from 2 * dim; bit 5 the row kernel
 // of 2D periodic von Neumann checkerboards with a 2 x 2 colour pattern; bit 6 the lottery
-// kernel for periodic obstacle-free lattices of at least 2 halo + 1 sites per axis.
+// kernel for periodic obstacle-free lattices of at least 2 halo + 1 sites per axis; bit 7,
+// with bit 6, its winner test over the radius-1 Moore conflict shell.
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz) {
-    if (id < 0 || id > 127) return 0;
+    if (id < 0 || id > 255) return 0;
     id &= 15;
     static const int table[][5] = {
         {2, 1, 128, 16, 1}, {2, 2, 128, 16, 1}, {3, 1, 32, 8, 8}, {3, 2, 32, 8, 8},
@@ -2627,7 +2655,7 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream) {
     using namespace cpm;
-    if (config < 0 || config > 127) return hipErrorInvalidValue;
+    if (config < 0 || config > 255) return hipErrorInvalidValue;
     if (config & 32) {
         // row kernels: checkerboard, periodic von Neumann proposal, 2 x 2 (x 2) colour pattern
         if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16)) return hipErrorInvalidValue;
@@ -2649,8 +2677,9 @@ extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, con
         return hipGetLastError();
     }
     const int g = (config >> 4) & 1;
-    const int plain = (config >> 6) & 1;
+    const int plain = (config >> 6) & 1 ? 1 + ((config >> 7) & 1) : 0;  // 2: Moore conflict shell
     if (plain && algorithm != CPM_ALG_LOTTERY) return hipErrorInvalidValue;
+    if ((config & 128) && (g || !plain)) return hipErrorInvalidValue;
     switch (config & 15) {
     case 0: return launch_propose_t<2, 1, 128, 16, 1>(algorithm, g, plain, *a, stream);
     case 1: return launch_propose_t<2, 2, 128, 16, 1>(algorithm, g, plain, *a, stream);

@@ -132,6 +132,21 @@ def test_cubes_3d_float_contact(gpu_lib, alg):
     run_parity(compiled, proposal, components, algorithm, seed, [1, 4])
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lottery_moore_shell_winner_test(gpu_lib, dim):
+    """Plain lotteries whose conflict shell is the radius-1 Moore shell (Moore or 26-neighbour
+    contact over a von Neumann proposal) take the winner test over compile-time ticket offsets
+    (kernel_config bit 7); results equal the oracle."""
+    if dim == 2:
+        model = workloads.scaled_2d(130, algorithm="lottery", seed=5)
+    else:
+        model = workloads.cubes_3d(L=24, side=6, algorithm="lottery", seed=5)
+    gpu, sim, reps = run_parity(*model, [1, 4])
+    assert gpu.kernel_config() & 64 and gpu.kernel_config() & 128
+    for rep in reps:
+        _partition(rep)
+
+
 def _walled_domain(alg_type, obstacles=True):
     L = 24
     wall = MediumOwner(7)
