@@ -239,6 +239,15 @@ def cubes_leg(steps: int, graph_chunk: int, shape: int = -1):
            "commit_kernel_avg_us": 1e3 * timing["commit_ms"] / max(timing["commit_launches"], 1),
            "propose_algorithmic_GBps": 4.0 * n / (1e-3 * timing["propose_ms"] / max(timing["propose_launches"], 1)) / 1e9,
            "last_report": rep._asdict()}
+    # FP64 energy bookkeeping over 20 more MCS (outside the timed region): the accepted copies'
+    # delta_h summed in FP64 against the change of the global FP64 energy
+    e0, acc = integ.total_energy(), 0.0
+    for _ in range(20):
+        integ.step(1)
+        acc += integ.accepted_energy()
+    e1 = integ.total_energy()
+    out["energy_fp64"] = {"mcs": 20, "E_start": e0, "E_end": e1, "accepted_dh_sum": acc,
+                          "drift": acc - (e1 - e0)}
     integ.close()
     return out
 
