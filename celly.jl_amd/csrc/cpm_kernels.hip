@@ -1155,23 +1155,51 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     };
     load_tile<DIM, H, TX, TY, TZ, NT>(d, x0, y0, z0, s_own, s_mut, PLAIN ? nullptr : s_sid);
     __syncthreads();
-    // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1);
-    // two independent draws per iteration keep two Philox chains in flight
+    // ticket word 1 of every mutable position (LotteryPriorityStream, subround = round - 1)
     if constexpr (PLAIN) {
-        // the tile and its halo only (the x pads beyond the halo are never read)
-        constexpr int WX = TX + 2 * H, WN = WX * G::LY * G::LZ;
-        for (int k = threadIdx.x; k < WN; k += 2 * NT) {
-            const int k2 = k + NT;
-            const int i = (G::PX - H) + k % WX + G::LX * (k / WX);
-            const int i2 = (G::PX - H) + k2 % WX + G::LX * (k2 / WX);
-            const uint32_t t1 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity,
-                                          sid(i), 0, 0).w0;
-            s_tk[i] = t1;
-            if (k2 < WN) {  // skipped by whole waves in the last iteration
-                const uint32_t t2 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0,
-                                              kSiteEntity, sid(i2), 0, 0).w0;
-                s_tk[i2] = t2;
+        auto ticket = [&](uint32_t site) __attribute__((always_inline)) {
+            return rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity, site, 0, 0).w0;
+        };
+        // The tile itself: on a tile that lies inside the lattice the ids follow from the tile
+        // coordinates (shifts, no wrap); a tile reaching past the upper faces wraps them.
+        const bool inside = x0 + TX <= X && y0 + TY <= Y && z0 + TZ <= Z;
+        for (int i = threadIdx.x; i < G::TN; i += NT) {
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+            const int pos = G::pos(ix, iy, iz);
+            const uint32_t site =
+                inside ? 1u + (uint32_t)(x0 + ix) + (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz))
+                       : sid(pos);
+            s_tk[pos] = ticket(site);
+        }
+        // The halo shell of the window (TX + 2H) x (TY + 2H) (x (TZ + 2H)): whole halo planes
+        // (3D), then per tile plane the H rows above and below and the H columns either side.
+        // The x pads beyond the halo are never read.
+        constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+        constexpr int PL = G::HZ > 0 ? 2 * G::HZ * WX * WY : 0;
+        constexpr int RG = 2 * H * WX + 2 * H * TY;
+        constexpr int HN = PL + TZ * RG;
+        for (int h = threadIdx.x; h < HN; h += NT) {
+            int wx, wy, wz;
+            if (PL > 0 && h < PL) {
+                const int p = h / (WX * WY), r = h % (WX * WY);
+                wz = p < G::HZ ? p : p + TZ;
+                wy = r / WX;
+                wx = r % WX;
+            } else {
+                const int q = h - PL, r = q % RG;
+                wz = G::HZ + q / RG;
+                if (r < 2 * H * WX) {
+                    const int row = r / WX;
+                    wx = r % WX;
+                    wy = row < H ? row : row + TY;
+                } else {
+                    const int c = (r - 2 * H * WX) / TY;
+                    wy = H + (r - 2 * H * WX) % TY;
+                    wx = c < H ? c : c + TX;
+                }
             }
+            const int pos = (G::PX - H + wx) + G::LX * (wy + G::LY * wz);
+            s_tk[pos] = ticket(sid(pos));
         }
     } else {
         for (int i = threadIdx.x; i < G::LN; i += 2 * NT) {
