@@ -265,6 +265,20 @@ struct Words {
     uint32_t w0, w1, w2, w3;
 };
 
+// a ^ b ^ c in one VALU instruction: v_bitop3_b32 with truth table 0x96 (odd parity of the
+// three input bits, whatever their order), which the compiler does not form for these XORs.
+// All three operands are vector registers: the round keys are loop-invariant and the compiler
+// hoists their copies out of the draw loops. (With the key as a scalar operand the compiler
+// put round keys in the scalar registers that the multiplies' carry-outs write, and results
+// were wrong.)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Rounds 1-9: two 32 x 32 -> 64-bit multiplies and two three-input XORs. Round 0 is left to
+// the compiler (the counter words other than the entity are often uniform there).
 __device__ __forceinline__ Words philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                           uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -275,8 +289,14 @@ __device__ __forceinline__ Words philox10(uint32_t c0, uint32_t c1, uint32_t c2,
         }
         const uint64_t p0 = (uint64_t)0xd2511f53u * c0;
         const uint64_t p1 = (uint64_t)0xcd9e8d57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n0, n2;
+        if (r == 0) {
+            n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+            n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        } else {
+            n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+            n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+        }
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
