@@ -1099,7 +1099,7 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     // workgroups per CU)
     __shared__ uint8_t s_mut[PLAIN ? 4 : G::LN];
     __shared__ uint32_t s_sid[PLAIN ? 1 : G::LN];
-    __shared__ uint32_t s_tk[G::LN];
+    __shared__ __align__(16) uint32_t s_tk[G::LN];
     __shared__ uint16_t s_work[G::TN];
     __shared__ int32_t s_ploff[KF > 0 ? KF : CPM_MAX_DIRS];
     __shared__ int32_t s_cloff[CPM_MAX_DIRS];
@@ -1218,82 +1218,128 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     uint32_t c01 = 0, c23 = 0;  // activated | same owner << 16, boundary null | realized << 16
     uint32_t nw = 0;
     uint16_t *my_work = s_work + wave * Q;
-    for (int i0 = 0; i0 < G::TN; i0 += NT) {
-        const int i = i0 + (int)threadIdx.x;
-        bool need = false;
-        if (i < G::TN) {
-            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
-            if (gx < X && gy < Y && gz < Z) {
-                const int pos = G::pos(ix, iy, iz);
-                if (PLAIN || s_mut[pos]) {
-                    // a tile site lies inside the lattice: its id needs no wrap
-                    const uint32_t site = PLAIN ? 1u + (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)
-                                                : s_sid[pos];
-                    const uint32_t mine = s_tk[pos];
-                    // strict lexicographic ticket maximum over the conflict neighbours
-                    // (lottery.jl:153-185): word 1 decides unless it ties, which needs the
-                    // full tickets (rare)
-                    bool beaten = false, tied = false;
-                    if constexpr (MOORE) {
-                        uint32_t mx = 0u;
+    // Full-ticket comparison against every tying conflict neighbour (rare): words 2-4 of the
+    // tickets, then the site id (lottery.jl:153-185).
+    auto tie_wins = [&](int pos, uint32_t site, uint32_t mine) __attribute__((always_inline)) {
+        bool wins = true;
+        const Words b4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity, site, 0, 0);
+        for (int k = 0; k < KC && wins; ++k) {
+            const int q = pos + s_cloff[k];
+            if (!PLAIN && !s_mut[q]) continue;
+            const uint32_t nsite = PLAIN ? sid(q) : s_sid[q];
+            if (nsite == site || s_tk[q] != mine) continue;
+            const Words a4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0, kSiteEntity, nsite, 0, 0);
+            bool better;
+            if (a4.w1 != b4.w1) better = a4.w1 > b4.w1;
+            else if (a4.w2 != b4.w2) better = a4.w2 > b4.w2;
+            else if (a4.w3 != b4.w3) better = a4.w3 > b4.w3;
+            else better = nsite < site;
+            if (better) wins = false;
+        }
+        return wins;
+    };
+    auto eligible_site = [&](uint32_t site) __attribute__((always_inline)) {
+        if (d.uniform_degree) return true;
+        const uint32_t degree = gp(d.site_degree)[site - 1];
+        return bounded_draw(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs, sub, 0, kSiteEntity, site, 0,
+                            const_bound(d.round_bound)) < degree + 1;
+    };
+    if constexpr (MOORE && DIM == 2) {
+        // Four consecutive sites of a row per lane: the three ticket rows over them in three
+        // 16-byte reads plus the columns either side. The neighbour maximum of site j is
+        // max(ud[j - 1], ud[j], ud[j + 1], mid[j - 1], mid[j + 1]) with ud[c] the larger of the
+        // tickets above and below column c.
+        static_assert(TX % 4 == 0 && G::PX % 4 == 0 && G::LX % 4 == 0, "quads are 16-byte aligned");
+        constexpr int QX = TX / 4, NQ = QX * TY;
+        for (int q0 = 0; q0 < NQ; q0 += NT) {
+            const int qd = q0 + (int)threadIdx.x;
+            uint32_t win4 = 0u;
+            int ix0 = 0, iy = 0;
+            if (qd < NQ) {
+                ix0 = 4 * (qd % QX);
+                iy = qd / QX;
+                const int gx0 = x0 + ix0, gy = y0 + iy;
+                const int pos0 = G::pos(ix0, iy, 0);
+                const v4i up = *(const v4i *)&s_tk[pos0 - G::LX];
+                const v4i md = *(const v4i *)&s_tk[pos0];
+                const v4i dn = *(const v4i *)&s_tk[pos0 + G::LX];
+                uint32_t ud[6], mid[6];
+                ud[0] = max(s_tk[pos0 - G::LX - 1], s_tk[pos0 + G::LX - 1]);
+                ud[5] = max(s_tk[pos0 - G::LX + 4], s_tk[pos0 + G::LX + 4]);
+                mid[0] = s_tk[pos0 - 1];
+                mid[5] = s_tk[pos0 + 4];
+                ud[1] = max((uint32_t)up.x, (uint32_t)dn.x);
+                ud[2] = max((uint32_t)up.y, (uint32_t)dn.y);
+                ud[3] = max((uint32_t)up.z, (uint32_t)dn.z);
+                ud[4] = max((uint32_t)up.w, (uint32_t)dn.w);
+                mid[1] = (uint32_t)md.x;
+                mid[2] = (uint32_t)md.y;
+                mid[3] = (uint32_t)md.z;
+                mid[4] = (uint32_t)md.w;
 #pragma unroll
-                        for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+                for (int j = 0; j < 4; ++j) {
+                    if (gx0 + j >= X || gy >= Y) continue;
+                    const uint32_t mine = mid[j + 1];
+                    const uint32_t mx = max(max(max(ud[j], ud[j + 1]), max(ud[j + 2], mid[j])), mid[j + 2]);
+                    bool wins = !(mx > mine);
+                    const uint32_t site = 1u + (uint32_t)(gx0 + j) + (uint32_t)X * (uint32_t)gy;
+                    if (wins && mx == mine) wins = tie_wins(pos0 + j, site, mine);
+                    if (wins && eligible_site(site)) win4 |= 1u << j;
+                }
+            }
 #pragma unroll
-                            for (int dy = -1; dy <= 1; ++dy)
+            for (int j = 0; j < 4; ++j)
+                wave_append((win4 >> j) & 1u, (uint16_t)(iy * TX + ix0 + j), nw, my_work);
+        }
+    } else {
+        for (int i0 = 0; i0 < G::TN; i0 += NT) {
+            const int i = i0 + (int)threadIdx.x;
+            bool need = false;
+            if (i < G::TN) {
+                const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
+                const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
+                if (gx < X && gy < Y && gz < Z) {
+                    const int pos = G::pos(ix, iy, iz);
+                    if (PLAIN || s_mut[pos]) {
+                        // a tile site lies inside the lattice: its id needs no wrap
+                        const uint32_t site = PLAIN ? 1u + (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)
+                                                    : s_sid[pos];
+                        const uint32_t mine = s_tk[pos];
+                        // strict lexicographic ticket maximum over the conflict neighbours
+                        // (lottery.jl:153-185): word 1 decides unless it ties, which needs the
+                        // full tickets (rare)
+                        bool beaten = false, tied = false;
+                        if constexpr (MOORE) {
+                            uint32_t mx = 0u;
 #pragma unroll
-                                for (int dx = -1; dx <= 1; ++dx)
-                                    if (dx != 0 || dy != 0 || dz != 0)
-                                        mx = max(mx, s_tk[pos + dx + G::LX * (dy + G::LY * dz)]);
-                        beaten = mx > mine;
-                        tied = mx == mine;  // a neighbour ties and none is higher
-                    } else {
+                            for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+#pragma unroll
+                                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                                    for (int dx = -1; dx <= 1; ++dx)
+                                        if (dx != 0 || dy != 0 || dz != 0)
+                                            mx = max(mx, s_tk[pos + dx + G::LX * (dy + G::LY * dz)]);
+                            beaten = mx > mine;
+                            tied = mx == mine;  // a neighbour ties and none is higher
+                        } else {
 #pragma unroll 8
-                        for (int k = 0; k < KC; ++k) {
-                            const int q = pos + s_cloff[k];
-                            const bool other_site = PLAIN || (s_mut[q] && s_sid[q] != site);
-                            const uint32_t other = s_tk[q];
-                            beaten |= other_site && other > mine;
-                            tied |= other_site && other == mine;
+                            for (int k = 0; k < KC; ++k) {
+                                const int q = pos + s_cloff[k];
+                                const bool other_site = PLAIN || (s_mut[q] && s_sid[q] != site);
+                                const uint32_t other = s_tk[q];
+                                beaten |= other_site && other > mine;
+                                tied |= other_site && other == mine;
+                            }
                         }
-                    }
-                    bool wins = !beaten;
-                    if (wins && tied) {
-                        const Words b4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority, mcs, sub, 0,
-                                                   kSiteEntity, site, 0, 0);
-                        for (int k = 0; k < KC && wins; ++k) {
-                            const int q = pos + s_cloff[k];
-                            if (!PLAIN && !s_mut[q]) continue;
-                            const uint32_t nsite = PLAIN ? sid(q) : s_sid[q];
-                            if (nsite == site || s_tk[q] != mine) continue;
-                            const Words a4 = rng_words(d.key_lo, d.key_hi, kStreamLotteryPriority,
-                                                       mcs, sub, 0, kSiteEntity, nsite, 0, 0);
-                            bool better;
-                            if (a4.w1 != b4.w1) better = a4.w1 > b4.w1;
-                            else if (a4.w2 != b4.w2) better = a4.w2 > b4.w2;
-                            else if (a4.w3 != b4.w3) better = a4.w3 > b4.w3;
-                            else better = nsite < site;
-                            if (better) wins = false;
-                        }
-                    }
-                    if (wins) {
-                        bool eligible = true;
-                        if (!d.uniform_degree) {
-                            const uint32_t degree = gp(d.site_degree)[site - 1];
-                            eligible = bounded_draw(d.key_lo, d.key_hi, kStreamLotteryActivation, mcs,
-                                                    sub, 0, kSiteEntity, site, 0, const_bound(d.round_bound)) < degree + 1;
-                        }
-                        if (eligible) {
-                            c01 += 1u;
-                            const int32_t own = s_own[pos];
-                            if (interior_site<KF, PLAIN>(s_own, s_mut, pos, own, po, K, s_ploff)) c01 += 1u << 16;
-                            else need = true;
-                        }
+                        bool wins = !beaten;
+                        if (wins && tied) wins = tie_wins(pos, site, mine);
+                        // activated: counted, classified and drawn for in the dense pass below
+                        if (wins) need = eligible_site(site);
                     }
                 }
             }
+            wave_append(need, (uint16_t)(i < G::TN ? i : 0), nw, my_work);
         }
-        wave_append(need, (uint16_t)(i < G::TN ? i : 0), nw, my_work);
     }
     if (__lane_id() == 0) s_len[wave] = nw;
     __syncthreads();
@@ -1308,10 +1354,21 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
         const uint32_t e = e0 + threadIdx.x;
         bool act = false;
         v4i v = v4i{0, 0, 0, 0};
+        int pos = 0;
+        bool drawn = false;
         if (e < nwork) {
             const int i = s_work[list_slot<Q>(e, p1, p2, p3)];
             const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
-            const int pos = G::pos(ix, iy, iz);
+            pos = G::pos(ix, iy, iz);
+            c01 += 1u;  // activated
+            // a winner whose proposal neighbours all share its owner is a same-owner no-op
+            // whatever direction it would draw
+            if (interior_site<KF, PLAIN>(s_own, s_mut, pos, s_own[pos], po, K, s_ploff)) c01 += 1u << 16;
+            else drawn = true;
+        }
+        if (drawn) {
+            const int i = s_work[list_slot<Q>(e, p1, p2, p3)];
+            const int ix = i % TX, iy = (i / TX) % TY, iz = i / (TX * TY);
             const uint32_t site = PLAIN ? 1u + (uint32_t)(x0 + ix) +
                                               (uint32_t)X * ((uint32_t)(y0 + iy) + (uint32_t)Y * (uint32_t)(z0 + iz))
                                         : s_sid[pos];
