@@ -2573,7 +2573,18 @@ __global__ __launch_bounds__(kBlock) void report_kernel(const RepDev *reps, cons
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void energy_partials_kernel(const RepDev *reps, int slot, double *partials) {
     __shared__ double s_part[kBlock];
-    const RepDev &d = reps[slot];
+    __shared__ uint8_t s_keep[CPM_MAX_DIRS];  // direction k is not after its opposite
+    const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
+    if ((int)threadIdx.x < d.cont.n) {
+        const int k = (int)threadIdx.x;
+        int opp = k;  // index of the opposite offset (k itself when the relation lacks it)
+        for (int e = 0; e < d.cont.n; ++e)
+            if (d.cont.off[e][0] == -d.cont.off[k][0] && d.cont.off[e][1] == -d.cont.off[k][1] &&
+                d.cont.off[e][2] == -d.cont.off[k][2])
+                opp = e;
+        s_keep[k] = k <= opp ? 1 : 0;
+    }
+    __syncthreads();
     bool has_vol = false, has_con = false, has_bnd = false;
     for (int t = 0; t < d.n_energies; ++t) {
         has_vol |= d.energy_order[t] == CPM_TERM_VOLUME;
@@ -2596,7 +2607,45 @@ __global__ __launch_bounds__(kBlock) void energy_partials_kernel(const RepDev *r
             }
         }
     }
-    if (has_con) {
+    if (has_con && d.simple) {
+        // periodic lattice without obstacles: every position is a mutable site, neighbours wrap
+        // once; eight neighbour loads in flight per step
+        const int nc = d.cont.n;
+        const uint32_t nt = (uint32_t)d.n_types;
+        const int X = d.dims[0], Y = d.dims[1], Z = d.dims[2];
+        const CPM_GLOBAL int32_t *lat = gp(d.lattice);
+        for (uint32_t i = first; i < d.nsites; i += stride) {
+            const int x = (int)(i % (uint32_t)X);
+            const int y = (int)((i / (uint32_t)X) % (uint32_t)Y);
+            const int z = (int)(i / ((uint32_t)X * (uint32_t)Y));
+            const int32_t o = lat[i];
+            const uint32_t ot = owner_type(d, o);
+            for (int k0 = 0; k0 < nc; k0 += 8) {
+                int32_t w[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = k0 + j;
+                    w[j] = o;
+                    if (k < nc) {
+                        int gx = x + d.cont.off[k][0], gy = y + d.cont.off[k][1], gz = z + d.cont.off[k][2];
+                        gx += gx < 0 ? X : 0;
+                        gx -= gx >= X ? X : 0;
+                        gy += gy < 0 ? Y : 0;
+                        gy -= gy >= Y ? Y : 0;
+                        gz += gz < 0 ? Z : 0;
+                        gz -= gz >= Z ? Z : 0;
+                        w[j] = lat[(uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz)];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = k0 + j;
+                    if (k < nc && w[j] != 0 && w[j] != o && s_keep[k])
+                        acc += d.cont.wd[k] * d.Jd[(ot - 1) * nt + (owner_type(d, w[j]) - 1)];
+                }
+            }
+        }
+    } else if (has_con) {
         const int nc = d.cont.n;
         const uint32_t nt = (uint32_t)d.n_types;
         for (uint32_t i = first; i < d.nsites; i += stride) {
@@ -2610,14 +2659,7 @@ __global__ __launch_bounds__(kBlock) void energy_partials_kernel(const RepDev *r
                 const Realized nb = realize_position(d, x + d.cont.off[k][0], y + d.cont.off[k][1],
                                                      z + d.cont.off[k][2]);
                 if (nb.word == 0 || nb.word == o) continue;
-                if (nb.site && nb.mut) {
-                    int opp = k;  // index of the opposite offset
-                    for (int e = 0; e < nc; ++e)
-                        if (d.cont.off[e][0] == -d.cont.off[k][0] && d.cont.off[e][1] == -d.cont.off[k][1] &&
-                            d.cont.off[e][2] == -d.cont.off[k][2])
-                            opp = e;
-                    if (k > opp) continue;
-                }
+                if (nb.site && nb.mut && !s_keep[k]) continue;  // the pair is counted from the other site
                 acc += d.cont.wd[k] * d.Jd[(ot - 1) * nt + (owner_type(d, nb.word) - 1)];
             }
         }
