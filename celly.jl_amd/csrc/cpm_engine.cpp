@@ -1557,7 +1557,7 @@ struct Engine {
                     hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
                     advance_mechanics(g, (uint32_t)p, 1);
                 }
-                metrics[0] += 3;
+                metrics[0] += 2;  // propose, commit
                 g.pass_counter++;
             }
         }
