@@ -121,6 +121,29 @@ class ReplicaBatch:
         spec = build_spec(state, proposal_relation, components, algorithm, seed)
         return self.add(state, spec, target_mcs)
 
+    def checkpoint(self, slot: int):
+        """CanonicalCheckpoint of one replica at its last completed MCS (synchronizes): the
+        state, seed and MCS counter from which resume() continues it exactly."""
+        from .checkpoint import capture_batch_checkpoint
+        return capture_batch_checkpoint(self, slot)
+
+    def resume(self, slot: int, cp, prototype_compiled: CompiledScientificState,
+               target_mcs: Optional[int] = None):
+        """Load a reserved replica from a checkpoint of its model (persistence/scientific.jl:
+        464-468): ownership, trackers rebuilt from it, mechanical states as captured, the
+        checkpoint's MCS counter and the reserved spec's seed (the checkpoint's own)."""
+        from .checkpoint import CheckpointCompatibilityError, checkpoint_compiled_state, \
+            model_fingerprint, topology_fingerprint
+        spec = self.specs[slot]
+        if cp.topology_fingerprint != topology_fingerprint(spec) or cp.model_fingerprint != model_fingerprint(spec):
+            raise CheckpointCompatibilityError("model", "replica model fingerprint", "checkpoint model fingerprint")
+        if int(cp.seed) != int(spec.seed):
+            raise CheckpointCompatibilityError("seed", str(spec.seed), str(cp.seed))
+        self.load(slot, checkpoint_compiled_state(cp, prototype_compiled), initialize_mechanics=False)
+        self.set_mcs(slot, int(cp.mcs))
+        if target_mcs is not None:
+            self.set_target(slot, target_mcs)
+
     def remove(self, slot: int):
         self._check(self.lib.cpm_batch_remove(self._b, int(slot)))
         self.specs.pop(slot, None)

@@ -10,12 +10,21 @@ memory for the next admissions, so jobs that arrive while others run join the ru
 Each job's trajectory depends only on its own state, seed and MCS counter, so its result
 equals a solo run of the same model (ScientificPottsIntegrator or the reference's
 init_scientific + step!).
+
+Pre-emption (`preempt_after_s`): when the head of the queue has waited that long without
+fitting, running jobs are evicted, largest first, until it fits. An evicted job is captured as
+a CanonicalCheckpoint at its last completed MCS (persistence/scientific.jl:250-292), removed
+(its device memory returns to the budget) and queued again; on re-admission it is restored from
+the checkpoint (state, seed, MCS counter: :464-468) and continues exactly where it stopped, so
+its result is still the solo run's. A job is evicted at most once and a resumed job never
+evicts others, so every job finishes.
 """
 from __future__ import annotations
 
 import collections
+import dataclasses
 import time
-from typing import Callable, Dict, List, NamedTuple, Optional
+from typing import Any, Callable, Dict, List, NamedTuple, Optional
 
 import numpy as np
 
@@ -41,10 +50,20 @@ class Completion(NamedTuple):
     attempts: int           # activated attempts over the job (mutable sites x MCS)
     state: dict             # downloaded arrays (cpm_download_state order)
     report: tuple           # ScientificMCSReport of the job's last MCS
+    preemptions: int = 0    # times the job was evicted and resumed from a checkpoint
 
     @property
     def latency_s(self) -> float:
         return self.finished_s - self.arrival_s
+
+
+@dataclasses.dataclass
+class _Job:
+    r: Request
+    queued_s: float                   # when the job (re-)entered the queue
+    admitted_s: Optional[float] = None  # first admission
+    cp: Any = None                    # checkpoint of an evicted job
+    preemptions: int = 0
 
 
 def make_request(key, arrival_s: float, state, proposal_relation, components, algorithm,
@@ -59,14 +78,16 @@ def make_request(key, arrival_s: float, state, proposal_relation, components, al
 class JobServer:
     """Continuous-batching server over one ReplicaBatch.
 
-    `batch` is normally a ReplicaBatch; anything with the same add / remove / set_target /
-    step / progress / download / report methods works (the scheduler is tested on the host
-    with a stand-in)."""
+    `batch` is normally a ReplicaBatch; anything with the same reserve / load / remove /
+    step / progress / download / report (and, for pre-emption, checkpoint / resume) methods
+    works (the scheduler is tested on the host with a stand-in). `preempt_after_s`: None
+    disables pre-emption."""
 
     def __init__(self, hbm_budget_bytes: int, chunk_mcs: int = 50, algorithm: int = ALG_CHECKERBOARD,
                  batch=None, bytes_of: Callable[[ModelSpec], int] = replica_bytes,
                  graph_chunk: Optional[int] = None, clock: Callable[[], float] = time.perf_counter,
-                 sleep: Callable[[float], None] = time.sleep, shapes=None):
+                 sleep: Callable[[float], None] = time.sleep, shapes=None,
+                 preempt_after_s: Optional[float] = None):
         if chunk_mcs < 1:
             raise ValueError("chunk_mcs must be positive")
         self.budget = int(hbm_budget_bytes)
@@ -83,6 +104,35 @@ class JobServer:
         self.peak_used = 0
         self.peak_jobs = 0
         self.steps = 0
+        self.preempt_after = None if preempt_after_s is None else float(preempt_after_s)
+        self.preemptions = 0
+
+    def _preempt(self, need: int, running: Dict[int, "_Job"], queue, now: float, size) -> bool:
+        """Evict running jobs, largest first, until `need` bytes fit; False (nothing evicted)
+        when the evictable jobs cannot make the room. Synchronizes with the running step."""
+        deficit = self.used + need - self.budget
+        progress = self.batch.progress()
+        victims, freed = [], 0
+        for slot in sorted(running, key=lambda s: (-size[id(running[s].r)], s)):
+            job = running[slot]
+            if job.preemptions or job.cp is not None or progress[slot] >= job.r.mcs:
+                continue  # evicted before, resumed, or finishing with this step
+            victims.append(slot)
+            freed += size[id(job.r)]
+            if freed >= deficit:
+                break
+        if freed < deficit:
+            return False
+        for slot in victims:
+            job = running.pop(slot)
+            job.cp = self.batch.checkpoint(slot)
+            self.batch.remove(slot)
+            self.used -= size[id(job.r)]
+            job.preemptions += 1
+            job.queued_s = now
+            self.preemptions += 1
+            queue.append(job)
+        return True
 
     def serve(self, requests: List[Request]) -> List[Completion]:
         """Runs the stream to completion; returns completions in finishing order.
@@ -98,48 +148,61 @@ class JobServer:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival_s))
         queue: collections.deque = collections.deque()
         reserved: List[tuple] = []
-        running: Dict[int, tuple] = {}
+        running: Dict[int, _Job] = {}
         done: List[Completion] = []
         inflight = False
         t0 = self.clock()
         while pending or queue or running or reserved:
             now = self.clock() - t0
             while pending and pending[0].arrival_s <= now:
-                queue.append(pending.popleft())
+                r = pending.popleft()
+                queue.append(_Job(r, r.arrival_s))
             # admission: the head of the queue joins as soon as it fits (first come, first served)
             while queue:
-                r = queue[0]
-                need = size[id(r)]
+                job = queue[0]
+                need = size[id(job.r)]
                 if self.used + need > self.budget:
+                    if (self.preempt_after is not None and job.cp is None and running
+                            and now - job.queued_s >= self.preempt_after
+                            and self._preempt(need, running, queue, now, size)):
+                        inflight = False  # the eviction synchronized with the step
+                        continue
                     break
                 try:
-                    slot = self.batch.reserve(r.spec)
+                    slot = self.batch.reserve(job.r.spec)
                 except EngineError as e:
                     if e.code == CPM_E_NOMEM and (running or reserved):
                         break  # the engine's own accounting says wait for a completion
                     raise
                 queue.popleft()
                 self.used += need
-                reserved.append((slot, r, self.clock() - t0, need))
-            if inflight:
+                if job.admitted_s is None:
+                    job.admitted_s = self.clock() - t0
+                reserved.append((slot, job))
+            if inflight or running:
                 progress = self.batch.progress()
                 inflight = False
                 for slot in sorted(running):
-                    r, admitted, need = running[slot]
+                    job = running[slot]
+                    r = job.r
                     if progress[slot] >= r.mcs:
                         state = self.batch.download(slot)
                         report = tuple(self.batch.report(slot))
                         finished = self.clock() - t0
                         self.batch.remove(slot)
-                        self.used -= need
+                        self.used -= size[id(r)]
                         del running[slot]
                         mutable = int(np.count_nonzero(r.spec.mutable_mask)) if r.spec.mutable_mask is not None \
                             else int(np.prod(r.spec.dims))
-                        done.append(Completion(r.key, r.arrival_s, admitted, finished, r.mcs,
-                                               mutable * r.mcs, state, report))
-            for slot, r, admitted, need in reserved:
-                self.batch.load(slot, r.compiled, target_mcs=r.mcs)
-                running[slot] = (r, admitted, need)
+                        done.append(Completion(r.key, r.arrival_s, job.admitted_s, finished, r.mcs,
+                                               mutable * r.mcs, state, report, job.preemptions))
+            for slot, job in reserved:
+                if job.cp is None:
+                    self.batch.load(slot, job.r.compiled, target_mcs=job.r.mcs)
+                else:
+                    self.batch.resume(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
+                    job.cp = None
+                running[slot] = job
             reserved = []
             self.peak_used = max(self.peak_used, self.used)
             self.peak_jobs = max(self.peak_jobs, len(running))

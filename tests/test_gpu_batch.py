@@ -137,6 +137,30 @@ def test_served_jobs_equal_solo_runs(gpu_lib, graph_chunk):
     assert s["jobs"] == len(jobs) and s["p99_s"] >= s["p50_s"] > 0
 
 
+@pytest.mark.parametrize("alg", ["checkerboard", "lottery"])
+def test_preempted_jobs_equal_solo_runs(gpu_lib, alg):
+    """A running job is evicted through a checkpoint when the head of the queue cannot fit the
+    budget beside it, and resumed from its checkpointed state, seed and MCS counter once the
+    head finishes: every job, evicted or not, still equals the oracle's solo run."""
+    models = {0: workloads.scaled_2d(120, algorithm=alg, seed=ensemble_seed(31, 1)),
+              1: workloads.block_cells(96, 2, 10, 12, ntypes=2, J=workloads.J_TWO_TYPES,
+                                       temperature=np.float32(12.0), seed=ensemble_seed(31, 2), algorithm=alg),
+              2: workloads.scaled_2d(60, algorithm=alg, seed=ensemble_seed(31, 3))}
+    mcs = {0: 30, 1: 9, 2: 7}
+    reqs = [make_request(k, 0.0, *models[k], mcs=mcs[k]) for k in (0, 1, 2)]
+    sizes = [replica_bytes(r.spec) for r in reqs]
+    # job 0 alone holds most of the budget: jobs 1 and 2 fit only after it leaves
+    server = JobServer(max(sizes) + min(sizes[1:]) // 2, chunk_mcs=3, preempt_after_s=0.0,
+                       algorithm=ALG_CHECKERBOARD if alg == "checkerboard" else ALG_LOTTERY)
+    done = server.serve(reqs)
+    by = {c.key: c for c in done}
+    assert sorted(by) == [0, 1, 2]
+    assert by[0].preemptions == 1 and server.preemptions >= 1
+    assert by[1].finished_s < by[0].finished_s
+    for c in done:
+        _check(c.state, c.report, _solo(models[c.key], c.mcs), f"job {c.key}")
+
+
 def test_batch_shapes_per_size_class_keep_results(gpu_lib):
     """Tile shapes tuned per size class (replicas of different shapes run in different launch
     groups) change speed only: the batch still equals the oracle's solo runs."""

@@ -51,6 +51,15 @@ class FakeBatch:
     def remove(self, slot):
         del self.mcs[slot], self.target[slot], self.bytes[slot]
 
+    def checkpoint(self, slot):
+        self.log.append(("checkpoint", slot, self.mcs[slot], self.clock()))
+        return {"mcs": self.mcs[slot]}
+
+    def resume(self, slot, cp, compiled, target_mcs=None):
+        self.mcs[slot] = cp["mcs"]
+        self.target[slot] = target_mcs
+        self.log.append(("resume", compiled, self.clock()))
+
     def step(self, n):
         for s in self.mcs:
             self.mcs[s] = min(self.target[s], self.mcs[s] + n)
@@ -70,10 +79,10 @@ def _requests(specs, arrivals, mcs):
     return [Request(k, a, k, s, m) for k, (s, a, m) in enumerate(zip(specs, arrivals, mcs))]
 
 
-def _server(clock, budget, chunk=10, engine_budget=None):
+def _server(clock, budget, chunk=10, engine_budget=None, preempt_after_s=None):
     batch = FakeBatch(clock, engine_budget)
     return JobServer(budget, chunk, batch=batch, bytes_of=lambda s: s.nbytes, clock=clock,
-                     sleep=clock.sleep), batch
+                     sleep=clock.sleep, preempt_after_s=preempt_after_s), batch
 
 
 def test_every_job_completes_once_with_its_mcs():
@@ -142,3 +151,46 @@ def test_latency_summary():
     s = latency_summary(done)
     assert s["jobs"] == 10 and s["p50_s"] <= s["p99_s"] <= s["max_s"]
     assert s["attempts"] == 100 * 50 * 10
+
+
+def test_preemption_admits_a_blocked_head_and_resumes_exactly():
+    """A long job holding most of the budget is evicted through a checkpoint when the head of
+    the queue has waited preempt_after_s; it resumes from its checkpointed MCS afterwards and
+    still performs exactly the MCS it asked for."""
+    clock = FakeClock()
+    server, batch = _server(clock, budget=100, chunk=10, preempt_after_s=0.005)
+    specs = [FakeSpec(10, 80), FakeSpec(10, 50)]
+    done = server.serve(_requests(specs, [0.0, 0.0015], [1000, 30]))
+    by = {c.key: c for c in done}
+    assert by[0].state["mcs"] == 1000 and by[1].state["mcs"] == 30
+    assert by[0].preemptions == 1 and by[1].preemptions == 0 and server.preemptions == 1
+    assert by[1].finished_s < by[0].finished_s
+    assert by[1].admitted_s - by[1].arrival_s < 0.02  # not held until job 0 finishes
+    ck = [e for e in batch.log if e[0] == "checkpoint"]
+    rs = [e for e in batch.log if e[0] == "resume"]
+    assert len(ck) == 1 and len(rs) == 1 and 0 < ck[0][2] < 1000
+    assert server.peak_used <= 100
+
+
+def test_without_preemption_the_head_waits():
+    clock = FakeClock()
+    server, _ = _server(clock, budget=100, chunk=10)
+    done = server.serve(_requests([FakeSpec(10, 80), FakeSpec(10, 50)], [0.0, 0.0015], [1000, 30]))
+    by = {c.key: c for c in done}
+    assert by[1].admitted_s >= by[0].finished_s and server.preemptions == 0
+
+
+def test_preemption_cannot_ping_pong():
+    """Evicted jobs are never evicted again and resumed jobs never evict: a stream of large jobs
+    that cannot share the budget still finishes, each job evicted at most once."""
+    clock = FakeClock()
+    server, batch = _server(clock, budget=100, chunk=10, preempt_after_s=0.0)
+    specs = [FakeSpec(10, 60) for _ in range(6)]
+    mcs = [200, 150, 120, 90, 60, 30]
+    done = server.serve(_requests(specs, [0.001 * k for k in range(6)], mcs))
+    assert sorted(c.key for c in done) == list(range(6))
+    for c in done:
+        assert c.state["mcs"] == mcs[c.key]
+        assert c.preemptions <= 1
+    assert server.peak_used <= 100
+    assert not batch.mcs
