@@ -196,13 +196,27 @@ struct RepDev {
     unsigned long long *tiled_counts;  // realized, same owner, boundary null, rejected, accepted
 };
 
-// Per-replica dynamic state, updated on the device by the begin-MCS kernel.
+// Per-replica dynamic state, updated on the device by the begin-MCS kernel (one MCS at a time)
+// or by the plan and end kernels around a planned chunk of MCS.
 struct RepDyn {
     uint64_t mcs;      // MCS being performed (after begin) / last completed
     uint64_t target;   // replica advances while mcs < target
     uint32_t active;   // performing an MCS in the current step
     uint32_t in_use;
     uint32_t order[kMaxRounds];
+};
+
+// What one replica does in the k-th MCS of a planned chunk: the MCS number (1-based, as in the
+// RNG addresses), whether it performs it (in use and below its target) and that MCS's colour or
+// round order. A plan depends only on the replica's MCS counter, target and key at the start of
+// the chunk, so one launch writes the plans of a whole chunk; the pass kernels of its k-th MCS
+// read plan k instead of a per-MCS state update.
+constexpr int kPlanMcs = 64;  // MCS one plan launch covers
+struct McsPlan {
+    uint64_t mcs;
+    uint32_t active;
+    uint32_t pad;
+    uint16_t order[kMaxRounds];
 };
 
 // One actionable copy attempt staged for the claim-resolution / commit kernel.
@@ -242,6 +256,8 @@ struct PassArgs {
     uint32_t parity;            // claim buffer used by this pass
     uint32_t total_claims;
     uint32_t fast_eval;         // RepDev::fast_eval shared by every replica of the group (0: mixed)
+    const McsPlan *plan;        // [slot][kPlanMcs]
+    uint32_t plan_k;            // MCS of the chunk this pass belongs to
 };
 
 // Device buffers are reached through pointers stored in RepDev; casting them to the
@@ -369,9 +385,10 @@ __device__ __forceinline__ float uniform_open01(uint32_t w0) {
 }
 
 // Fisher-Yates schedule permutation (checkerboard.jl:226-242, lottery.jl:124-140).
+template <class T>
 __device__ inline void schedule_order(uint32_t key_lo, uint32_t key_hi, uint32_t stream,
                                       uint32_t operation, uint64_t mcs, int count,
-                                      uint32_t *order) {
+                                      T *order) {
     constexpr int kRegs = 16;
     if (count <= kRegs) {
         // the permutation in registers (swaps by selects), stored once: no store-to-load
@@ -394,15 +411,15 @@ __device__ inline void schedule_order(uint32_t key_lo, uint32_t key_hi, uint32_t
         }
 #pragma unroll
         for (int i = 0; i < kRegs; ++i)
-            if (i < count) order[i] = p[i];
+            if (i < count) order[i] = (T)p[i];
         return;
     }
-    for (int i = 0; i < count; ++i) order[i] = (uint32_t)(i + 1);
+    for (int i = 0; i < count; ++i) order[i] = (T)(i + 1);
     for (int c = count; c >= 2; --c) {
         const uint32_t draw = (uint32_t)(count - c);
         const uint32_t sel = bounded_uint(key_lo, key_hi, stream, mcs, 0, operation,
                                           kGlobalEntity, 0, draw, (uint32_t)c) + 1;
-        const uint32_t t = order[c - 1];
+        const T t = order[c - 1];
         order[c - 1] = order[sel - 1];
         order[sel - 1] = t;
     }

@@ -20,9 +20,9 @@
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz);
 extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const cpm::PassArgs *a,
                                                hipStream_t stream);
-extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, const cpm::McsPlan *plan, int slot,
                                                        uint32_t capacity, uint32_t pass, uint32_t phase,
-                                                       hipStream_t stream);
+                                                       uint32_t plan_k, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_mech_init(const cpm::RepDev *reps, int slot, uint32_t capacity,
                                                     hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
@@ -34,8 +34,12 @@ extern "C" hipError_t cpm_internal_launch_tiled_batch(const cpm::RepDev *reps, c
 extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
                                                           uint32_t capacity, hipStream_t stream);
 extern "C" void cpm_internal_time_next(hipEvent_t start, hipEvent_t stop);
-extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
+extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn, cpm::McsPlan *plan,
                                                 int nslots, hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_plan(const cpm::RepDev *reps, const cpm::RepDyn *dyn, cpm::McsPlan *plan,
+                                               int nslots, int K, hipStream_t stream);
+extern "C" hipError_t cpm_internal_launch_plan_end(const cpm::RepDev *reps, cpm::RepDyn *dyn,
+                                                   const cpm::McsPlan *plan, int nslots, int K, hipStream_t stream);
 extern "C" hipError_t cpm_internal_launch_set_dyn(cpm::RepDyn *dyn, int slot, uint64_t mcs,
                                                   uint64_t target, uint32_t in_use, uint32_t mask,
                                                   hipStream_t stream);
@@ -1210,6 +1214,7 @@ struct Engine {
     std::vector<Replica *> reps;
     cpm::RepDev *d_reps = nullptr;
     cpm::RepDyn *d_dyn = nullptr;
+    cpm::McsPlan *d_plan = nullptr;  // [slot][kPlanMcs]
     int slot_cap = 0;
     std::vector<Group> groups;
     unsigned long long *d_report = nullptr;
@@ -1233,6 +1238,7 @@ struct Engine {
         for (Group &g : groups) g.release();
         dev_free(d_reps);
         dev_free(d_dyn);
+        dev_free(d_plan);
         dev_free(d_report);
         if (h_report) (void)hipHostFree(h_report);
         dev_free(d_many);
@@ -1257,7 +1263,7 @@ struct Engine {
     }
 
     uint64_t bytes() const {
-        uint64_t b = (uint64_t)slot_cap * (sizeof(cpm::RepDev) + sizeof(cpm::RepDyn));
+        uint64_t b = (uint64_t)slot_cap * (sizeof(cpm::RepDev) + sizeof(cpm::RepDyn) + cpm::kPlanMcs * sizeof(cpm::McsPlan));
         for (const Replica *r : reps)
             if (r) b += r->bytes;
         for (const Group &g : groups) b += g.bytes;
@@ -1282,8 +1288,12 @@ struct Engine {
         hip_check(hipStreamSynchronize(stream), "sync");
         dev_free(d_reps);
         dev_free(d_dyn);
+        dev_free(d_plan);
         d_reps = nr;
         d_dyn = nd;
+        // plans are written at the start of every chunk: nothing to carry over
+        d_plan = nullptr;
+        hip_check(hipMalloc(&d_plan, (size_t)cap * cpm::kPlanMcs * sizeof(cpm::McsPlan)), "hipMalloc plans");
         slot_cap = cap;
         reps.resize((size_t)cap, nullptr);
         drop_graph();
@@ -1495,8 +1505,10 @@ struct Engine {
             if (g.dirty) rebuild(g);
     }
 
-    cpm::PassArgs args(const Group &g, uint32_t pass, uint32_t parity) const {
+    cpm::PassArgs args(const Group &g, uint32_t pass, uint32_t parity, uint32_t plan_k) const {
         cpm::PassArgs a{};
+        a.plan = d_plan;
+        a.plan_k = plan_k;
         a.reps = d_reps;
         a.dyn = d_dyn;
         a.tiles = g.d_tiles;
@@ -1530,11 +1542,16 @@ struct Engine {
         cpm_internal_time_next(ev[0], ev[1]);
     }
 
-    // Enqueue one MCS for every active replica (begin + 2 launches per pass).
-    void enqueue_mcs() {
-        mark(2);
-        hip_check(cpm_internal_launch_begin(d_reps, d_dyn, slot_cap, stream), "begin_mcs");
-        metrics[0]++;
+    // MCS are planned in chunks (plan kernel, the chunk's passes, end kernel) unless a
+    // TiledCheckerboardCPM group needs its per-MCS counter reset by the begin kernel.
+    bool chunk_plans() const {
+        for (const Group &g : groups)
+            if (g.ntiles && g.tiled()) return false;
+        return true;
+    }
+
+    // The colour passes of every non-tiled group for the plan_k-th MCS of the current chunk.
+    void enqueue_passes(uint32_t plan_k) {
         for (Group &g : groups) {
             if (g.ntiles == 0) continue;
             if (g.tiled()) {
@@ -1542,24 +1559,52 @@ struct Engine {
                 continue;
             }
             for (int p = 0; p < g.max_rounds; ++p) {
-                const cpm::PassArgs a = args(g, (uint32_t)p, (uint32_t)(g.pass_counter & 1));
+                const cpm::PassArgs a = args(g, (uint32_t)p, (uint32_t)(g.pass_counter & 1), plan_k);
                 // mechanics: half-interval OU steps before and after the pass
                 // (checkerboard.jl:522-557, lottery.jl:353-391); timed with the adjacent kernel
                 if (timing_events) {
-                    advance_mechanics(g, (uint32_t)p, 0);
+                    advance_mechanics(g, (uint32_t)p, 0, plan_k);
                     mark(0);
                     hip_check(cpm_internal_launch_propose(g.config, algorithm, &a, stream), "propose");
                     mark(1);
                     hip_check(cpm_internal_launch_commit(&a, stream), "commit");
-                    advance_mechanics(g, (uint32_t)p, 1);
+                    advance_mechanics(g, (uint32_t)p, 1, plan_k);
                 } else {
-                    advance_mechanics(g, (uint32_t)p, 0);
+                    advance_mechanics(g, (uint32_t)p, 0, plan_k);
                     hip_check(cpm_internal_launch_pass(g.config, algorithm, &a, stream), "pass kernels");
-                    advance_mechanics(g, (uint32_t)p, 1);
+                    advance_mechanics(g, (uint32_t)p, 1, plan_k);
                 }
                 metrics[0] += 2;  // propose, commit
                 g.pass_counter++;
             }
+        }
+    }
+
+    // One MCS for every active replica: begin (mcs + 1, order) + the passes.
+    void enqueue_mcs() {
+        mark(2);
+        hip_check(cpm_internal_launch_begin(d_reps, d_dyn, d_plan, slot_cap, stream), "begin_mcs");
+        metrics[0]++;
+        enqueue_passes(0);
+    }
+
+    // n MCS: chunks of at most kPlanMcs MCS, each one plan launch, its passes and one end launch
+    // (the counters advance at the end), or one begin launch per MCS where a tiled group runs.
+    void enqueue_steps(uint64_t n) {
+        if (!chunk_plans()) {
+            for (uint64_t i = 0; i < n; ++i) enqueue_mcs();
+            return;
+        }
+        while (n > 0) {
+            const int K = (int)std::min<uint64_t>(n, (uint64_t)cpm::kPlanMcs);
+            mark(2);
+            hip_check(cpm_internal_launch_plan(d_reps, d_dyn, d_plan, slot_cap, K, stream), "plan");
+            metrics[0]++;
+            for (int k = 0; k < K; ++k) enqueue_passes((uint32_t)k);
+            mark(2);
+            hip_check(cpm_internal_launch_plan_end(d_reps, d_dyn, d_plan, slot_cap, K, stream), "plan end");
+            metrics[0]++;
+            n -= (uint64_t)K;
         }
     }
 
@@ -1601,7 +1646,7 @@ struct Engine {
         try {
             for (hipEvent_t &e : wall) hip_check(hipEventCreate(&e), "hipEventCreate");
             hip_check(hipEventRecord(wall[0], stream), "hipEventRecord");
-            for (uint64_t i = 0; i < n; ++i) enqueue_mcs();
+            enqueue_steps(n);
             hip_check(hipEventRecord(wall[1], stream), "hipEventRecord");
         } catch (...) {
             timing_events = nullptr;
@@ -1650,7 +1695,7 @@ struct Engine {
         std::vector<uint64_t> saved;
         for (Group &g : groups) saved.push_back(g.pass_counter);
         try {
-            for (uint32_t i = 0; i < mcs; ++i) enqueue_mcs();
+            enqueue_steps(mcs);
         } catch (...) {
             hipGraph_t tmp = nullptr;
             (void)hipStreamEndCapture(stream, &tmp);
@@ -1682,9 +1727,16 @@ struct Engine {
                     n -= chunk;
                     continue;
                 }
+                if (n >= chunk) {
+                    // one MCS of plain launches brings the claim-buffer parity back for the graph
+                    enqueue_steps(1);
+                    n -= 1;
+                    continue;
+                }
             }
-            enqueue_mcs();
-            n -= 1;
+            // the rest (or every MCS without graphs): planned chunks of plain launches
+            enqueue_steps(n);
+            n = 0;
         }
     }
 
@@ -1923,11 +1975,12 @@ struct Engine {
     }
 
     // Half-interval OU steps of the replicas of a group that carry mechanics.
-    void advance_mechanics(const Group &g, uint32_t pass, uint32_t phase) {
+    void advance_mechanics(const Group &g, uint32_t pass, uint32_t phase, uint32_t plan_k) {
         for (int s : g.slots) {
             const Replica *r = reps[s];
             if (!r || r->desc.n_mechanics == 0) continue;
-            hip_check(cpm_internal_launch_mech_advance(d_reps, d_dyn, s, r->desc.capacity, pass, phase, stream),
+            hip_check(cpm_internal_launch_mech_advance(d_reps, d_plan, s, r->desc.capacity, pass, phase, plan_k,
+                                                       stream),
                       "mechanics");
             metrics[0]++;
         }

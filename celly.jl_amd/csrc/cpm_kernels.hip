@@ -412,7 +412,7 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
     // descriptor and MCS state are not written by this kernel (constant address space:
     // scalar loads that stay valid across the kernel's own stores)
     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
-    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[r];
+    const CPM_CONST McsPlan &y = ((const CPM_CONST McsPlan *)a.plan)[(size_t)r * kPlanMcs + a.plan_k];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[ti.rep];
-    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[ti.rep];
+    const CPM_CONST McsPlan &y = ((const CPM_CONST McsPlan *)a.plan)[(size_t)ti.rep * kPlanMcs + a.plan_k];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows3_kernel(PassArgs a) {
     const uint32_t tile = xcd_tile(blockIdx.x, a.ntiles);
     const TileInfo ti = a.tiles[tile];
     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[ti.rep];
-    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[ti.rep];
+    const CPM_CONST McsPlan &y = ((const CPM_CONST McsPlan *)a.plan)[(size_t)ti.rep * kPlanMcs + a.plan_k];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -1112,7 +1112,7 @@ __global__ __launch_bounds__(NT) void lot_propose_kernel(PassArgs a) {
     // descriptor and MCS state are not written by this kernel (constant address space:
     // scalar loads that stay valid across the kernel's own stores)
     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
-    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[r];
+    const CPM_CONST McsPlan &y = ((const CPM_CONST McsPlan *)a.plan)[(size_t)r * kPlanMcs + a.plan_k];
     if (!y.active || a.pass >= (uint32_t)d.rounds) {
         if (threadIdx.x < kWaves) a.seg_count[tile * kWaves + threadIdx.x] = kIdleTile;
         return;
@@ -1990,8 +1990,8 @@ __global__ __launch_bounds__(64 * NWV) void commit_kernel(PassArgs a) {
                 const uint32_t r = one ? r_first : a.tiles[t0 + k].rep;
                 if (w < nw && (one || kw == (int)k)) {
                     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)a.reps)[r];
-                    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)a.dyn)[r];
-                    const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1 : 0;
+                    const CPM_CONST McsPlan &y = ((const CPM_CONST McsPlan *)a.plan)[(size_t)r * kPlanMcs + a.plan_k];
+                    const uint32_t sub = d.algorithm == CPM_ALG_LOTTERY ? y.order[a.pass] - 1u : 0u;
                     const Entry ent{(uint32_t)rec.x, 0u, rec.z, rec.w};
                     Staged st;
                     float dh;
@@ -2067,10 +2067,10 @@ __device__ inline void mech_inputs(const CPM_CONST RepDev &d, int k, const CPM_G
 // One half-interval OU step of every mechanical state of every cell of one replica,
 // before (phase 0) or after (phase 1) pass `pass` (_advance_checkerboard_mechanics!,
 // scientific_mechanics.jl:170-196; lottery.jl:353-391 through _advance_mechanics!).
-__global__ __launch_bounds__(kBlock) void mech_advance_kernel(const RepDev *reps, const RepDyn *dyn, int slot,
-                                                              uint32_t pass, uint32_t phase) {
+__global__ __launch_bounds__(kBlock) void mech_advance_kernel(const RepDev *reps, const McsPlan *plan, int slot,
+                                                              uint32_t pass, uint32_t phase, uint32_t plan_k) {
     const CPM_CONST RepDev &d = ((const CPM_CONST RepDev *)reps)[slot];
-    const CPM_CONST RepDyn &y = ((const CPM_CONST RepDyn *)dyn)[slot];
+    const CPM_CONST McsPlan &y = ((const CPM_CONST McsPlan *)plan)[(size_t)slot * kPlanMcs + plan_k];
     if (!y.active || pass >= (uint32_t)d.rounds) return;
     const uint32_t cell = 1u + blockIdx.x * kBlock + threadIdx.x;
     if (cell > d.capacity) return;
@@ -2499,27 +2499,67 @@ __global__ __launch_bounds__(kBlock) void tiled_reconcile_kernel(const RepDev *r
 
 // Starts one MCS for every replica below its target: mcs += 1 and the colour / round
 // order of that MCS (checkerboard.jl:226-242, lottery.jl:124-140).
-__global__ void begin_mcs_kernel(const RepDev *reps, RepDyn *dyn, int nslots) {
+// The colour / round order of MCS m of replica d (checkerboard.jl:226-242, lottery.jl:124-140,
+// tiled_checkerboard_device.jl:743-760).
+template <class T>
+__device__ inline void mcs_order(const RepDev &d, uint64_t m, T *order) {
+    if (d.algorithm == CPM_ALG_LOTTERY) schedule_order(d.key_lo, d.key_hi, kStreamLotteryActivation, 1, m, d.rounds, order);
+    else if (d.algorithm == CPM_ALG_TILED) schedule_order(d.key_lo, d.key_hi, kStreamTiledOrder, 0, m, d.rounds, order);
+    else schedule_order(d.key_lo, d.key_hi, kStreamCheckerboardOrder, 0, m, d.rounds, order);
+}
+
+// Starts one MCS for every replica below its target: mcs += 1 and the colour / round order of
+// that MCS, also as plan 0 for the pass kernels; TiledCheckerboardCPM counters are reset.
+__global__ void begin_mcs_kernel(const RepDev *reps, RepDyn *dyn, McsPlan *plan, int nslots) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nslots) return;
     RepDyn &y = dyn[r];
+    McsPlan &p = plan[(size_t)r * kPlanMcs];
     if (!y.in_use || y.mcs >= y.target) {
         y.active = 0;
+        p.active = 0;
         return;
     }
     const uint64_t m = y.mcs + 1;
     y.mcs = m;
     y.active = 1;
+    p.mcs = m;
+    p.active = 1;
     const RepDev &d = reps[r];
-    if (d.algorithm == CPM_ALG_LOTTERY) {
-        schedule_order(d.key_lo, d.key_hi, kStreamLotteryActivation, 1, m, d.rounds, y.order);
-    } else if (d.algorithm == CPM_ALG_TILED) {
-        // tile colour order (tiled_checkerboard_device.jl:743-760) and fresh MCS counters
-        schedule_order(d.key_lo, d.key_hi, kStreamTiledOrder, 0, m, d.rounds, y.order);
+    mcs_order(d, m, y.order);
+    for (int i = 0; i < d.rounds; ++i) p.order[i] = (uint16_t)y.order[i];
+    if (d.algorithm == CPM_ALG_TILED)
         for (int i = 0; i < 5; ++i) d.tiled_counts[i] = 0;
-    } else {
-        schedule_order(d.key_lo, d.key_hi, kStreamCheckerboardOrder, 0, m, d.rounds, y.order);
-    }
+}
+
+// Plans of a chunk of K MCS (K <= kPlanMcs), one thread per (replica, MCS): the k-th MCS is
+// mcs + k + 1 and runs while the replica is in use and below its target. The counters stay
+// untouched until plan_end_kernel.
+__global__ void plan_kernel(const RepDev *reps, const RepDyn *dyn, McsPlan *plan, int nslots, int K) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots * K) return;
+    const int r = i / K, k = i % K;
+    const RepDyn &y = dyn[r];
+    McsPlan &p = plan[(size_t)r * kPlanMcs + k];
+    const bool run = y.in_use && y.mcs + (uint64_t)k < y.target;
+    const uint64_t m = y.mcs + (uint64_t)k + 1;
+    p.mcs = m;
+    p.active = run ? 1u : 0u;
+    if (run) mcs_order(reps[r], m, p.order);
+}
+
+// End of a planned chunk of K MCS: every replica's counter advances by the MCS it performed,
+// and its last performed MCS's order becomes the current one.
+__global__ void plan_end_kernel(const RepDev *reps, RepDyn *dyn, const McsPlan *plan, int nslots, int K) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nslots) return;
+    RepDyn &y = dyn[r];
+    const uint64_t done = y.in_use && y.target > y.mcs ? min((uint64_t)K, y.target - y.mcs) : 0;
+    y.active = done > 0 ? 1u : 0u;
+    if (done == 0) return;
+    const McsPlan &p = plan[(size_t)r * kPlanMcs + (done - 1)];
+    y.mcs += done;
+    for (int i = 0; i < reps[r].rounds; ++i) y.order[i] = p.order[i];
 }
 
 // Sets the host-controlled fields of one replica's dynamic state.
@@ -2837,10 +2877,25 @@ extern "C" hipError_t cpm_internal_launch_pass(int config, int algorithm, const 
     return cpm::launch_commit(*a, stream);
 }
 
-extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn,
+extern "C" hipError_t cpm_internal_launch_begin(const cpm::RepDev *reps, cpm::RepDyn *dyn, cpm::McsPlan *plan,
                                                 int nslots, hipStream_t stream) {
     const int blocks = (nslots + 63) / 64;
-    CPM_LAUNCH(cpm::begin_mcs_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, nslots);
+    CPM_LAUNCH(cpm::begin_mcs_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, plan, nslots);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cpm_internal_launch_plan(const cpm::RepDev *reps, const cpm::RepDyn *dyn, cpm::McsPlan *plan,
+                                               int nslots, int K, hipStream_t stream) {
+    if (K < 1 || K > cpm::kPlanMcs) return hipErrorInvalidValue;
+    const int blocks = (nslots * K + 63) / 64;
+    CPM_LAUNCH(cpm::plan_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, plan, nslots, K);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cpm_internal_launch_plan_end(const cpm::RepDev *reps, cpm::RepDyn *dyn,
+                                                   const cpm::McsPlan *plan, int nslots, int K, hipStream_t stream) {
+    const int blocks = (nslots + 63) / 64;
+    CPM_LAUNCH(cpm::plan_end_kernel, dim3(blocks), dim3(64), 0, stream, reps, dyn, plan, nslots, K);
     return hipGetLastError();
 }
 
@@ -2871,13 +2926,13 @@ extern "C" hipError_t cpm_internal_launch_tiled_reconcile(const cpm::RepDev *rep
     return hipGetLastError();
 }
 
-extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, const cpm::RepDyn *dyn, int slot,
+extern "C" hipError_t cpm_internal_launch_mech_advance(const cpm::RepDev *reps, const cpm::McsPlan *plan, int slot,
                                                        uint32_t capacity, uint32_t pass, uint32_t phase,
-                                                       hipStream_t stream) {
+                                                       uint32_t plan_k, hipStream_t stream) {
     const uint32_t blocks = (capacity + cpm::kBlock - 1) / cpm::kBlock;
     if (blocks == 0) return hipSuccess;
-    CPM_LAUNCH(cpm::mech_advance_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, dyn, slot, pass,
-                       phase);
+    CPM_LAUNCH(cpm::mech_advance_kernel, dim3(blocks), dim3(cpm::kBlock), 0, stream, reps, plan, slot, pass,
+                       phase, plan_k);
     return hipGetLastError();
 }
 

@@ -254,8 +254,9 @@ int32_t cpm_metrics(const cpm_ctx *ctx, uint64_t out[6]);
 /* Like cpm_step, but attaches a HIP event pair to every kernel launch on the context stream
  * (the pair brackets that kernel's execution) and synchronizes at the end. out[0..7] =
  * propose-kernel total ms (TiledCheckerboardCPM: batch kernel), propose launches,
- * commit-kernel total ms (tiled: reconciliation), commit launches, begin-kernel total ms,
- * begin launches, wall ms of the whole run, MCS run. Mechanics kernels are not in any of the
+ * commit-kernel total ms (tiled: reconciliation), commit launches, MCS-order kernels' total ms
+ * (the plan and end launches around each chunk of up to 64 MCS, or the begin launch of every MCS
+ * where a TiledCheckerboardCPM replica runs), their launches, wall ms of the whole run, MCS run. Mechanics kernels are not in any of the
  * kernel totals. (No counterpart in the reference: the measurement hook behind the per-kernel
  * roofline.) */
 int32_t cpm_step_timed(cpm_ctx *ctx, uint64_t n, double out[8]);
