@@ -13,6 +13,7 @@ pin the oracle over long runs, including every acceptance decision of the trajec
 By default a spread of CPM_MIGRATION_REPLICAS (24) replicas per algorithm runs here; set it to
 512 to check every replica (about 2 minutes on 8 cores).
 """
+import multiprocessing
 import os
 from concurrent.futures import ProcessPoolExecutor
 
@@ -57,7 +58,10 @@ def test_oracle_reproduces_realistic_trajectories(algorithm):
     col = load_migration()["algorithms"][ALGORITHMS[algorithm]]
     idx = _replicas()
     jobs = [(algorithm, int(col["dynamics_seed_hex"][i], 16)) for i in idx]
-    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
+    # spawned workers: a forked child of a process whose OpenMP thread pool already exists
+    # (earlier oracle tests in the same process) can crash in libgomp
+    ctx = multiprocessing.get_context("spawn")
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1), mp_context=ctx) as pool:
         results = list(pool.map(_run, jobs))
     mismatches = []
     for i, obs in zip(idx, results):
