@@ -114,7 +114,7 @@ def default_tile_shapes():
                 set_tile_shape(algorithm, ndim, halo, -1)
 
 
-def batched_leg(replicas: int, mcs: int, graph_chunk: int):
+def batched_leg(replicas: int, mcs: int, graph_chunk: int, plain: bool = True, tune: bool = True):
     """Batched ensemble of `replicas` 2D replicas (sides log-uniform in [64, 1024], T, J and
     seed per replica) for `mcs` MCS in one ReplicaBatch, against the same replicas run one
     after another through the plain single-lattice path on the same GPU."""
@@ -129,7 +129,8 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
     # tile shape per size class, tuned on other draws of the same ensemble distribution
     t0 = time.perf_counter()
     shapes = BatchShapes()
-    shapes.tune([workloads.ensemble_replica(replicas + i) for i in range(128)], per_class=16, mcs=10)
+    if tune:
+        shapes.tune([workloads.ensemble_replica(replicas + i) for i in range(128)], per_class=16, mcs=10)
     tune_s = time.perf_counter() - t0
     log(f"batched: shapes tuned in {tune_s:.1f} s: {shapes.describe()}")
 
@@ -149,7 +150,7 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
     default_tile_shapes()
     plain_s = 0.0
     same = 0
-    for m, d in zip(models, digests):
+    for m, d in zip(models if plain else [], digests):
         integ = init_scientific(*m)
         integ.set_graph_chunk(graph_chunk)
         integ.synchronize()
@@ -162,8 +163,10 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int):
     attempts = sum(sites) * mcs
     return {"replicas": replicas, "mcs": mcs, "sites": sum(sites), "graph_chunk": graph_chunk,
             "batched_attempts_per_s": attempts / batched_s, "batched_s": batched_s,
-            "plain_sequential_attempts_per_s": attempts / plain_s, "plain_sequential_s": plain_s,
-            "speedup": plain_s / batched_s, "replicas_identical_to_plain": same,
+            "plain_sequential_attempts_per_s": attempts / plain_s if plain else None,
+            "plain_sequential_s": plain_s if plain else None,
+            "speedup": plain_s / batched_s if plain else None,
+            "replicas_identical_to_plain": same if plain else None,
             "device_bytes": device_bytes, "tile_shapes": shapes.describe(), "tune_s": tune_s}
 
 
@@ -443,6 +446,10 @@ def main():
     ap.add_argument("--cubes-steps", type=int, default=40)
     ap.add_argument("--batch-replicas", type=int, default=512)
     ap.add_argument("--batch-mcs", type=int, default=500)
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="skip the sequential plain runs of the batched leg (profiling the batch alone)")
+    ap.add_argument("--batch-no-tune", action="store_true",
+                    help="default tile shapes for the batched leg instead of tuning them")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tiled", action="store_true")
     ap.add_argument("--no-headline", action="store_true",
@@ -482,7 +489,8 @@ def main():
     if not args.no_tiled:
         extra["tiled"] = tiled_leg(args.tiled_steps)
     if not args.no_batch:
-        extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk)
+        extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk,
+                                       not args.batch_no_plain, not args.batch_no_tune)
     if not args.no_serve:
         extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0, args.serve_graph_chunk,
                                        args.serve_rate)
