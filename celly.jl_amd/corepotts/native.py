@@ -65,7 +65,7 @@ EXPORTS = (
     "cpm_last_error", "cpm_abi_version", "cpm_create", "cpm_destroy", "cpm_upload_state",
     "cpm_download_state", "cpm_set_seed", "cpm_set_mcs", "cpm_get_mcs", "cpm_step", "cpm_report",
     "cpm_sync", "cpm_schedule", "cpm_site_colors", "cpm_metrics", "cpm_step_timed",
-    "cpm_set_graph_chunk", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
+    "cpm_set_graph_chunk", "cpm_prepare_graph", "cpm_copy_probe", "cpm_tile_shape_info", "cpm_set_tile_shape",
     "cpm_set_row_kernels", "cpm_set_tiled_staging", "cpm_kernel_config", "cpm_upload_mechanics", "cpm_download_mechanics",
     "cpm_initialize_mechanics", "cpm_batch_upload_mechanics", "cpm_batch_download_mechanics",
     "cpm_batch_initialize_mechanics", "cpm_tiled_layout", "cpm_model_colors",
@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_site_colors.argtypes = [P, P]
     lib.cpm_metrics.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.cpm_set_graph_chunk.argtypes = [P, C.c_uint32]
+    lib.cpm_prepare_graph.argtypes = [P, C.POINTER(C.c_int32)]
     lib.cpm_step_timed.argtypes = [P, C.c_uint64, C.POINTER(C.c_double)]
     lib.cpm_copy_probe.argtypes = [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]
     lib.cpm_tile_shape_info.argtypes = [C.c_int32, C.POINTER(C.c_int32)]
@@ -449,6 +450,13 @@ class ScientificPottsIntegrator:
 
     def set_graph_chunk(self, chunk: int):
         self._check(self.lib.cpm_set_graph_chunk(self._ctx, int(chunk)))
+
+    def prepare_graph(self) -> bool:
+        """Capture the graph the next steps replay without performing an MCS; True when the
+        next step of at least one chunk starts with it."""
+        ready = C.c_int32(0)
+        self._check(self.lib.cpm_prepare_graph(self._ctx, C.byref(ready)))
+        return bool(ready.value)
 
 
 def init_scientific(state: CompiledScientificState, proposal_relation, components, algorithm,

@@ -1709,6 +1709,22 @@ struct Engine {
         graph_valid = true;
     }
 
+    // Captures the graph that the next steps replay, performing no MCS, so that capture and
+    // instantiation stay out of a caller's timed loop. False when graphs are off or the claim
+    // buffers' parity would make the next step start with a plain MCS.
+    bool prepare_graph() {
+        rebuild_dirty();
+        if (graph_chunk == 0) return false;
+        uint32_t chunk = graph_chunk;
+        bool even = true;
+        for (const Group &g : groups)
+            if (g.ntiles && ((uint64_t)chunk * g.max_rounds) % 2) even = false;
+        if (!even) chunk *= 2;
+        if (!graph_parity_ok(chunk)) return false;
+        if (!graph_valid || graph_mcs != chunk) capture(chunk);
+        return true;
+    }
+
     void step(uint64_t n) {
         rebuild_dirty();
         for (Group &g : groups) g.dirty_arena_fresh = false;
@@ -2302,6 +2318,12 @@ int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk) {
     ctx->engine.graph_chunk = chunk;
     ctx->engine.drop_graph();
     return CPM_OK;
+}
+
+int32_t cpm_prepare_graph(cpm_ctx *ctx, int32_t *ready) {
+    if (!ctx || !ready) return CPM_E_INVALID;
+    *ready = 0;
+    return guarded(&ctx->engine.err, [&] { *ready = ctx->engine.prepare_graph() ? 1 : 0; });
 }
 
 uint64_t cpm_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t rerun) {

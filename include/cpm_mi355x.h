@@ -285,6 +285,12 @@ int32_t cpm_copy_probe(uint64_t bytes, int32_t reps, double *out_ms);
 /* Capture chunks of `chunk` MCS into a HIP graph that cpm_step replays when n is a
  * multiple of chunk (0 disables graphs). */
 int32_t cpm_set_graph_chunk(cpm_ctx *ctx, uint32_t chunk);
+/* Capture the HIP graph that the next cpm_step calls replay, performing no MCS, so that
+ * capture and instantiation stay out of a caller's timed loop. *ready = 1 when the next
+ * cpm_step of at least one chunk starts with that graph, 0 when graphs are off or the claim
+ * buffers' parity makes it start with one MCS of plain launches. (No counterpart in the
+ * reference: launch management of this engine.) */
+int32_t cpm_prepare_graph(cpm_ctx *ctx, int32_t *ready);
 
 /* Philox-derived replica seed (sciml/interface.jl:1009-1026). */
 uint64_t cpm_ensemble_seed(uint64_t master_seed, uint32_t trajectory, uint32_t rerun);

@@ -222,3 +222,25 @@ def test_graph_chunks_match_plain_launches(gpu_lib):
         assert a.current_mcs_report() == b.current_mcs_report()
     np.testing.assert_array_equal(a.download()["ids"], b.download()["ids"])
     assert b.metrics()["graph_launches"] >= 4
+
+
+def test_prepared_graph_matches_plain_launches(gpu_lib):
+    """A graph captured ahead of the steps (cpm_prepare_graph performs no MCS) replays the same
+    MCS as plain launches, after a warmup whose length is not a multiple of the chunk."""
+    compiled, proposal, components, algorithm, seed = workloads.config1_plain()
+    a = init_scientific(compiled, proposal, components, algorithm, seed=seed)
+    b = init_scientific(compiled, proposal, components, algorithm, seed=seed)
+    a.step(5)
+    b.step(5)
+    b.set_graph_chunk(10)
+    assert b.prepare_graph()
+    assert b.metrics()["graph_launches"] == 0
+    assert b.current_mcs_report() == a.current_mcs_report()  # nothing performed
+    a.step(20)
+    b.step(20)
+    assert b.metrics()["graph_launches"] == 2
+    assert a.current_mcs_report() == b.current_mcs_report()
+    for f in ("ids", "tags", "finite_volumes", "boundary_measures"):
+        np.testing.assert_array_equal(a.download()[f], b.download()[f], err_msg=f)
+    b.set_graph_chunk(0)
+    assert not b.prepare_graph()
