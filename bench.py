@@ -184,7 +184,12 @@ def lottery_leg(steps: int, shape: int = -1):
     else:
         ltuned = tune_tile_shape(compiled, proposal, components, lalg, seed=lseed, mcs=2, warmup=1)
     lot = init_scientific(compiled, proposal, components, lalg, seed=lseed)
-    lot.step(5)
+    # 6 warmup MCS (an even number of rounds), then graphs of the largest chunk of at most
+    # 10 MCS that divides the timed steps, captured before the timed region
+    lot.step(6)
+    chunk = graph_chunk_for(steps, 10)
+    lot.set_graph_chunk(chunk)
+    prepared = lot.prepare_graph() if chunk else False
     lot.synchronize()
     t0 = time.perf_counter()
     lot.step(steps)
@@ -198,6 +203,7 @@ def lottery_leg(steps: int, shape: int = -1):
     mcs_bytes = 4.0 * trep.internal_rounds * n + trep.accepted_copies * (4 + 16 + 32)
     out = {"copy_attempts_per_s": lrep.activated_attempts * steps / dt,
            "mcs_per_s": steps / dt, "rounds": lrep.internal_rounds, "tile_shape": ltuned,
+           "graph_chunk": chunk, "graphs_prepared": prepared,
            "propose_kernel_avg_us": propose_us, "commit_kernel_avg_us": commit_us,
            "algorithmic_bytes_per_mcs": mcs_bytes,
            "whole_mcs_algorithmic_GBps": mcs_bytes * steps / dt / 1e9,
