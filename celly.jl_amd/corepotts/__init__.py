@@ -17,7 +17,7 @@ from .model import (AlgorithmTemperatureNoise, ConstitutiveMeanInitialization, F
                     required_properties, static_relation, tracker_conformance_errors,
                     validate_relation_domain, von_neumann_offsets)
 from .spec import ModelSpec, build_spec, state_arrays
-from .native import (EngineError, ScientificMCSReport, ScientificPottsIntegrator, copy_probe,
+from .native import (DeviceSnapshot, EngineError, ScientificMCSReport, ScientificPottsIntegrator, copy_probe,
                      ensemble_seed, init_scientific, load_library)
 from .batch import ReplicaBatch, replica_bytes
 from .serve import Completion, JobServer, Request, latency_summary, make_request

@@ -144,6 +144,34 @@ class ReplicaBatch:
         if target_mcs is not None:
             self.set_target(slot, target_mcs)
 
+    @staticmethod
+    def snapshot_bytes(spec: ModelSpec) -> int:
+        """Device bytes of a snapshot of a replica of this model: the owner words, the 64-byte
+        cell records (capacity + 1) and the medium-volume counters (cpm_snapshot_info)."""
+        return 4 * int(np.prod(spec.dims)) + 64 * (int(spec.capacity) + 1) + 4 * 16
+
+    def snapshot(self, slot: int):
+        """Device-resident snapshot of one replica at its last completed MCS (synchronizes; no
+        host transfer): restore_snapshot() continues it exactly in a reserved slot."""
+        from .native import DeviceSnapshot
+        h = C.c_void_p()
+        self._check(self.lib.cpm_batch_snapshot_capture(self._b, int(slot), C.byref(h)))
+        return DeviceSnapshot(self.lib, h, self.specs[slot])
+
+    def restore_snapshot(self, slot: int, snap, prototype_compiled: Optional[CompiledScientificState] = None,
+                         target_mcs: Optional[int] = None):
+        """Load a reserved replica of the snapshot's model from a device snapshot (state,
+        seed and MCS counter, copied on the device) and set its target. prototype_compiled
+        (the model's compiled state) serves logical_state() afterwards."""
+        spec = self.specs[slot]
+        if int(snap.seed) != int(spec.seed):
+            raise ValueError(f"snapshot seed {snap.seed} differs from the replica's {spec.seed}")
+        self._check(self.lib.cpm_batch_snapshot_restore(self._b, int(slot), snap.handle))
+        if prototype_compiled is not None:
+            self.compiled[slot] = prototype_compiled
+        if target_mcs is not None:
+            self.set_target(slot, target_mcs)
+
     def remove(self, slot: int):
         self._check(self.lib.cpm_batch_remove(self._b, int(slot)))
         self.specs.pop(slot, None)

@@ -74,6 +74,8 @@ EXPORTS = (
     "cpm_batch_download_state", "cpm_batch_set_target", "cpm_batch_set_mcs", "cpm_batch_step", "cpm_batch_progress",
     "cpm_batch_capacity", "cpm_batch_report", "cpm_batch_sync", "cpm_batch_set_graph_chunk",
     "cpm_batch_bytes", "cpm_report_energy", "cpm_total_energy", "cpm_total_energy_noalloc",
+    "cpm_snapshot_capture", "cpm_batch_snapshot_capture", "cpm_snapshot_restore", "cpm_batch_snapshot_restore",
+    "cpm_snapshot_download", "cpm_snapshot_download_mechanics", "cpm_snapshot_info", "cpm_snapshot_destroy",
 )
 
 _lib = None
@@ -133,6 +135,14 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpm_batch_upload_state.argtypes = [P, C.c_int32] + state_args
     lib.cpm_batch_download_state.argtypes = [P, C.c_int32] + state_args
     lib.cpm_batch_set_target.argtypes = [P, C.c_int32, C.c_uint64]
+    lib.cpm_snapshot_capture.argtypes = [P, C.POINTER(P)]
+    lib.cpm_batch_snapshot_capture.argtypes = [P, C.c_int32, C.POINTER(P)]
+    lib.cpm_snapshot_restore.argtypes = [P, P]
+    lib.cpm_batch_snapshot_restore.argtypes = [P, C.c_int32, P]
+    lib.cpm_snapshot_download.argtypes = [P] + state_args
+    lib.cpm_snapshot_download_mechanics.argtypes = [P, P, P]
+    lib.cpm_snapshot_info.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.cpm_snapshot_destroy.argtypes = [P]
     lib.cpm_batch_set_mcs.argtypes = [P, C.c_int32, C.c_uint64]
     lib.cpm_upload_mechanics.argtypes = [P, P, P]
     lib.cpm_download_mechanics.argtypes = [P, P, P]
@@ -282,6 +292,60 @@ def snapshot(compiled: CompiledScientificState, s: dict) -> CompiledScientificSt
         if name in snap.properties and name in s:
             snap.properties[name] = s[name].astype(snap.properties[name].dtype)
     return snap
+
+
+class DeviceSnapshot:
+    """One replica's state at a completed MCS held in device memory (cpm_snapshot_*): owner
+    words, cell records, medium volumes, MCS counter and seed. restore() on an integrator or a
+    batch slot continues exactly; download() gives the state arrays of a download;
+    to_checkpoint() builds the canonical checkpoint (persistence/scientific.jl:250-292) on the
+    host from them."""
+
+    def __init__(self, lib, handle, spec: ModelSpec):
+        self.lib = lib
+        self._s = handle
+        self.spec = spec
+        mcs, seed, nbytes = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        lib.cpm_snapshot_info(handle, C.byref(mcs), C.byref(seed), C.byref(nbytes))
+        self.mcs, self.seed, self.bytes = int(mcs.value), int(seed.value), int(nbytes.value)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise EngineError(rc, self.lib.cpm_last_error(None).decode())
+
+    @property
+    def handle(self):
+        if not self._s:
+            raise RuntimeError("the snapshot has been closed")
+        return self._s
+
+    def download(self) -> dict:
+        out = state_buffers(self.spec)
+        self._check(self.lib.cpm_snapshot_download(self.handle, *state_pointers(out)))
+        out["medium_volumes"] = out["medium_volumes"][:len(self.spec.medium_ids)]
+        if self.spec.mechanics:
+            qp, qt = (None if key is None else np.zeros(self.spec.capacity, np.float32)
+                      for key in mechanics_columns(self.spec))
+            self._check(self.lib.cpm_snapshot_download_mechanics(self.handle, _ptr(qp), _ptr(qt)))
+            for key, arr in zip(mechanics_columns(self.spec), (qp, qt)):
+                if key is not None:
+                    out[key] = arr
+        return out
+
+    def to_checkpoint(self, compiled: CompiledScientificState, ancestry=None):
+        from .checkpoint import capture_arrays
+        return capture_arrays(self.download(), compiled, self.spec, self.mcs, self.seed, ancestry)
+
+    def close(self):
+        if self._s:
+            self.lib.cpm_snapshot_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ScientificMCSReport(NamedTuple):
@@ -450,6 +514,18 @@ class ScientificPottsIntegrator:
 
     def set_graph_chunk(self, chunk: int):
         self._check(self.lib.cpm_set_graph_chunk(self._ctx, int(chunk)))
+
+    def snapshot(self) -> DeviceSnapshot:
+        """Device-resident snapshot of the state at the current MCS (no host transfer)."""
+        h = C.c_void_p()
+        self._check(self.lib.cpm_snapshot_capture(self._ctx, C.byref(h)))
+        return DeviceSnapshot(self.lib, h, self.spec)
+
+    def restore_snapshot(self, snap: DeviceSnapshot):
+        """Continue from a device snapshot of this model (state, seed and MCS counter)."""
+        self._check(self.lib.cpm_snapshot_restore(self._ctx, snap.handle))
+        self.seed = snap.seed
+        return self
 
     def prepare_graph(self) -> bool:
         """Capture the graph the next steps replay without performing an MCS; True when the

@@ -62,7 +62,8 @@ class _Job:
     r: Request
     queued_s: float                   # when the job (re-)entered the queue
     admitted_s: Optional[float] = None  # first admission
-    cp: Any = None                    # checkpoint of an evicted job
+    cp: Any = None                    # checkpoint (or device snapshot) of an evicted job
+    cp_bytes: int = 0                 # device bytes the evicted job's snapshot holds
     preemptions: int = 0
 
 
@@ -81,13 +82,17 @@ class JobServer:
     `batch` is normally a ReplicaBatch; anything with the same reserve / load / remove /
     step / progress / download / report (and, for pre-emption, checkpoint / resume) methods
     works (the scheduler is tested on the host with a stand-in). `preempt_after_s`: None
-    disables pre-emption."""
+    disables pre-emption. `preempt_to`: "host" keeps an evicted job as a canonical checkpoint
+    in host memory (its whole device footprint returns to the budget); "device" keeps it as a
+    device snapshot (owner words, cell records, medium volumes: about 4 bytes per site plus 64
+    per cell stay charged to the budget) and resumes it with device copies only (batch methods
+    snapshot / restore_snapshot, snapshot_bytes)."""
 
     def __init__(self, hbm_budget_bytes: int, chunk_mcs: int = 50, algorithm: int = ALG_CHECKERBOARD,
                  batch=None, bytes_of: Callable[[ModelSpec], int] = replica_bytes,
                  graph_chunk: Optional[int] = None, clock: Callable[[], float] = time.perf_counter,
                  sleep: Callable[[float], None] = time.sleep, shapes=None,
-                 preempt_after_s: Optional[float] = None):
+                 preempt_after_s: Optional[float] = None, preempt_to: str = "host"):
         if chunk_mcs < 1:
             raise ValueError("chunk_mcs must be positive")
         self.budget = int(hbm_budget_bytes)
@@ -105,6 +110,9 @@ class JobServer:
         self.peak_jobs = 0
         self.steps = 0
         self.preempt_after = None if preempt_after_s is None else float(preempt_after_s)
+        if preempt_to not in ("host", "device"):
+            raise ValueError("preempt_to is 'host' or 'device'")
+        self.preempt_to = preempt_to
         self.preemptions = 0
 
     def _preempt(self, need: int, running: Dict[int, "_Job"], queue, now: float, size) -> bool:
@@ -119,15 +127,22 @@ class JobServer:
                 continue  # evicted before, resumed, or finishing with this step
             victims.append(slot)
             freed += size[id(job.r)]
+            if self.preempt_to == "device":
+                freed -= self.batch.snapshot_bytes(job.r.spec)
             if freed >= deficit:
                 break
         if freed < deficit:
             return False
         for slot in victims:
             job = running.pop(slot)
-            job.cp = self.batch.checkpoint(slot)
+            if self.preempt_to == "device":
+                job.cp = self.batch.snapshot(slot)
+                job.cp_bytes = int(job.cp.bytes)
+            else:
+                job.cp = self.batch.checkpoint(slot)
+                job.cp_bytes = 0
             self.batch.remove(slot)
-            self.used -= size[id(job.r)]
+            self.used -= size[id(job.r)] - job.cp_bytes
             job.preemptions += 1
             job.queued_s = now
             self.preemptions += 1
@@ -199,6 +214,11 @@ class JobServer:
             for slot, job in reserved:
                 if job.cp is None:
                     self.batch.load(slot, job.r.compiled, target_mcs=job.r.mcs)
+                elif job.cp_bytes:
+                    self.batch.restore_snapshot(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
+                    job.cp.close()
+                    self.used -= job.cp_bytes
+                    job.cp, job.cp_bytes = None, 0
                 else:
                     self.batch.resume(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
                     job.cp = None

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -119,6 +120,31 @@ T *dev_alloc(size_t count, uint64_t &bytes) {
 
 void dev_free(void *p) {
     if (p) (void)hipFree(p);
+}
+
+// Device owner words and cell records (host copies) -> the per-site and per-cell arrays of
+// cpm_download_state (null outputs are skipped).
+void unpack_state(const int32_t *words, size_t n, const cpm::CellRec *cells, size_t cap, uint8_t *tags,
+                  uint32_t *ids, int32_t *fv, int64_t *bm, uint8_t *active, uint32_t *ctype, double *tv,
+                  double *vs, int64_t *tb, double *bs) {
+    if (words)
+        for (size_t i = 0; i < n; ++i) {
+            const int32_t w = words[i];
+            if (tags) tags[i] = w > 0 ? 1 : 2;
+            if (ids) ids[i] = w > 0 ? cpm::word_cell(w) : (uint32_t)-w;
+        }
+    if (cells)
+        for (size_t c = 0; c < cap; ++c) {
+            const cpm::CellRec &q = cells[c + 1];
+            if (fv) fv[c] = q.vol;
+            if (bm) bm[c] = q.bnd;
+            if (active) active[c] = q.active;
+            if (ctype) ctype[c] = q.ctype;
+            if (tv) tv[c] = q.tv;
+            if (vs) vs[c] = q.vs;
+            if (tb) tb[c] = q.tb;
+            if (bs) bs[c] = q.bs;
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1208,6 +1234,31 @@ struct Group {
     }
 };
 
+}  // namespace
+
+// In-HBM snapshot of one replica at a completed MCS: its owner words, cell records (trackers,
+// property columns, mechanical state) and medium volumes, copied on the device, with the MCS
+// counter and seed. The snapshot owns its device buffers.
+struct cpm_snapshot {
+    int ndim = 0;
+    int64_t dims[3] = {1, 1, 1};
+    size_t n = 0;
+    uint32_t capacity = 0;
+    int32_t n_media = 0, n_types = 0;
+    uint64_t mcs = 0, seed = 0;
+    int32_t *d_lattice = nullptr;
+    cpm::CellRec *d_cells = nullptr;
+    int32_t *d_mvol = nullptr;
+    uint64_t bytes = 0;
+    ~cpm_snapshot() {
+        if (d_lattice) (void)hipFree(d_lattice);
+        if (d_cells) (void)hipFree(d_cells);
+        if (d_mvol) (void)hipFree(d_mvol);
+    }
+};
+
+namespace {
+
 struct Engine {
     int algorithm = CPM_ALG_CHECKERBOARD;
     hipStream_t stream = nullptr;
@@ -1928,28 +1979,15 @@ struct Engine {
         if (tags || ids) {
             std::vector<int32_t> words(n);
             hip_check(hipMemcpy(words.data(), r->d_lattice, n * 4, hipMemcpyDeviceToHost), "download lattice");
-            for (size_t i = 0; i < n; ++i) {
-                const int32_t w = words[i];
-                if (tags) tags[i] = w > 0 ? 1 : 2;
-                if (ids) ids[i] = w > 0 ? cpm::word_cell(w) : (uint32_t)-w;
-            }
+            unpack_state(words.data(), n, nullptr, 0, tags, ids, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, nullptr);
         }
         if (fv || bm || active || ctype || tv || vs || tb || bs) {
             // 64-byte cell records -> per-cell arrays
             std::vector<cpm::CellRec> cells(cap1);
             hip_check(hipMemcpy(cells.data(), r->d_cells, cap1 * sizeof(cpm::CellRec), hipMemcpyDeviceToHost),
                       "download");
-            for (size_t c = 0; c < cap; ++c) {
-                const cpm::CellRec &q = cells[c + 1];
-                if (fv) fv[c] = q.vol;
-                if (bm) bm[c] = q.bnd;
-                if (active) active[c] = q.active;
-                if (ctype) ctype[c] = q.ctype;
-                if (tv) tv[c] = q.tv;
-                if (vs) vs[c] = q.vs;
-                if (tb) tb[c] = q.tb;
-                if (bs) bs[c] = q.bs;
-            }
+            unpack_state(nullptr, 0, cells.data(), cap, nullptr, nullptr, fv, bm, active, ctype, tv, vs, tb, bs);
         }
         if (mv && r->desc.n_media > 0)
             hip_check(hipMemcpy(mv, r->d_mvol, (size_t)r->desc.n_media * 4, hipMemcpyDeviceToHost), "download");
@@ -2009,6 +2047,55 @@ struct Engine {
         r->dev.key_lo = (uint32_t)key;
         r->dev.key_hi = (uint32_t)(key >> 32);
         upload_rep(slot);
+    }
+
+    // Device-to-device copies of the replica's state into a new snapshot at MCS `mcs`.
+    cpm_snapshot *capture_snapshot(int slot, uint64_t mcs) {
+        Replica *r = check_slot(slot);
+        std::unique_ptr<cpm_snapshot> sn(new cpm_snapshot());
+        const size_t n = (size_t)r->n, cap1 = (size_t)r->desc.capacity + 1;
+        sn->ndim = r->desc.ndim;
+        for (int a = 0; a < 3; ++a) sn->dims[a] = r->desc.dims[a];
+        sn->n = n;
+        sn->capacity = r->desc.capacity;
+        sn->n_media = r->desc.n_media;
+        sn->n_types = r->desc.n_types;
+        sn->mcs = mcs;
+        sn->seed = r->desc.seed;
+        sn->d_lattice = dev_alloc<int32_t>(n, sn->bytes);
+        sn->d_cells = dev_alloc<cpm::CellRec>(cap1, sn->bytes);
+        sn->d_mvol = dev_alloc<int32_t>(CPM_MAX_MEDIA, sn->bytes);
+        hip_check(hipMemcpyAsync(sn->d_lattice, r->d_lattice, n * 4, hipMemcpyDeviceToDevice, stream), "snapshot");
+        hip_check(hipMemcpyAsync(sn->d_cells, r->d_cells, cap1 * sizeof(cpm::CellRec), hipMemcpyDeviceToDevice,
+                                 stream), "snapshot");
+        hip_check(hipMemcpyAsync(sn->d_mvol, r->d_mvol, CPM_MAX_MEDIA * 4, hipMemcpyDeviceToDevice, stream),
+                  "snapshot");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        return sn.release();
+    }
+
+    // The snapshot's state copied into the replica (device to device) with its seed; the
+    // caller sets the MCS counter. The replica must have the snapshot's lattice, capacity,
+    // media and cell types.
+    void restore_snapshot(int slot, const cpm_snapshot &sn) {
+        Replica *r = check_slot(slot);
+        bool same = sn.ndim == r->desc.ndim && sn.capacity == r->desc.capacity && sn.n_media == r->desc.n_media &&
+                    sn.n_types == r->desc.n_types && sn.n == (size_t)r->n;
+        for (int a = 0; a < 3; ++a) same = same && sn.dims[a] == r->desc.dims[a];
+        if (!same) fail(CPM_E_INVALID, "the snapshot's lattice, capacity, media or cell types differ from the replica's");
+        const size_t cap1 = (size_t)r->desc.capacity + 1;
+        hip_check(hipStreamSynchronize(stream), "sync");
+        hip_check(hipMemcpyAsync(r->d_lattice, sn.d_lattice, sn.n * 4, hipMemcpyDeviceToDevice, stream), "restore");
+        hip_check(hipMemcpyAsync(r->d_cells, sn.d_cells, cap1 * sizeof(cpm::CellRec), hipMemcpyDeviceToDevice,
+                                 stream), "restore");
+        hip_check(hipMemcpyAsync(r->d_mvol, sn.d_mvol, CPM_MAX_MEDIA * 4, hipMemcpyDeviceToDevice, stream),
+                  "restore");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        metrics[1]++;
+        if (sn.seed != r->desc.seed) set_seed(slot, sn.seed);
+        r->base_mcs = sn.mcs;
+        r->report_cached = false;
     }
 };
 
@@ -2511,6 +2598,94 @@ int32_t cpm_batch_set_mcs(cpm_batch *b, int32_t slot, uint64_t mcs) {
         r->report_cached = false;
     });
 }
+
+int32_t cpm_snapshot_capture(cpm_ctx *ctx, cpm_snapshot **out) {
+    if (!ctx || !out) return CPM_E_INVALID;
+    *out = nullptr;
+    return guarded(&ctx->engine.err, [&] { *out = ctx->engine.capture_snapshot(ctx->slot, ctx->host_mcs); });
+}
+
+int32_t cpm_batch_snapshot_capture(cpm_batch *b, int32_t slot, cpm_snapshot **out) {
+    if (!b || !out) return CPM_E_INVALID;
+    *out = nullptr;
+    return guarded(&b->engine.err, [&] {
+        b->engine.check_slot(slot);
+        *out = b->engine.capture_snapshot(slot, b->engine.read_mcs(slot));
+    });
+}
+
+int32_t cpm_snapshot_restore(cpm_ctx *ctx, const cpm_snapshot *snapshot) {
+    if (!ctx || !snapshot) return CPM_E_INVALID;
+    return guarded(&ctx->engine.err, [&] {
+        ctx->engine.restore_snapshot(ctx->slot, *snapshot);
+        ctx->engine.set_dyn(ctx->slot, snapshot->mcs, snapshot->mcs, 1, 1 | 2 | 8);
+        ctx->host_mcs = snapshot->mcs;
+    });
+}
+
+int32_t cpm_batch_snapshot_restore(cpm_batch *b, int32_t slot, const cpm_snapshot *snapshot) {
+    if (!b || !snapshot) return CPM_E_INVALID;
+    return guarded(&b->engine.err, [&] {
+        b->engine.restore_snapshot(slot, *snapshot);
+        b->engine.set_dyn(slot, snapshot->mcs, snapshot->mcs, 1, 1 | 2 | 8);
+    });
+}
+
+int32_t cpm_snapshot_download(const cpm_snapshot *sn, uint8_t *tags, uint32_t *ids, int32_t *finite_volumes,
+                              int32_t *medium_volumes, int64_t *boundary_measures, uint8_t *active,
+                              uint32_t *cell_types, double *target_volume, double *volume_strength,
+                              int64_t *target_boundary, double *boundary_strength) {
+    if (!sn) {
+        g_create_error = "null snapshot";
+        return CPM_E_INVALID;
+    }
+    return guarded(&g_create_error, [&] {
+        const size_t cap = sn->capacity;
+        if (tags || ids) {
+            std::vector<int32_t> words(sn->n);
+            hip_check(hipMemcpy(words.data(), sn->d_lattice, sn->n * 4, hipMemcpyDeviceToHost), "snapshot download");
+            unpack_state(words.data(), sn->n, nullptr, 0, tags, ids, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, nullptr);
+        }
+        if (finite_volumes || boundary_measures || active || cell_types || target_volume || volume_strength ||
+            target_boundary || boundary_strength) {
+            std::vector<cpm::CellRec> cells(cap + 1);
+            hip_check(hipMemcpy(cells.data(), sn->d_cells, (cap + 1) * sizeof(cpm::CellRec), hipMemcpyDeviceToHost),
+                      "snapshot download");
+            unpack_state(nullptr, 0, cells.data(), cap, nullptr, nullptr, finite_volumes, boundary_measures, active,
+                         cell_types, target_volume, volume_strength, target_boundary, boundary_strength);
+        }
+        if (medium_volumes && sn->n_media > 0)
+            hip_check(hipMemcpy(medium_volumes, sn->d_mvol, (size_t)sn->n_media * 4, hipMemcpyDeviceToHost),
+                      "snapshot download");
+    });
+}
+
+int32_t cpm_snapshot_download_mechanics(const cpm_snapshot *sn, float *volume_pressure, float *surface_tension) {
+    if (!sn) {
+        g_create_error = "null snapshot";
+        return CPM_E_INVALID;
+    }
+    return guarded(&g_create_error, [&] {
+        const size_t cap = sn->capacity;
+        if (volume_pressure)
+            hip_check(hipMemcpy2D(volume_pressure, sizeof(float), &sn->d_cells[1].qp, sizeof(cpm::CellRec), sizeof(float),
+                                  cap, hipMemcpyDeviceToHost), "snapshot mechanics");
+        if (surface_tension)
+            hip_check(hipMemcpy2D(surface_tension, sizeof(float), &sn->d_cells[1].qt, sizeof(cpm::CellRec), sizeof(float),
+                                  cap, hipMemcpyDeviceToHost), "snapshot mechanics");
+    });
+}
+
+int32_t cpm_snapshot_info(const cpm_snapshot *sn, uint64_t *mcs, uint64_t *seed, uint64_t *bytes) {
+    if (!sn) return CPM_E_INVALID;
+    if (mcs) *mcs = sn->mcs;
+    if (seed) *seed = sn->seed;
+    if (bytes) *bytes = sn->bytes;
+    return CPM_OK;
+}
+
+void cpm_snapshot_destroy(cpm_snapshot *sn) { delete sn; }
 
 int32_t cpm_batch_step(cpm_batch *b, uint64_t n) {
     if (!b) return CPM_E_INVALID;

@@ -333,6 +333,35 @@ int32_t cpm_batch_initialize_mechanics(cpm_batch *b, int32_t slot);
 int32_t cpm_batch_set_mcs(cpm_batch *b, int32_t slot, uint64_t mcs);
 /* The replica advances until its MCS counter reaches target_mcs. */
 int32_t cpm_batch_set_target(cpm_batch *b, int32_t slot, uint64_t target_mcs);
+
+/* Device-resident snapshots. A snapshot holds one replica's state at its last completed MCS in
+ * device memory: owner words, cell records (trackers, property columns, mechanical state),
+ * medium volumes, the MCS counter and the seed, which is everything exact continuation needs
+ * (persistence/scientific.jl:464-468: state, seed, MCS). Capture and restore are device-to-device
+ * copies with no host transfer, so a served job can be pre-empted into HBM at about 4 bytes per
+ * site plus 64 per cell and resumed at once. The canonical checkpoint of
+ * persistence/scientific.jl:250-292 is built on the host from cpm_snapshot_download. (The
+ * reference keeps checkpoints on the host only; this is the engine's device-side form.) */
+typedef struct cpm_snapshot cpm_snapshot;
+/* Capture the context's replica (synchronizes). *out is freed with cpm_snapshot_destroy. */
+int32_t cpm_snapshot_capture(cpm_ctx *ctx, cpm_snapshot **out);
+int32_t cpm_batch_snapshot_capture(cpm_batch *b, int32_t slot, cpm_snapshot **out);
+/* Copy a snapshot into a replica with the same lattice, capacity, media and cell types, and set
+ * its seed and MCS counter to the snapshot's (a batch replica then waits for a target). */
+int32_t cpm_snapshot_restore(cpm_ctx *ctx, const cpm_snapshot *snapshot);
+int32_t cpm_batch_snapshot_restore(cpm_batch *b, int32_t slot, const cpm_snapshot *snapshot);
+/* The snapshot's state in cpm_download_state's arrays and order (null pointers skip arrays), and
+ * its mechanical state as cpm_download_mechanics. Errors are reported by cpm_last_error(NULL). */
+int32_t cpm_snapshot_download(const cpm_snapshot *snapshot, uint8_t *tags, uint32_t *ids,
+                              int32_t *finite_volumes, int32_t *medium_volumes,
+                              int64_t *boundary_measures, uint8_t *active, uint32_t *cell_types,
+                              double *target_volume, double *volume_strength,
+                              int64_t *target_boundary, double *boundary_strength);
+int32_t cpm_snapshot_download_mechanics(const cpm_snapshot *snapshot, float *volume_pressure,
+                                        float *surface_tension);
+/* MCS counter, seed and device bytes of a snapshot. */
+int32_t cpm_snapshot_info(const cpm_snapshot *snapshot, uint64_t *mcs, uint64_t *seed, uint64_t *bytes);
+void cpm_snapshot_destroy(cpm_snapshot *snapshot);
 /* Enqueue up to n MCS for every replica below its target (asynchronous). */
 int32_t cpm_batch_step(cpm_batch *b, uint64_t n);
 /* Current MCS of every slot (synchronizes); out needs cpm_batch_capacity entries,
