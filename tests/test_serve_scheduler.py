@@ -194,3 +194,69 @@ def test_preemption_cannot_ping_pong():
         assert c.preemptions <= 1
     assert server.peak_used <= 100
     assert not batch.mcs
+
+
+class FakeSnapshot:
+    def __init__(self, mcs, nbytes):
+        self.mcs, self.bytes, self.closed = mcs, nbytes, False
+
+    def close(self):
+        self.closed = True
+
+
+class FakeSnapshotBatch(FakeBatch):
+    """FakeBatch with device snapshots of a fixed 20 % of a replica's bytes."""
+
+    @staticmethod
+    def snapshot_bytes(spec):
+        return spec.nbytes // 5
+
+    def snapshot(self, slot):
+        self.log.append(("snapshot", slot, self.mcs[slot], self.clock()))
+        return FakeSnapshot(self.mcs[slot], self.bytes[slot] // 5)
+
+    def restore_snapshot(self, slot, snap, compiled, target_mcs=None):
+        self.mcs[slot] = snap.mcs
+        self.target[slot] = target_mcs
+        self.log.append(("restore", compiled, self.clock()))
+
+
+def test_preemption_into_device_snapshots_keeps_their_bytes_charged():
+    """preempt_to="device": the evicted job's snapshot stays charged to the budget while it
+    waits (so the head is admitted only if the rest fits beside it), the job resumes from the
+    snapshot's MCS and performs exactly the MCS it asked for, and the snapshot is released."""
+    clock = FakeClock()
+    batch = FakeSnapshotBatch(clock)
+    server = JobServer(100, chunk_mcs=10, batch=batch, bytes_of=lambda s: s.nbytes, clock=clock,
+                       sleep=clock.sleep, preempt_after_s=0.005, preempt_to="device")
+    specs = [FakeSpec(10, 80), FakeSpec(10, 50)]
+    done = server.serve(_requests(specs, [0.0, 0.0015], [1000, 30]))
+    by = {c.key: c for c in done}
+    assert by[0].state["mcs"] == 1000 and by[1].state["mcs"] == 30
+    assert by[0].preemptions == 1 and server.preemptions == 1
+    assert by[1].finished_s < by[0].finished_s
+    snaps = [e for e in batch.log if e[0] == "snapshot"]
+    restores = [e for e in batch.log if e[0] == "restore"]
+    assert len(snaps) == 1 and len(restores) == 1 and 0 < snaps[0][2] < 1000
+    assert not [e for e in batch.log if e[0] in ("checkpoint", "resume")]
+    assert server.peak_used <= 100 and server.used == 0
+    # 80 + 50 > 100, but 80 // 5 + 50 <= 100: the head fits beside the snapshot
+
+
+def test_device_preemption_that_cannot_make_room_evicts_nothing():
+    """A snapshot that would keep too much of the budget frees too little: nothing is evicted
+    and the head waits for the running job to finish."""
+    clock = FakeClock()
+    batch = FakeSnapshotBatch(clock)
+    server = JobServer(100, chunk_mcs=10, batch=batch, bytes_of=lambda s: s.nbytes, clock=clock,
+                       sleep=clock.sleep, preempt_after_s=0.0, preempt_to="device")
+    specs = [FakeSpec(10, 60), FakeSpec(10, 95)]  # 60 // 5 + 95 > 100
+    done = server.serve(_requests(specs, [0.0, 0.0015], [200, 30]))
+    by = {c.key: c for c in done}
+    assert server.preemptions == 0 and by[1].admitted_s >= by[0].finished_s
+    assert server.used == 0
+
+
+def test_preempt_to_must_name_host_or_device():
+    with pytest.raises(ValueError):
+        JobServer(100, batch=FakeBatch(FakeClock()), preempt_to="disk")
