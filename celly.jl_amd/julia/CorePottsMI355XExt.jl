@@ -353,4 +353,34 @@ function logical_state(integrator::MI355XIntegrator)
     return CorePotts.logical_snapshot(integrator.plan, st.potts)
 end
 
+# Device-resident snapshots (cpm_snapshot_*): the state, MCS counter and seed of the integrator's
+# context kept in HBM. restore! continues exactly with device copies only; the canonical
+# checkpoint (persistence/scientific.jl:250-292) stays the host capture of CorePotts itself.
+mutable struct MI355XSnapshot
+    handle::Ptr{Cvoid}
+    mcs::UInt64
+    seed::UInt64
+end
+
+function snapshot(integrator::MI355XIntegrator)
+    ws = ensure_context!(integrator)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:cpm_snapshot_capture, LIB), Int32, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), ws.ctx, h), ws.ctx)
+    mcs, seed, nbytes = Ref{UInt64}(0), Ref{UInt64}(0), Ref{UInt64}(0)
+    check(ccall((:cpm_snapshot_info, LIB), Int32, (Ptr{Cvoid}, Ptr{UInt64}, Ptr{UInt64}, Ptr{UInt64}),
+                h[], mcs, seed, nbytes), C_NULL)
+    snap = MI355XSnapshot(h[], mcs[], seed[])
+    finalizer(s -> s.handle == C_NULL || ccall((:cpm_snapshot_destroy, LIB), Cvoid, (Ptr{Cvoid},), s.handle), snap)
+    return snap
+end
+
+function restore!(integrator::MI355XIntegrator, snap::MI355XSnapshot)
+    ws = ensure_context!(integrator)
+    check(ccall((:cpm_snapshot_restore, LIB), Int32, (Ptr{Cvoid}, Ptr{Cvoid}), ws.ctx, snap.handle), ws.ctx)
+    integrator.mcs = snap.mcs
+    integrator.seed = snap.seed
+    ws.seed = snap.seed
+    return integrator
+end
+
 end # module

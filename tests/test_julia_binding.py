@@ -25,7 +25,7 @@ JULIA = os.path.join(ROOT, "celly.jl_amd", "julia", "CorePottsMI355XExt.jl")
 CONSTS = {"CPM_MAX_DIRS": 128, "CPM_MAX_TYPES": 16, "CPM_MAX_MEDIA": 16, "CPM_MAX_MECHANICS": 2,
           "MAX_DIRS": 128, "MAX_TYPES": 16, "MAX_MEDIA": 16}
 SCALARS = {"int32_t": "Int32", "uint32_t": "UInt32", "int64_t": "Int64", "uint64_t": "UInt64",
-           "double": "Float64", "float": "Float32", "uint8_t": "UInt8"}
+           "double": "Float64", "float": "Float32", "uint8_t": "UInt8", "void": "Cvoid"}
 STRUCTS = {"cpm_relation_desc": "CpmRelation", "cpm_mechanics_desc": "CpmMechanics",
            "cpm_model_desc": "CpmModelDesc"}
 SIZES = {"Int32": 4, "UInt32": 4, "Int64": 8, "UInt64": 8, "Float64": 8, "Float32": 4, "UInt8": 1}
@@ -140,7 +140,7 @@ def _c_to_julia(ctype):
     m = re.match(r"(const )?(\w+) ?(\**)\s*\w*(\[\d*\])?$", ctype)
     assert m, ctype
     base, stars, arr = m.group(2), m.group(3), m.group(4)
-    if base in ("cpm_ctx", "cpm_batch"):
+    if base in ("cpm_ctx", "cpm_batch", "cpm_snapshot"):
         return "Ref{Ptr{Cvoid}}" if len(stars) == 2 else "Ptr{Cvoid}"
     if base == "cpm_model_desc":
         return "Ref{CpmModelDesc}"
