@@ -48,9 +48,10 @@ class Completion(NamedTuple):
     finished_s: float
     mcs: int
     attempts: int           # activated attempts over the job (mutable sites x MCS)
-    state: dict             # downloaded arrays (cpm_download_state order)
-    report: tuple           # ScientificMCSReport of the job's last MCS
+    state: Optional[dict]   # downloaded arrays (cpm_download_state order); None if the job failed
+    report: Optional[tuple]  # ScientificMCSReport of the job's last MCS; None if the job failed
     preemptions: int = 0    # times the job was evicted and resumed from a checkpoint
+    error: Optional[str] = None  # why the job failed (its own engine or validation error)
 
     @property
     def latency_s(self) -> float:
@@ -86,7 +87,9 @@ class JobServer:
     in host memory (its whole device footprint returns to the budget); "device" keeps it as a
     device snapshot (owner words, cell records, medium volumes: about 4 bytes per site plus 64
     per cell stay charged to the budget) and resumes it with device copies only (batch methods
-    snapshot / restore_snapshot, snapshot_bytes)."""
+    snapshot / restore_snapshot, snapshot_bytes). A job whose own reserve, load, resume or
+    download raises an engine or validation error completes with `error` set (no state or
+    report); its slot and budget share are released and the other jobs are not affected."""
 
     def __init__(self, hbm_budget_bytes: int, chunk_mcs: int = 50, algorithm: int = ALG_CHECKERBOARD,
                  batch=None, bytes_of: Callable[[ModelSpec], int] = replica_bytes,
@@ -114,6 +117,7 @@ class JobServer:
             raise ValueError("preempt_to is 'host' or 'device'")
         self.preempt_to = preempt_to
         self.preemptions = 0
+        self.failures = 0
 
     def _preempt(self, need: int, running: Dict[int, "_Job"], queue, now: float, size) -> bool:
         """Evict running jobs, largest first, until `need` bytes fit; False (nothing evicted)
@@ -148,6 +152,26 @@ class JobServer:
             self.preemptions += 1
             queue.append(job)
         return True
+
+    def _fail(self, job: "_Job", slot: Optional[int], err: Exception, t0: float, done: list, size):
+        """A job whose own reserve, load, resume or download raised: its slot (if any) and budget
+        share are released, a held device snapshot is freed, and it completes with the error.
+        The batch and the other jobs carry on."""
+        if slot is not None:
+            try:
+                self.batch.remove(slot)
+            except Exception:
+                pass  # the batch already dropped the slot
+            self.used -= size[id(job.r)]
+        if job.cp_bytes:
+            job.cp.close()
+            self.used -= job.cp_bytes
+            job.cp, job.cp_bytes = None, 0
+        now = self.clock() - t0
+        r = job.r
+        done.append(Completion(r.key, r.arrival_s, now if job.admitted_s is None else job.admitted_s, now, 0, 0,
+                               None, None, job.preemptions, f"{type(err).__name__}: {err}"))
+        self.failures += 1
 
     def serve(self, requests: List[Request]) -> List[Completion]:
         """Runs the stream to completion; returns completions in finishing order.
@@ -185,10 +209,12 @@ class JobServer:
                     break
                 try:
                     slot = self.batch.reserve(job.r.spec)
-                except EngineError as e:
-                    if e.code == CPM_E_NOMEM and (running or reserved):
+                except (EngineError, ValueError) as e:
+                    if isinstance(e, EngineError) and e.code == CPM_E_NOMEM and (running or reserved):
                         break  # the engine's own accounting says wait for a completion
-                    raise
+                    queue.popleft()
+                    self._fail(job, None, e, t0, done, size)
+                    continue
                 queue.popleft()
                 self.used += need
                 if job.admitted_s is None:
@@ -201,8 +227,13 @@ class JobServer:
                     job = running[slot]
                     r = job.r
                     if progress[slot] >= r.mcs:
-                        state = self.batch.download(slot)
-                        report = tuple(self.batch.report(slot))
+                        try:
+                            state = self.batch.download(slot)
+                            report = tuple(self.batch.report(slot))
+                        except (EngineError, ValueError) as e:
+                            del running[slot]
+                            self._fail(job, slot, e, t0, done, size)
+                            continue
                         finished = self.clock() - t0
                         self.batch.remove(slot)
                         self.used -= size[id(r)]
@@ -212,16 +243,20 @@ class JobServer:
                         done.append(Completion(r.key, r.arrival_s, job.admitted_s, finished, r.mcs,
                                                mutable * r.mcs, state, report, job.preemptions))
             for slot, job in reserved:
-                if job.cp is None:
-                    self.batch.load(slot, job.r.compiled, target_mcs=job.r.mcs)
-                elif job.cp_bytes:
-                    self.batch.restore_snapshot(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
-                    job.cp.close()
-                    self.used -= job.cp_bytes
-                    job.cp, job.cp_bytes = None, 0
-                else:
-                    self.batch.resume(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
-                    job.cp = None
+                try:
+                    if job.cp is None:
+                        self.batch.load(slot, job.r.compiled, target_mcs=job.r.mcs)
+                    elif job.cp_bytes:
+                        self.batch.restore_snapshot(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
+                        job.cp.close()
+                        self.used -= job.cp_bytes
+                        job.cp, job.cp_bytes = None, 0
+                    else:
+                        self.batch.resume(slot, job.cp, job.r.compiled, target_mcs=job.r.mcs)
+                        job.cp = None
+                except (EngineError, ValueError) as e:
+                    self._fail(job, slot, e, t0, done, size)
+                    continue
                 running[slot] = job
             reserved = []
             self.peak_used = max(self.peak_used, self.used)
@@ -238,14 +273,17 @@ class JobServer:
 
 
 def latency_summary(done: List[Completion], wall_s: Optional[float] = None) -> dict:
-    """p50 / p99 time-to-completion and aggregate attempts/s of a served stream."""
+    """p50 / p99 time-to-completion and aggregate attempts/s of a served stream (failed jobs
+    are counted, not timed)."""
+    failed = sum(1 for c in done if c.error is not None)
+    done = [c for c in done if c.error is None]
     if not done:
-        return {"jobs": 0}
+        return {"jobs": 0, "failed": failed}
     lat = np.array([c.latency_s for c in done])
     first = min(c.arrival_s for c in done)
     last = max(c.finished_s for c in done)
     span = wall_s if wall_s is not None else last - first
     attempts = sum(c.attempts for c in done)
-    return {"jobs": len(done), "p50_s": float(np.percentile(lat, 50)),
+    return {"jobs": len(done), "failed": failed, "p50_s": float(np.percentile(lat, 50)),
             "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max()),
             "attempts": attempts, "span_s": span, "attempts_per_s": attempts / span}

@@ -2,6 +2,8 @@
 sizes, dimensions, temperatures, contact matrices, seeds and MCS targets; replicas joining
 and leaving mid-run) must end bit-exactly where the oracle's solo run of the same model
 ends, with the same last-MCS report."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -159,6 +161,28 @@ def test_preempted_jobs_equal_solo_runs(gpu_lib, alg):
     assert by[1].finished_s < by[0].finished_s
     for c in done:
         _check(c.state, c.report, _solo(models[c.key], c.mcs), f"job {c.key}")
+
+
+def test_a_bad_job_fails_alone(gpu_lib):
+    """Per-job failure isolation: a job whose ownership names a cell beyond its capacity is
+    refused at upload (host-side checks, before any device write) and completes with that
+    error; the jobs served beside it still equal their solo runs and the budget drains."""
+    models = {k: workloads.scaled_2d(L, seed=ensemble_seed(53, k + 1)) for k, L in enumerate((48, 64, 40))}
+    reqs = [make_request(k, 0.001 * k, *models[k], mcs=6 + k) for k in (0, 1, 2)]
+    good = reqs[1].compiled
+    ids = np.array(good.ids, copy=True)
+    ids[np.flatnonzero(np.asarray(good.tags) == 1)[0]] = good.capacity + 5
+    reqs[1] = reqs[1]._replace(compiled=dataclasses.replace(good, ids=ids))
+    server = JobServer(sum(replica_bytes(r.spec) for r in reqs) + (8 << 20), chunk_mcs=3)
+    done = server.serve(reqs)
+    by = {c.key: c for c in done}
+    assert sorted(by) == [0, 1, 2]
+    assert by[1].state is None and "capacity" in by[1].error
+    assert server.failures == 1 and server.used == 0
+    for k in (0, 2):
+        assert by[k].error is None
+        _check(by[k].state, by[k].report, _solo(models[k], by[k].mcs), f"job {k}")
+    assert latency_summary(done)["failed"] == 1
 
 
 def test_batch_shapes_per_size_class_keep_results(gpu_lib):

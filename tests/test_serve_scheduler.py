@@ -260,3 +260,50 @@ def test_device_preemption_that_cannot_make_room_evicts_nothing():
 def test_preempt_to_must_name_host_or_device():
     with pytest.raises(ValueError):
         JobServer(100, batch=FakeBatch(FakeClock()), preempt_to="disk")
+
+
+class FailingBatch(FakeBatch):
+    """FakeBatch whose reserve, load or download fails for chosen jobs (by request state)."""
+
+    def __init__(self, clock, bad_reserve=(), bad_load=(), bad_download=()):
+        super().__init__(clock)
+        self.bad_reserve, self.bad_load, self.bad_download = set(bad_reserve), set(bad_load), set(bad_download)
+        self.owner = {}
+
+    def reserve(self, spec):
+        if spec.nbytes in self.bad_reserve:
+            raise EngineError(-2, "unsupported model")
+        return super().reserve(spec)
+
+    def load(self, slot, compiled, target_mcs=None):
+        if compiled in self.bad_load:
+            self.remove(slot)  # as ReplicaBatch.load does on a failed upload
+            raise ValueError("owner identity exceeds the capacity")
+        self.owner[slot] = compiled
+        super().load(slot, compiled, target_mcs)
+
+    def download(self, slot):
+        if self.owner.get(slot) in self.bad_download:
+            raise EngineError(-1, "download failed")
+        return super().download(slot)
+
+
+def test_a_failing_job_does_not_poison_the_stream():
+    """Per-job failure isolation: jobs whose reserve, load or download raise complete with their
+    error; every other job completes with its MCS, and the budget is fully released."""
+    clock = FakeClock()
+    # request k has compiled state k; reserve fails for the spec of 33 bytes
+    specs = [FakeSpec(10, 20), FakeSpec(10, 33), FakeSpec(10, 20), FakeSpec(10, 20), FakeSpec(10, 20)]
+    batch = FailingBatch(clock, bad_reserve={33}, bad_load={2}, bad_download={4})
+    server = JobServer(100, chunk_mcs=10, batch=batch, bytes_of=lambda s: s.nbytes, clock=clock, sleep=clock.sleep)
+    done = server.serve(_requests(specs, [0.0, 0.001, 0.002, 0.003, 0.004], [30, 30, 30, 30, 30]))
+    by = {c.key: c for c in done}
+    assert sorted(by) == [0, 1, 2, 3, 4]
+    for k in (1, 2, 4):
+        assert by[k].error is not None and by[k].state is None and by[k].report is None
+    assert "unsupported" in by[1].error and "capacity" in by[2].error and "download" in by[4].error
+    for k in (0, 3):
+        assert by[k].error is None and by[k].state["mcs"] == 30
+    assert server.failures == 3 and server.used == 0 and not batch.mcs
+    s = latency_summary(done)
+    assert s["jobs"] == 2 and s["failed"] == 3
