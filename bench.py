@@ -114,10 +114,12 @@ def default_tile_shapes():
                 set_tile_shape(algorithm, ndim, halo, -1)
 
 
-def batched_leg(replicas: int, mcs: int, graph_chunk: int, plain: bool = True, tune: bool = True):
+def batched_leg(replicas: int, mcs: int, graph_chunk: int, plain: bool = True, tune: bool = True,
+                shape: int = -1):
     """Batched ensemble of `replicas` 2D replicas (sides log-uniform in [64, 1024], T, J and
     seed per replica) for `mcs` MCS in one ReplicaBatch, against the same replicas run one
-    after another through the plain single-lattice path on the same GPU."""
+    after another through the plain single-lattice path on the same GPU. shape >= 0 gives
+    every size class that tile shape instead of tuning."""
     from corepotts import BatchShapes, ReplicaBatch, init_scientific, workloads
     from corepotts.spec import ALG_CHECKERBOARD
     log(f"batched: building {replicas} replicas")
@@ -129,7 +131,10 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int, plain: bool = True, t
     # tile shape per size class, tuned on other draws of the same ensemble distribution
     t0 = time.perf_counter()
     shapes = BatchShapes()
-    if tune:
+    if shape >= 0:
+        from corepotts.autotune import SIZE_CLASS_BOUNDS
+        shapes.shapes = {(ALG_CHECKERBOARD, 2, 1, c): shape for c in range(len(SIZE_CLASS_BOUNDS) + 1)}
+    elif tune:
         shapes.tune([workloads.ensemble_replica(replicas + i) for i in range(128)], per_class=16, mcs=10)
     tune_s = time.perf_counter() - t0
     log(f"batched: shapes tuned in {tune_s:.1f} s: {shapes.describe()}")
@@ -471,6 +476,8 @@ def main():
                     help="skip the sequential plain runs of the batched leg (profiling the batch alone)")
     ap.add_argument("--batch-no-tune", action="store_true",
                     help="default tile shapes for the batched leg instead of tuning them")
+    ap.add_argument("--batch-tile-shape", type=int, default=-1,
+                    help="one tile shape for every size class of the batched leg instead of tuning")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tiled", action="store_true")
     ap.add_argument("--no-headline", action="store_true",
@@ -511,7 +518,7 @@ def main():
         extra["tiled"] = tiled_leg(args.tiled_steps)
     if not args.no_batch:
         extra["batched"] = batched_leg(args.batch_replicas, args.batch_mcs, args.graph_chunk,
-                                       not args.batch_no_plain, not args.batch_no_tune)
+                                       not args.batch_no_plain, not args.batch_no_tune, args.batch_tile_shape)
     if not args.no_serve:
         extra["serving"] = serving_leg(args.serve_jobs, args.serve_chunk, 200.0, args.serve_graph_chunk,
                                        args.serve_rate)

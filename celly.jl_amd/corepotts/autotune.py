@@ -96,16 +96,18 @@ class BatchShapes:
 
     Replicas of different tile shapes run in different launch groups, so a class's best shape
     depends on the others (one group for everything saves launches; a shape fitted to a class
-    saves halo work). tune() therefore times the whole sample batch: starting from the engine
-    defaults it goes class by class, from the largest lattices down, tries every compiled
-    shape for that class with the other classes fixed, and keeps the fastest (HIP graphs of
-    `mcs` MCS, timed after a warm-up step of the same length). apply(spec) sets the
-    process-wide preference for the next replica of that class; ReplicaBatch.reserve calls it.
-    Shapes change speed only, never results."""
+    saves halo work). tune() therefore times the whole sample batch (HIP graphs of `mcs` MCS,
+    timed after a warm-up step of the same length). It first gives every class of a
+    (algorithm, dimension, halo) family the same shape, tries every compiled shape that way
+    and starts from the fastest; then it goes class by class, from the largest lattices down,
+    tries every shape for that class with the other classes fixed, and keeps the fastest.
+    apply(spec) sets the process-wide preference for the next replica of that class;
+    ReplicaBatch.reserve calls it. Shapes change speed only, never results."""
 
     def __init__(self):
         self.shapes: Dict[tuple, int] = {}
         self.timings: Dict[tuple, Dict[int, float]] = {}
+        self.uniform_timings: Dict[tuple, Dict[int, float]] = {}
 
     @staticmethod
     def key(spec) -> tuple:
@@ -143,6 +145,16 @@ class BatchShapes:
         sample = [x for members in classes.values() for x in members]
         if not sample:
             return self.shapes
+        for fam in sorted({k[:3] for k in classes}):
+            members = [k for k in classes if k[:3] == fam]
+            times = {}
+            for shape, _tile in tile_shapes(fam[1], fam[2]):
+                for k in members:
+                    self.shapes[k] = shape
+                times[shape] = self._time_sample(sample, mcs)
+            for k in members:
+                self.shapes[k] = min(times, key=times.get)
+            self.uniform_timings[fam] = times
         for k in sorted(classes, key=lambda k: -k[3]):
             times = {}
             for shape, _tile in tile_shapes(k[1], k[2]):
@@ -162,4 +174,7 @@ class BatchShapes:
             out["%d/%dd/h%d/c%d" % k] = {
                 "shape": shape, "tile": list(tiles[shape]),
                 "ms": {int(s): round(t * 1e3, 3) for s, t in self.timings.get(k, {}).items()}}
+        for fam, times in self.uniform_timings.items():
+            out["%d/%dd/h%d/every class" % fam] = {
+                "ms": {int(s): round(t * 1e3, 3) for s, t in times.items()}}
         return out

@@ -203,6 +203,9 @@ def test_batch_shapes_per_size_class_keep_results(gpu_lib):
     for k, shape in tuned.items():
         assert shape in dict(tile_shapes(k[1], k[2]))
         assert set(shapes.timings[k]) == set(dict(tile_shapes(k[1], k[2])))
+    # every shape was also timed on all classes at once
+    assert set(shapes.uniform_timings) == {(ALG_CHECKERBOARD, 2, 1)}
+    assert set(shapes.uniform_timings[(ALG_CHECKERBOARD, 2, 1)]) == set(dict(tile_shapes(2, 1)))
     # force three different shapes so the batch has one launch group per class
     shapes.shapes = {k: s for k, s in zip(sorted(tuned), [7, 0, 12])}
     batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=3, shapes=shapes)
