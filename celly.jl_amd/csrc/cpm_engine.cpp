@@ -910,17 +910,18 @@ void compile_replica(Replica &r) {
         }
     }
     // row kernel: 2D, periodic and obstacle-free, canonical von Neumann proposal, one
-    // residue class of a 2 x 2 pattern per colour, rows of whole 16-byte groups
+    // residue class of a 2 x 2 pattern per colour, even x side (rows of whole 16-byte groups,
+    // or of 8-byte halves when the side is 2 mod 4)
     if (g_row_kernels.load() && d.algorithm == CPM_ALG_CHECKERBOARD && simple && d.ndim == 2 && v.pat_fast &&
         (r.config & 16) == 0 && v.pat[0] == 2 && v.pat[1] == 2 && v.pat[2] == 1 && r.rounds == 4 &&
-        r.desc.dims[0] % 4 == 0 &&
+        r.desc.dims[0] % 2 == 0 &&
         (shape == 0 || shape == 1 || (shape >= 4 && shape <= 7) || shape == 12 || shape == 13)) {
         static const int vn[4][2] = {{-1, 0}, {0, -1}, {0, 1}, {1, 0}};
         bool ok = d.proposal.count == 4;
         for (int k = 0; ok && k < 4; ++k)
             ok = d.proposal.offsets[k][0] == vn[k][0] && d.proposal.offsets[k][1] == vn[k][1];
         for (int c = 1; ok && c <= 4; ++c) ok = v.col_count[c] == 1;
-        if (ok) r.config |= 32;
+        if (ok) r.config |= r.desc.dims[0] % 4 == 0 ? 32 : 32 | 64;
     }
     // 3D row kernel: periodic and obstacle-free, canonical von Neumann proposal (six
     // directions), one residue class of a 2 x 2 x 2 pattern per colour, rows of whole 16-byte
@@ -941,7 +942,7 @@ void compile_replica(Replica &r) {
     if (d.algorithm == CPM_ALG_LOTTERY && all_periodic && wide) r.config |= 64;
     // plain lottery whose conflict shell is the whole radius-1 Moore shell (8 positions in 2D,
     // 26 in 3D): the winner test reads the tickets at compile-time offsets
-    if ((r.config & 64) && (r.config & 16) == 0) {
+    if (d.algorithm == CPM_ALG_LOTTERY && (r.config & 64) && (r.config & 16) == 0) {
         const int want = d.ndim == 2 ? 8 : 26;
         bool moore = (int)(r.conf_off.size() / 3) == want;
         for (size_t q = 0; moore && q < r.conf_off.size(); q += 3) {

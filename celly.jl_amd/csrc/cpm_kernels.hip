@@ -620,7 +620,10 @@ __global__ __launch_bounds__(NT) void cb_propose_kernel(PassArgs a) {
 // plus the one x neighbour outside the 16 bytes), settles interior sites, and compacts
 // boundary sites and then actionable attempts in wave-private LDS lists so that the
 // Philox draws run on full waves. Outputs (claims, segment stretches, counters) are those
-// of cb_propose_kernel with NT = 256.
+// of cb_propose_kernel with NT = 256. An x side of 2 mod 4 (HALF) puts every other row 8
+// bytes into a 16-byte group, and the last group of a row holds two sites: its loads are
+// 8-byte halves, the upper half of the last group is not read, and the x + 1 neighbour of
+// the row's last site is the row's first word.
 // ---------------------------------------------------------------------------
 template <int TX, int TY>
 struct RowGeom {
@@ -642,9 +645,25 @@ struct RowStrip {
     int32_t side;
     uint32_t site0;  // 1-based site of the first of the 16 bytes
     bool valid;
+    bool full;       // all four sites lie in the row (false only for the last group of a HALF row)
 };
 
-template <int RX, int TX, int TY>
+// Four words at p: one 16-byte load, or (HALF: p is 8-byte aligned) two 8-byte loads, the
+// upper one only when the group is whole.
+template <bool HALF>
+__device__ __forceinline__ v4i row_group(const CPM_GLOBAL int32_t *p, bool full) {
+    if constexpr (!HALF) {
+        return *(const CPM_GLOBAL v4i *)p;
+    } else {
+        typedef int32_t v2i __attribute__((ext_vector_type(2)));
+        const v2i lo = *(const CPM_GLOBAL v2i *)p;
+        v2i hi = v2i{0, 0};
+        if (full) hi = *(const CPM_GLOBAL v2i *)(p + 2);
+        return v4i{lo.x, lo.y, hi.x, hi.y};
+    }
+}
+
+template <int RX, int TX, int TY, bool HALF>
 __device__ __forceinline__ RowStrip row_load(const CPM_GLOBAL int32_t *lat, int X, int Y, int x0, int y0, int ry,
                                              int wave, int it) {
     using R = RowGeom<TX, TY>;
@@ -653,8 +672,9 @@ __device__ __forceinline__ RowStrip row_load(const CPM_GLOBAL int32_t *lat, int 
     RowStrip st;
     const int gx = x0 + 4 * (lane % R::LPR);
     const int gy = y0 + wave * R::BAND + 2 * k + ry;
-    // a tile at the upper x or y edge may extend past the lattice (whose X is a multiple of 4)
+    // a tile at the upper x or y edge may extend past the lattice
     st.valid = k < R::CROWS && gx < X && gy < Y;
+    st.full = !HALF || gx + 4 <= X;
     st.mid = st.dn = st.up = v4i{0, 0, 0, 0};
     st.side = 0;
     st.site0 = 1u + (uint32_t)gx + (uint32_t)gy * (uint32_t)X;
@@ -662,11 +682,13 @@ __device__ __forceinline__ RowStrip row_load(const CPM_GLOBAL int32_t *lat, int 
         const int gym = gy == 0 ? Y - 1 : gy - 1;
         const int gyp = gy + 1 == Y ? 0 : gy + 1;
         const uint32_t row = (uint32_t)gy * (uint32_t)X;
-        st.mid = *(const CPM_GLOBAL v4i *)(lat + row + (uint32_t)gx);
-        st.dn = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gym * (uint32_t)X + (uint32_t)gx);
-        st.up = *(const CPM_GLOBAL v4i *)(lat + (uint32_t)gyp * (uint32_t)X + (uint32_t)gx);
-        const int sx = RX == 0 ? (gx == 0 ? X - 1 : gx - 1) : (gx + 4 == X ? 0 : gx + 4);
+        st.mid = row_group<HALF>(lat + row + (uint32_t)gx, st.full);
+        st.dn = row_group<HALF>(lat + (uint32_t)gym * (uint32_t)X + (uint32_t)gx, st.full);
+        st.up = row_group<HALF>(lat + (uint32_t)gyp * (uint32_t)X + (uint32_t)gx, st.full);
+        const int sx = RX == 0 ? (gx == 0 ? X - 1 : gx - 1) : (gx + 4 >= X ? 0 : gx + 4);
         st.side = lat[row + (uint32_t)sx];
+        // the last group of a HALF row: its second site (x = X - 1) has the row's first word at x + 1
+        if (HALF && RX == 1 && !st.full) st.mid.z = st.side;
     }
     return st;
 }
@@ -677,10 +699,11 @@ template <int RX>
 __device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand, uint32_t &nsites,
                                             uint32_t &nsame, uint32_t *s_site, int32_t *s_own,
                                             int32_t (*s_nb)[2 * 64]) {
-    nsites += 2u * (uint32_t)__popcll(__ballot(st.valid));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int p = RX + 2 * h;
+        const bool valid = h == 0 ? st.valid : st.valid && st.full;
+        nsites += (uint32_t)__popcll(__ballot(valid));
         const v4i mid = st.mid, dn = st.dn, up = st.up;
         const int32_t own = p == 0 ? mid.x : p == 1 ? mid.y : p == 2 ? mid.z : mid.w;
         const int32_t left = p == 0 ? st.side : p == 1 ? mid.x : p == 2 ? mid.y : mid.z;
@@ -688,8 +711,8 @@ __device__ __forceinline__ void row_collect(const RowStrip &st, uint32_t &ncand,
         const int32_t below = p == 0 ? dn.x : p == 1 ? dn.y : p == 2 ? dn.z : dn.w;
         const int32_t above = p == 0 ? up.x : p == 1 ? up.y : p == 2 ? up.z : up.w;
         const bool interior = own == left && own == below && own == above && own == right;
-        const bool boundary = st.valid && !interior;
-        nsame += (uint32_t)__popcll(__ballot(st.valid && interior));
+        const bool boundary = valid && !interior;
+        nsame += (uint32_t)__popcll(__ballot(valid && interior));
         const uint64_t m = __ballot(boundary);
         if (boundary) {
             const uint32_t e = ncand + __popcll(m & __lanemask_lt());
@@ -726,7 +749,7 @@ struct RowLds {
 // cells in the LDS table and writes the tile's segment (stretch 0). Pooling over the waves
 // fills the Philox chunks: a strip holds about 40 boundary sites and 13 actionable attempts
 // per wave, so four waves need 3 + 1 wave-wide Philox calls instead of 4 + 4.
-template <int RX, int TX, int TY>
+template <int RX, int TX, int TY, bool HALF>
 __device__ __forceinline__ void row_tile(const PassArgs &a, const CPM_CONST RepDev &d, const TileInfo &ti,
                                          uint32_t tile, uint64_t mcs, int ry, int wave, RowLds &L, uint32_t *s_ckey,
                                          unsigned long long *s_cval, uint32_t &nsites, uint32_t &nsame,
@@ -738,7 +761,7 @@ __device__ __forceinline__ void row_tile(const PassArgs &a, const CPM_CONST RepD
     const uint32_t tid = threadIdx.x;
     RowStrip st[R::ITERS];
 #pragma unroll
-    for (int it = 0; it < R::ITERS; ++it) st[it] = row_load<RX, TX, TY>(lat, X, Y, ti.x0, ti.y0, ry, wave, it);
+    for (int it = 0; it < R::ITERS; ++it) st[it] = row_load<RX, TX, TY, HALF>(lat, X, Y, ti.x0, ti.y0, ry, wave, it);
     __syncthreads();  // the claim table and the first strip's counter are clear (loads in flight)
     unsigned long long *claims = a.claims + (size_t)a.parity * a.total_claims + d.claim_base;
     CPM_GLOBAL v4i *seg = (CPM_GLOBAL v4i *)(a.seg + (size_t)tile * a.seg_cap);
@@ -799,7 +822,7 @@ __device__ __forceinline__ void row_tile(const PassArgs &a, const CPM_CONST RepD
     }
 }
 
-template <int TX, int TY>
+template <int TX, int TY, bool HALF>
 __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     __shared__ RowLds L;
     __shared__ uint32_t s_cnt[kWaves][2];
@@ -826,9 +849,9 @@ __global__ __launch_bounds__(kBlock) void cb_propose_rows_kernel(PassArgs a) {
     const int wave = (int)(threadIdx.x >> 6);
     uint32_t nsites = 0, nsame = 0, nact = 0;
     if (rx == 0)
-        row_tile<0, TX, TY>(a, d, ti, tile, mcs, ry, wave, L, s_ckey, s_cval, nsites, nsame, nact);
+        row_tile<0, TX, TY, HALF>(a, d, ti, tile, mcs, ry, wave, L, s_ckey, s_cval, nsites, nsame, nact);
     else
-        row_tile<1, TX, TY>(a, d, ti, tile, mcs, ry, wave, L, s_ckey, s_cval, nsites, nsame, nact);
+        row_tile<1, TX, TY, HALF>(a, d, ti, tile, mcs, ry, wave, L, s_ckey, s_cval, nsites, nsame, nact);
     if (__lane_id() == 0) {
         s_cnt[wave][0] = nsites;
         s_cnt[wave][1] = nsame;
@@ -2794,9 +2817,10 @@ static hipError_t launch_commit(const PassArgs &a, hipStream_t stream) {
 
 // Tile configurations: id & 15 -> (dim, halo, TX, TY, TZ); bit 4 of id selects the kernels
 // for proposal relations whose direction count differs from 2 * dim; bit 5 the row kernel
-// of 2D periodic von Neumann checkerboards with a 2 x 2 colour pattern; bit 6 the lottery
-// kernel for periodic obstacle-free lattices of at least 2 halo + 1 sites per axis; bit 7,
-// with bit 6, its winner test over the radius-1 Moore conflict shell.
+// of 2D periodic von Neumann checkerboards with a 2 x 2 colour pattern (with bit 6: an x
+// side of 2 mod 4, rows of 8-byte halves); bit 6 without bit 5 the lottery kernel for
+// periodic obstacle-free lattices of at least 2 halo + 1 sites per axis; bit 7, with bit 6,
+// its winner test over the radius-1 Moore conflict shell.
 extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, int *ty, int *tz) {
     if (id < 0 || id > 255) return 0;
     id &= 15;
@@ -2819,22 +2843,33 @@ extern "C" int cpm_internal_tile_config(int id, int *dim, int *halo, int *tx, in
     return 1;
 }
 
+namespace cpm {
+template <int TX, int TY>
+static void launch_rows(bool half, const PassArgs *a, hipStream_t stream) {
+    if (half)
+        CPM_LAUNCH((cb_propose_rows_kernel<TX, TY, true>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a);
+    else
+        CPM_LAUNCH((cb_propose_rows_kernel<TX, TY, false>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a);
+}
+}  // namespace cpm
+
 extern "C" hipError_t cpm_internal_launch_propose(int config, int algorithm, const cpm::PassArgs *a,
                                                   hipStream_t stream) {
     using namespace cpm;
     if (config < 0 || config > 255) return hipErrorInvalidValue;
     if (config & 32) {
         // row kernels: checkerboard, periodic von Neumann proposal, 2 x 2 (x 2) colour pattern
-        if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16)) return hipErrorInvalidValue;
+        if (algorithm != CPM_ALG_CHECKERBOARD || (config & 16) || (config & 128)) return hipErrorInvalidValue;
+        const bool half = (config & 64) != 0;  // 2D rows of 8-byte halves (x side 2 mod 4)
         switch (config & 15) {
         case 0:
-        case 1: CPM_LAUNCH((cb_propose_rows_kernel<128, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 4: CPM_LAUNCH((cb_propose_rows_kernel<64, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 5: CPM_LAUNCH((cb_propose_rows_kernel<256, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 6: CPM_LAUNCH((cb_propose_rows_kernel<64, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 7: CPM_LAUNCH((cb_propose_rows_kernel<32, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 12: CPM_LAUNCH((cb_propose_rows_kernel<256, 16>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
-        case 13: CPM_LAUNCH((cb_propose_rows_kernel<128, 32>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
+        case 1: launch_rows<128, 16>(half, a, stream); break;
+        case 4: launch_rows<64, 32>(half, a, stream); break;
+        case 5: launch_rows<256, 8>(half, a, stream); break;
+        case 6: launch_rows<64, 16>(half, a, stream); break;
+        case 7: launch_rows<32, 32>(half, a, stream); break;
+        case 12: launch_rows<256, 16>(half, a, stream); break;
+        case 13: launch_rows<128, 32>(half, a, stream); break;
         // 3D: 2 x 2 x 2 colour pattern, six von Neumann directions
         case 2: CPM_LAUNCH((cb_propose_rows3_kernel<32, 8, 8>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;
         case 8: CPM_LAUNCH((cb_propose_rows3_kernel<64, 8, 4>), dim3(a->ntiles), dim3(kBlock), 0, stream, *a); break;

@@ -240,8 +240,9 @@ int32_t cpm_set_tiled_staging(int32_t enable);
  * (checkerboard) or the halo colour of its tile (tiled). Not for lottery. */
 int32_t cpm_site_colors(const cpm_ctx *ctx, uint8_t *colors);
 /* Kernel configuration of the context's lattice: tile shape id (bits 0-3), kernels for
- * proposal relations of other than 2 * ndim directions (bit 4), row propose kernel (bit 5),
- * lottery kernel for periodic obstacle-free lattices (bit 6), 256 for TiledCheckerboardCPM;
+ * proposal relations of other than 2 * ndim directions (bit 4), row propose kernel (bit 5;
+ * with bit 6: an x side of 2 mod 4), lottery kernel for periodic obstacle-free lattices
+ * (bit 6 without bit 5), 256 for TiledCheckerboardCPM;
  * bits 8-11: the commit kernel's specialised evaluation (0: general; 1-3 2D with 0 / 4 / 8
  * contact directions, 4-6 3D with 0 / 6 / 26).
  * (No counterpart in the reference: kernel selection of this engine.) */

@@ -89,6 +89,23 @@ def test_2d_row_kernels(gpu_lib, shape, rows):
         set_row_kernels(True)
 
 
+@pytest.mark.parametrize("shape", ROW_SHAPES)
+def test_2d_row_kernels_half_rows(gpu_lib, shape):
+    """x sides of 2 mod 4: every other row starts 8 bytes into a 16-byte group and each row
+    ends in a group of two sites, whose last site wraps to the row's first word (the row
+    kernel's variant of 8-byte halves, kernel configuration bits 5 and 6)."""
+    set_tile_shape(ALG_CHECKERBOARD, 2, 1, shape)
+    try:
+        for model in (workloads.scaled_2d(130, seed=9003), _rectangle(66, 40, seed=9004),
+                      _rectangle(22, 26, seed=9005)):
+            gpu = init_scientific(*model[:4], seed=model[4])
+            assert gpu.kernel_config() & 96 == 96
+            gpu.close()
+            _parity(model, 5, True)
+    finally:
+        set_tile_shape(ALG_CHECKERBOARD, 2, 1, -1)
+
+
 @pytest.mark.parametrize("shape", [s for s, _ in tile_shapes(3, 1)])
 @pytest.mark.parametrize("alg", ["checkerboard", "lottery"])
 def test_3d_shapes(gpu_lib, shape, alg):
