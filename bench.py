@@ -128,14 +128,16 @@ def batched_leg(replicas: int, mcs: int, graph_chunk: int, plain: bool = True, t
     sites = [int(m[0].tags.size) for m in models]
     log(f"batched: built in {time.perf_counter() - t0:.1f} s, {sum(sites) / 1e6:.1f} M sites")
 
-    # tile shape per size class, tuned on other draws of the same ensemble distribution
+    # tile shape, tuned on as many other draws of the same ensemble distribution (a smaller
+    # sample weighs the fixed launch costs more and prefers larger tiles)
     t0 = time.perf_counter()
     shapes = BatchShapes()
     if shape >= 0:
         from corepotts.autotune import SIZE_CLASS_BOUNDS
         shapes.shapes = {(ALG_CHECKERBOARD, 2, 1, c): shape for c in range(len(SIZE_CLASS_BOUNDS) + 1)}
     elif tune:
-        shapes.tune([workloads.ensemble_replica(replicas + i) for i in range(128)], per_class=16, mcs=10)
+        shapes.tune([workloads.ensemble_replica(replicas + i) for i in range(replicas)], per_class=replicas,
+                    mcs=10, greedy=False)
     tune_s = time.perf_counter() - t0
     log(f"batched: shapes tuned in {tune_s:.1f} s: {shapes.describe()}")
 

@@ -133,9 +133,11 @@ class BatchShapes:
         finally:
             batch.close()
 
-    def tune(self, models, per_class: int = 16, mcs: int = 10) -> Dict[tuple, int]:
+    def tune(self, models, per_class: int = 16, mcs: int = 10, greedy: bool = True) -> Dict[tuple, int]:
         """models: init_scientific argument tuples of one algorithm; the first per_class
-        replicas of each class form the timed sample."""
+        replicas of each class form the timed sample. greedy=False keeps the best shared shape
+        (no class-by-class step): for a sample as large as the batch it serves, whose fixed
+        launch costs weigh as they will there."""
         classes: Dict[tuple, list] = {}
         for m in models:
             spec = build_spec(*m)
@@ -155,7 +157,7 @@ class BatchShapes:
             for k in members:
                 self.shapes[k] = min(times, key=times.get)
             self.uniform_timings[fam] = times
-        for k in sorted(classes, key=lambda k: -k[3]):
+        for k in sorted(classes, key=lambda k: -k[3]) if greedy else ():
             times = {}
             for shape, _tile in tile_shapes(k[1], k[2]):
                 self.shapes[k] = shape

@@ -206,6 +206,9 @@ def test_batch_shapes_per_size_class_keep_results(gpu_lib):
     # every shape was also timed on all classes at once
     assert set(shapes.uniform_timings) == {(ALG_CHECKERBOARD, 2, 1)}
     assert set(shapes.uniform_timings[(ALG_CHECKERBOARD, 2, 1)]) == set(dict(tile_shapes(2, 1)))
+    shared = BatchShapes()
+    shared.tune(models, per_class=2, mcs=2, greedy=False)
+    assert len(set(shared.shapes.values())) == 1 and len(shared.shapes) == 3 and not shared.timings
     # force three different shapes so the batch has one launch group per class
     shapes.shapes = {k: s for k, s in zip(sorted(tuned), [7, 0, 12])}
     batch = ReplicaBatch(ALG_CHECKERBOARD, graph_chunk=3, shapes=shapes)
