@@ -41,9 +41,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(passes: int):
-    """Measured HBM bytes of one MCS (passes x (propose + commit) + colour-order kernel) from the
-    committed counter summary, if present."""
+def traffic_from_profiles(passes: int, chunk_mcs: int):
+    """Measured HBM bytes of one MCS (passes x (propose + commit), plus the plan and end launches
+    of a chunk of chunk_mcs MCS shared over its MCS) from the committed counter summary, if
+    present."""
     path = os.path.join(ROOT, "profiles", "cb_4096_traffic.json")
     if not os.path.exists(path):
         return None
@@ -51,7 +52,7 @@ def traffic_from_profiles(passes: int):
         t = json.load(handle)
     try:
         return (passes * (t["propose"]["bytes_per_launch"] + t["commit"]["bytes_per_launch"])
-                + t["begin_mcs"]["bytes_per_launch"])
+                + (t["plan"]["bytes_per_launch"] + t["plan_end"]["bytes_per_launch"]) / max(chunk_mcs, 1))
     except KeyError:
         return None
 
@@ -407,7 +408,7 @@ def headline_leg(args):
     mcs_bytes = 4.0 * passes * n_sites + rep_t.accepted_copies * (4 + 16 + 32)
     mcs_per_s = args.steps / elapsed
     achieved = mcs_bytes * mcs_per_s / 1e9
-    traffic = traffic_from_profiles(passes)
+    traffic = traffic_from_profiles(passes, chunk if chunk > 0 else 1)
     propose_gbps = 4.0 * n_sites / (propose_avg_ms * 1e-3) / 1e9
 
     extra = {
